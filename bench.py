@@ -1,0 +1,203 @@
+"""Benchmark of the X-engine HERK on one MI355X.
+
+Headline workload (one "step" = one plain HERK call over all channels):
+2048 dual-pol antennas (4096 inputs), 96 channels, K = 2048 time samples;
+int4 complex voltages uniform in [-7, 7] (synthetic, seeded), promoted to OCP
+FP8 E4M3 inside the kernel, FP32 accumulate, packed lower-triangle complex FP32
+output.  Metric: 4M-equivalent TFLOP/s = 8 N^2 K batch / t (the reference's
+convention, herk_vs_tcc.sh:141, cutlass_interface/test_herk_int4.cu:97) and its
+fraction of the MI355X FP8 dense MFMA peak.
+
+Prints ONE JSON line.  The CPU baseline is the oracle (C, OpenMP, int64
+accumulation) timed on a bounded slice of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FP8_DENSE_PEAK_TFLOPS = 5033.0   # 256 CU x 2.4 GHz x 8192 FLOP/CU/clk (AMD published, dense)
+HBM_PEAK_GBS = 8000.0
+METRIC = "HERK TFLOP/s (4M-equiv) & % FP8 MFMA peak, 4096 inputs; batched vs plain"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def synth_qc(torch, n: int, seed: int, device) -> "torch.Tensor":
+    """QC bytes with Re, Im i.i.d. uniform integers in [-7, 7] (sign-magnitude nibbles)."""
+    g = torch.Generator(device=device).manual_seed(seed)
+    v = torch.randint(-7, 8, (2, n), generator=g, device=device, dtype=torch.int16)
+    nib = torch.where(v < 0, 8 + (-v), v).to(torch.uint8)
+    return (nib[0] << 4) | nib[1]
+
+
+def cpu_baseline(N: int, K: int, seed: int, target_s: float) -> dict:
+    """Oracle HERK (int64, OpenMP) on rows of one channel; rate in 4M-equiv TFLOP/s."""
+    import numpy as np
+    from oracle import oracle
+    threads = len(os.sched_getaffinity(0))
+    env_t = os.environ.get("OMP_NUM_THREADS")
+    if env_t and env_t.isdigit():
+        threads = min(threads, int(env_t))
+    A1 = oracle.gen_qc_uniform(seed, N * K)
+    C1 = np.zeros((1, N * (N + 1) // 2, 2), dtype=np.int64)
+    tri_macs = K * N * (N + 1) / 2  # complex MACs of one channel's lower triangle
+
+    # calibrate on the heaviest rows of one channel, doubling until >= 0.5 s
+    cal_rows = 32
+    while True:
+        t0 = time.perf_counter()
+        oracle.herk_int64_rows(A1, N, K, 1, N - cal_rows, N, threads=threads, out=C1)
+        t_cal = time.perf_counter() - t0
+        if t_cal >= 0.5 or cal_rows >= N:
+            break
+        cal_rows = min(N, cal_rows * 4)
+    rate = sum(i + 1 for i in range(N - cal_rows, N)) * K / max(t_cal, 1e-6)
+    t_channel = tri_macs / rate
+    if t_channel >= target_s:
+        # part of one channel: the heaviest rows first, until ~target_s of work
+        want, r0, macs = rate * target_s, N, 0
+        while r0 > 0 and macs < want:
+            r0 -= 1
+            macs += (r0 + 1) * K
+        t0 = time.perf_counter()
+        oracle.herk_int64_rows(A1, N, K, 1, r0, N, threads=threads, out=C1)
+        t = time.perf_counter() - t0
+        channels = macs / tri_macs
+        sample = f"rows {r0}..{N - 1} of one channel ({channels * 100:.1f}% of its lower triangle)"
+    else:
+        nch = max(1, min(96, int(round(target_s / t_channel))))
+        A = oracle.gen_qc_uniform(seed, N * K * nch)
+        C = np.zeros((nch, N * (N + 1) // 2, 2), dtype=np.int64)
+        t0 = time.perf_counter()
+        oracle.herk_int64_rows(A, N, K, nch, 0, N, threads=threads, out=C)
+        t = time.perf_counter() - t0
+        channels = float(nch)
+        sample = f"{nch} full channel(s)"
+    flops_equiv = 8.0 * N * N * K * channels
+    return {
+        "value": flops_equiv / t / 1e12,
+        "unit": "TFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle int64 HERK (OpenMP, {threads} threads), {sample} of the same "
+                   f"workload (N={N}, K={K}), {t:.1f} s; 4M-equiv FLOP = 8 N^2 K per channel"),
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--N", type=int, default=4096)
+    ap.add_argument("--K", type=int, default=2048)
+    ap.add_argument("--batch", type=int, default=96)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "channels shard across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    import torch
+    from ftd_amd import ftdx
+
+    N, K, B = args.N, args.K, args.batch
+    dev = torch.device("cuda:0")
+    log(f"bench: plain HERK N={N} K={K} batch={B} steps={args.steps} warmup={args.warmup}")
+    A = synth_qc(torch, N * K * B, args.seed, dev)
+    C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        ftdx.herk_int4(A, N, K, B, C=C, stream=stream)
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        starts[s].record(stream)
+        ftdx.herk_int4(A, N, K, B, C=C, stream=stream)
+        ends[s].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = sorted(starts[s].elapsed_time(ends[s]) for s in range(args.steps))
+    ms_total = sum(per)
+    ms_per_step = wall * 1e3 / args.steps
+    kern_ms = ms_total / args.steps
+
+    flop = 8.0 * N * N * K * B
+    value = flop / (ms_per_step * 1e-3) / 1e12
+    achieved = flop / (kern_ms * 1e-3) / 1e12
+    log(f"bench: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms "
+        f"(median {per[len(per) // 2]:.3f}), {value:.1f} TFLOP/s 4M-equiv")
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle) ...")
+        cpu = cpu_baseline(N, K, args.seed, args.cpu_seconds)
+        log(f"bench: CPU {cpu['value']:.4f} TFLOP/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp8_e4m3 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain HERK, int4 QC voltages -> FP8 E4M3 MFMA, packed lower-triangle FP32",
+            "inputs": N, "channels": B, "K": K, "seed": args.seed,
+            "parallelism": "single GPU",
+        },
+        "pct_fp8_peak": round(100.0 * value / FP8_DENSE_PEAK_TFLOPS, 2),
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(achieved, 2),
+            "peak": FP8_DENSE_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP8_DENSE_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "herk_int4_fp8_tile_kernel",
+            "kernel_ms": round(kern_ms, 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

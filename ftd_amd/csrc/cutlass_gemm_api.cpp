@@ -1,0 +1,80 @@
+// cutlass_gemm_api.cpp -- C++ drop-in CutlassComplexGemm over the ftdx C ABI.
+// Host code only (built with the host compiler); every call goes through ftdx.h.
+#include "cutlass_gemm_api.h"
+
+#include <stdexcept>
+#include <string>
+
+namespace cutlass_gemm_api {
+
+namespace {
+
+// Reference behaviour: unsupported combinations and device errors throw
+// (cutlass_interface/cutlass_gemm_api.cu:1716-1723, shared/config_common.hpp:35-43).
+int check(int rc, const char* what) {
+    if (rc == FTDX_OK) return 0;
+    if (rc == FTDX_ERR_UNSUPPORTED || rc == FTDX_ERR_HIP || rc == FTDX_ERR_NO_MEMORY)
+        throw std::runtime_error(std::string(what) + ": " + ftdx_last_error());
+    return rc;
+}
+
+void check_void(int rc, const char* what) {
+    if (rc != FTDX_OK) throw std::runtime_error(std::string(what) + ": " + ftdx_last_error());
+}
+
+}  // namespace
+
+CutlassComplexGemm::CutlassComplexGemm() : ctx_(nullptr) {
+    check_void(ftdx_create(&ctx_), "CutlassComplexGemm");
+}
+
+CutlassComplexGemm::~CutlassComplexGemm() { ftdx_destroy(ctx_); }
+
+void CutlassComplexGemm::set_herk_mode(HerkMode mode) {
+    check_void(ftdx_set_herk_mode(ctx_, static_cast<int>(mode)), "set_herk_mode");
+}
+
+void CutlassComplexGemm::set_direct_tile_config(DirectTileConfig config) {
+    check_void(ftdx_set_direct_tile_config(ctx_, static_cast<int>(config)),
+               "set_direct_tile_config");
+}
+
+void CutlassComplexGemm::set_direct_herk_graph(bool enable) {
+    check_void(ftdx_set_direct_herk_graph(ctx_, enable ? 1 : 0), "set_direct_herk_graph");
+}
+
+void CutlassComplexGemm::set_tune_cache_path(const std::string& path) {
+    check_void(ftdx_set_tune_cache_path(ctx_, path.c_str()), "set_tune_cache_path");
+}
+
+void CutlassComplexGemm::set_strategy_tune_verbosity(int level) {
+    check_void(ftdx_set_strategy_tune_verbosity(ctx_, level), "set_strategy_tune_verbosity");
+}
+
+void CutlassComplexGemm::set_kernel_tune_verbosity(int level) {
+    check_void(ftdx_set_kernel_tune_verbosity(ctx_, level), "set_kernel_tune_verbosity");
+}
+
+void CutlassComplexGemm::set_kernel_tune_cache_path(const std::string& path) {
+    check_void(ftdx_set_kernel_tune_cache_path(ctx_, path.c_str()), "set_kernel_tune_cache_path");
+}
+
+void CutlassComplexGemm::init_herk(int N, int K, int batch_count, hipStream_t stream) {
+    check_void(ftdx_init_herk(ctx_, N, K, batch_count, stream), "init_herk");
+}
+
+void CutlassComplexGemm::end_herk() { ftdx_end_herk(ctx_); }
+
+int CutlassComplexGemm::herk(const void* A, void* C, int N, int K, int batch_count,
+                             InputPrecision input, ComputePrecision compute,
+                             OutputPrecision output, OutputFormat format, float alpha,
+                             float beta, hipStream_t stream, bool tune) {
+    return check(ftdx_herk(ctx_, A, C, N, K, batch_count, static_cast<int>(input),
+                           static_cast<int>(compute), static_cast<int>(output),
+                           static_cast<int>(format), alpha, beta, stream, tune ? 1 : 0),
+                 "herk");
+}
+
+void CutlassComplexGemm::print_build_info() { ftdx_print_build_info(); }
+
+}  // namespace cutlass_gemm_api

@@ -1,0 +1,66 @@
+// herk_common.h -- device helpers shared by the gfx950 HERK kernels.
+//
+// Data conventions (cutlass_gemm_api side of the reference):
+//   * input: one "QC" byte per complex sample, high nibble = Re, low nibble = Im,
+//     each sign-magnitude (bit 3 sign, bits 2:0 magnitude), layout [batch][N][K]
+//     (cutlass_interface/cutlass_gemm_api.h:185-187, :727-728);
+//   * output: packed lower triangle, tri(i,j) = i(i+1)/2 + j, interleaved [re, im],
+//     batch stride N(N+1) scalars (cutlass_interface/shared/herk_kernel_common.hpp:353-424).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ftdx {
+
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// 4 QC bytes (4 complex samples) -> 8 OCP E4M3 bytes [re0, im0, re1, im1 | re2, im2, re3, im3].
+// The magnitude table {0,1,..,7} -> {0x00,0x38,0x40,0x44,0x48,0x4A,0x4C,0x4E} is two
+// dwords indexed by v_perm_b32; the sign bit of each nibble moves to bit 7 of its byte.
+// A "-0" nibble becomes 0x80 (OCP E4M3 negative zero, a valid zero on gfx950).
+__device__ __forceinline__ void qc4_to_fp8x8(uint32_t x, uint32_t& o0, uint32_t& o1) {
+    const uint32_t TLO = 0x44403800u;  // fp8(0..3) in bytes 0..3
+    const uint32_t THI = 0x4E4C4A48u;  // fp8(4..7) in bytes 4..7
+    uint32_t m_im = x & 0x07070707u;
+    uint32_t m_re = (x >> 4) & 0x07070707u;
+    uint32_t f_im = __builtin_amdgcn_perm(THI, TLO, m_im);
+    uint32_t f_re = __builtin_amdgcn_perm(THI, TLO, m_re);
+    f_im |= (x << 4) & 0x80808080u;
+    f_re |= x & 0x80808080u;
+    // bytes 0..3 of the 8-byte pool are f_re, bytes 4..7 are f_im
+    o0 = __builtin_amdgcn_perm(f_im, f_re, 0x05010400u);
+    o1 = __builtin_amdgcn_perm(f_im, f_re, 0x07030602u);
+}
+
+// [re, im, re', im'] (fp8) -> [im, -re, im', -re'].  Row vector v_i with
+// v_i . u_j = Im(a_i conj(a_j)) when u_j is the plain interleaved row.
+__device__ __forceinline__ uint32_t conj_swap_fp8(uint32_t v) {
+    return __builtin_amdgcn_perm(v, v, 0x02030001u) ^ 0x80008000u;
+}
+
+__device__ __forceinline__ v8i conj_swap_fp8x32(v8i a) {
+    v8i r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r[i] = (int)conj_swap_fp8((uint32_t)a[i]);
+    return r;
+}
+
+// D = A(32x64 fp8) * B(64x32 fp8) + C, FP32 accumulate (plain, unscaled form).
+__device__ __forceinline__ v16f mfma_32x32x64_fp8(v8i a, v8i b, v16f c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
+}
+
+// lower-triangle tile index t -> (bi, bj), bj <= bi, t = bi(bi+1)/2 + bj
+__device__ __forceinline__ void tri_decode(int t, int& bi, int& bj) {
+    int r = (int)((__builtin_sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while (r * (r + 1) / 2 > t) --r;
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    bi = r;
+    bj = t - r * (r + 1) / 2;
+}
+
+__device__ __forceinline__ int64_t tri_index(int64_t i, int64_t j) { return i * (i + 1) / 2 + j; }
+
+}  // namespace ftdx

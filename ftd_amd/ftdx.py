@@ -1,0 +1,303 @@
+"""Python host side of the X-engine HERK library: a ctypes binding of the C ABI
+in ``include/ftdx.h`` (``libftdx.so``) plus a Python mirror of the reference's
+``cutlass_gemm_api::CutlassComplexGemm`` HERK surface
+(cutlass_interface/cutlass_gemm_api.h:157-789) and GGP glue
+(lib/cutlass_interface.cu:150-289).
+
+Device memory and streams come from PyTorch (ROCm); all arithmetic runs in the
+HIP kernels of libftdx.so.  There is no CPU fallback: if the library is not
+built, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+import re
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libftdx.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "ftdx.h")
+
+FTDX_OK = 0
+FTDX_ERR_INVALID_ARG = 1
+FTDX_ERR_UNSUPPORTED = 2
+FTDX_ERR_HIP = 3
+FTDX_ERR_NO_MEMORY = 4
+FTDX_ERR_IO = 5
+
+
+class InputPrecision(enum.IntEnum):
+    FP16 = 0
+    INT4 = 1
+
+
+class ComputePrecision(enum.IntEnum):
+    FP8 = 0
+    FP6 = 1
+    FP4 = 2
+
+
+class OutputPrecision(enum.IntEnum):
+    FP16 = 0
+    FP32 = 1
+
+
+class OutputFormat(enum.IntEnum):
+    PackedTriangle = 0
+
+
+class HerkMode(enum.IntEnum):
+    Auto = 0
+    ForceDirect = 1
+    ForceBaseline = 2
+
+
+class DirectTileConfig(enum.IntEnum):
+    Auto = 0
+    K32_B3 = 1
+    K64_B2 = 2
+    K64_B3 = 3
+    K128_B2 = 4
+
+
+class FtdxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ftdx error {code}: {msg}")
+        self.code = code
+
+
+class UnsupportedError(FtdxError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_S = ctypes.c_char_p
+
+# name -> (restype, argtypes); must cover every function declared in ftdx.h
+_SIGNATURES = {
+    "ftdx_last_error": (ctypes.c_char_p, []),
+    "ftdx_abi_version": (_I, []),
+    "ftdx_create": (_I, [ctypes.POINTER(_VP)]),
+    "ftdx_destroy": (_I, [_VP]),
+    "ftdx_herk": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _I, _F, _F, _VP, _I]),
+    "ftdx_init_herk": (_I, [_VP, _I, _I, _I, _VP]),
+    "ftdx_end_herk": (_I, [_VP]),
+    "ftdx_set_herk_mode": (_I, [_VP, _I]),
+    "ftdx_set_direct_tile_config": (_I, [_VP, _I]),
+    "ftdx_set_direct_herk_graph": (_I, [_VP, _I]),
+    "ftdx_set_tune_cache_path": (_I, [_VP, _S]),
+    "ftdx_set_strategy_tune_verbosity": (_I, [_VP, _I]),
+    "ftdx_set_kernel_tune_verbosity": (_I, [_VP, _I]),
+    "ftdx_set_kernel_tune_cache_path": (_I, [_VP, _S]),
+    "ftdx_print_build_info": (_I, []),
+    "ftdx_herk_int4": (_I, [_VP, _VP, _I, _I, _I, _I, _F, _F, _VP]),
+}
+
+
+def declared_functions(header: str = HEADER_PATH) -> list[str]:
+    """Names of the functions declared in include/ftdx.h."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ftdx_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """Load libftdx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise RuntimeError(
+                        f"libftdx.so not found at {LIB_PATH}: build it first "
+                        "(python -c 'import __graft_entry__ as g; g.build()')")
+                L = ctypes.CDLL(LIB_PATH)
+                for name, (res, args) in _SIGNATURES.items():
+                    fn = getattr(L, name)
+                    fn.restype = res
+                    fn.argtypes = args
+                _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().ftdx_last_error().decode(errors="replace")
+
+
+def _check(rc: int, what: str) -> None:
+    if rc == FTDX_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if rc == FTDX_ERR_UNSUPPORTED:
+        raise UnsupportedError(rc, msg)
+    raise FtdxError(rc, msg)
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream or None
+    if isinstance(stream, int):
+        return stream or None
+    return stream.cuda_stream or None
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("ftdx: tensors must be device (HIP) tensors")
+    if not t.is_contiguous():
+        raise ValueError("ftdx: tensors must be contiguous")
+    return t.data_ptr()
+
+
+def packed_len(N: int) -> int:
+    return N * (N + 1) // 2
+
+
+class CutlassComplexGemm:
+    """Mirror of cutlass_gemm_api::CutlassComplexGemm's HERK surface."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().ftdx_create(ctypes.byref(h)), "ftdx_create")
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ftdx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def set_herk_mode(self, mode: HerkMode) -> None:
+        _check(lib().ftdx_set_herk_mode(self._h, int(mode)), "set_herk_mode")
+
+    def set_direct_tile_config(self, cfg: DirectTileConfig) -> None:
+        _check(lib().ftdx_set_direct_tile_config(self._h, int(cfg)), "set_direct_tile_config")
+
+    def set_direct_herk_graph(self, enable: bool) -> None:
+        _check(lib().ftdx_set_direct_herk_graph(self._h, int(bool(enable))),
+               "set_direct_herk_graph")
+
+    def set_tune_cache_path(self, path: str) -> None:
+        _check(lib().ftdx_set_tune_cache_path(self._h, path.encode()), "set_tune_cache_path")
+
+    def set_strategy_tune_verbosity(self, level: int) -> None:
+        _check(lib().ftdx_set_strategy_tune_verbosity(self._h, level),
+               "set_strategy_tune_verbosity")
+
+    def set_kernel_tune_verbosity(self, level: int) -> None:
+        _check(lib().ftdx_set_kernel_tune_verbosity(self._h, level), "set_kernel_tune_verbosity")
+
+    def set_kernel_tune_cache_path(self, path: str) -> None:
+        _check(lib().ftdx_set_kernel_tune_cache_path(self._h, path.encode()),
+               "set_kernel_tune_cache_path")
+
+    def init_herk(self, N: int, K: int, batch_count: int, stream=None) -> None:
+        _check(lib().ftdx_init_herk(self._h, N, K, batch_count, _stream_ptr(stream)), "init_herk")
+
+    def end_herk(self) -> None:
+        _check(lib().ftdx_end_herk(self._h), "end_herk")
+
+    def herk(self, A, C, N: int, K: int, batch_count: int,
+             input: InputPrecision = InputPrecision.FP16,
+             compute: ComputePrecision = ComputePrecision.FP8,
+             output: OutputPrecision = OutputPrecision.FP16,
+             format: OutputFormat = OutputFormat.PackedTriangle,
+             alpha: float = 1.0, beta: float = 0.0, stream=None, tune: bool = True) -> int:
+        """C = alpha * A A^H + beta * C (batched), packed lower triangle.
+
+        Raises UnsupportedError for unsupported precision combinations (as the
+        reference throws); returns the non-zero status for invalid arguments.
+        """
+        rc = lib().ftdx_herk(self._h, _ptr(A), _ptr(C), N, K, batch_count, int(input),
+                             int(compute), int(output), int(format), alpha, beta,
+                             _stream_ptr(stream), int(bool(tune)))
+        if rc in (FTDX_ERR_UNSUPPORTED, FTDX_ERR_HIP, FTDX_ERR_NO_MEMORY):
+            _check(rc, "herk")
+        return rc
+
+    @staticmethod
+    def print_build_info() -> None:
+        lib().ftdx_print_build_info()
+
+
+def herk_int4(A, N: int, K: int, batch: int, C=None, output: str = "fp32",
+              alpha: float = 1.0, beta: float = 0.0, stream=None):
+    """Plain batched HERK of QC bytes A [batch, N, K] (uint8, device).
+
+    Returns C [batch, N(N+1)/2, 2] (float32 or float16) on the same device.
+    """
+    import torch
+    if A.dtype != torch.uint8:
+        raise ValueError("herk_int4: A must be uint8 QC bytes")
+    if A.numel() != N * K * batch:
+        raise ValueError("herk_int4: A has the wrong number of elements")
+    dt = torch.float32 if output == "fp32" else torch.float16
+    if C is None:
+        C = torch.empty((batch, packed_len(N), 2), dtype=dt, device=A.device)
+        if beta != 0.0:
+            raise ValueError("herk_int4: beta != 0 needs an input C")
+    elif C.dtype != dt or C.numel() != batch * packed_len(N) * 2:
+        raise ValueError("herk_int4: C has the wrong dtype or size")
+    out = int(OutputPrecision.FP32 if output == "fp32" else OutputPrecision.FP16)
+    _check(lib().ftdx_herk_int4(_ptr(A), _ptr(C), N, K, batch, out, alpha, beta,
+                                _stream_ptr(stream)), "ftdx_herk_int4")
+    return C
+
+
+# ---- GGP glue (lib/cutlass_interface.cu:150-289) ----
+_ggp_api: CutlassComplexGemm | None = None
+_ggp_tune = False
+
+
+def _ggp() -> CutlassComplexGemm:
+    global _ggp_api
+    if _ggp_api is None:
+        _ggp_api = CutlassComplexGemm()
+    return _ggp_api
+
+
+def initCutlassApi(kernel_tune_verbosity: int, kernel_tune_cache_path: str | None,
+                   strategy_tune_cache_path: str | None, herk_tune: bool,
+                   strategy_tune_verbosity: int = 1) -> None:
+    global _ggp_tune
+    a = _ggp()
+    if kernel_tune_verbosity > 0:
+        a.set_kernel_tune_verbosity(kernel_tune_verbosity)
+    a.set_strategy_tune_verbosity(strategy_tune_verbosity)
+    if kernel_tune_cache_path:
+        a.set_kernel_tune_cache_path(kernel_tune_cache_path)
+    if strategy_tune_cache_path:
+        a.set_tune_cache_path(strategy_tune_cache_path)
+    _ggp_tune = bool(herk_tune)
+
+
+def herkBatchedCutlassQC(qc_data, tri_output, N: int, K: int, batch: int, stream=None) -> int:
+    return _ggp().herk(qc_data, tri_output, N, K, batch, InputPrecision.INT4,
+                       ComputePrecision.FP8, OutputPrecision.FP32, OutputFormat.PackedTriangle,
+                       1.0, 0.0, stream, _ggp_tune)
+
+
+def herkBatchedCutlassQC_FP16(qc_data, tri_output, N: int, K: int, batch: int,
+                              stream=None) -> int:
+    return _ggp().herk(qc_data, tri_output, N, K, batch, InputPrecision.INT4,
+                       ComputePrecision.FP8, OutputPrecision.FP16, OutputFormat.PackedTriangle,
+                       1.0, 0.0, stream, _ggp_tune)

@@ -1,0 +1,285 @@
+/*
+ * herk_oracle.c -- CPU oracle for the X-engine HERK.  TEST INFRASTRUCTURE ONLY:
+ * loaded by tests/, the smoke check and bench.py's CPU-baseline leg as the
+ * checker; never linked into or called by the product library.
+ *
+ * Restates, in plain C:
+ *   encode_int4           cutlass_interface/test_herk_int4.cu:39-45
+ *   decode_qc_byte        tests/visibility_pipeline_test.cpp:43-53
+ *   qc_to_fp8_lut_256     cutlass_interface/cutlass_gemm_api.cu:1436-1536
+ *   cpu_herk              cutlass_interface/test_herk_int4.cu:51-70 (int64 instead
+ *                         of complex<double>: exact, same values)
+ *   cpu_herk_ab           cutlass_interface/test_gemm_api.cu:726-751
+ *   FP32/FP16 epilogue    cutlass_interface/shared/herk_kernel_common.hpp:405-420
+ * Parallelism follows the reference's OpenMP "parallel for schedule(dynamic)"
+ * over (batch, row) (test_herk_int4.cu:56, :323).
+ *
+ * Build: -O3 -fopenmp -ffp-contract=off (the epilogue's fma is explicit).
+ */
+#include "herk_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ */
+/* mt19937_64 (Matsumoto & Nishimura 2000; parameters as in C++11 std::mt19937_64) */
+#define MT_N 312
+#define MT_M 156
+#define MT_UM 0xFFFFFFFF80000000ULL
+#define MT_LM 0x000000007FFFFFFFULL
+
+void ftdo_mt64_seed(ftdo_mt64* s, uint64_t seed) {
+    s->mt[0] = seed;
+    for (int i = 1; i < MT_N; ++i)
+        s->mt[i] = 6364136223846793005ULL * (s->mt[i - 1] ^ (s->mt[i - 1] >> 62)) + (uint64_t)i;
+    s->idx = MT_N;
+}
+
+static void mt64_twist(ftdo_mt64* s) {
+    for (int i = 0; i < MT_N; ++i) {
+        uint64_t x = (s->mt[i] & MT_UM) | (s->mt[(i + 1) % MT_N] & MT_LM);
+        uint64_t xa = x >> 1;
+        if (x & 1ULL) xa ^= 0xB5026F5AA96619E9ULL;
+        s->mt[i] = s->mt[(i + MT_M) % MT_N] ^ xa;
+    }
+    s->idx = 0;
+}
+
+uint64_t ftdo_mt64_next(ftdo_mt64* s) {
+    if (s->idx >= MT_N) mt64_twist(s);
+    uint64_t y = s->mt[s->idx++];
+    y ^= (y >> 29) & 0x5555555555555555ULL;
+    y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+    y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+    y ^= (y >> 43);
+    return y;
+}
+
+uint64_t ftdo_mt64_nth(uint64_t seed, uint64_t n) {
+    ftdo_mt64 s;
+    ftdo_mt64_seed(&s, seed);
+    uint64_t v = 0;
+    for (uint64_t i = 0; i < n; ++i) v = ftdo_mt64_next(&s);
+    return v;
+}
+
+/* ------------------------------------------------------------------ */
+/* QC codec */
+uint8_t ftdo_encode_qc(int re, int im) {
+    uint8_t rn = (re < 0) ? (uint8_t)(0x8 | ((-re) & 0x7)) : (uint8_t)(re & 0x7);
+    uint8_t in = (im < 0) ? (uint8_t)(0x8 | ((-im) & 0x7)) : (uint8_t)(im & 0x7);
+    return (uint8_t)((rn << 4) | in);
+}
+
+static int decode_nibble(unsigned n) {
+    int mag = (int)(n & 0x7);
+    return (n & 0x8) ? -mag : mag;
+}
+
+void ftdo_decode_qc(uint8_t b, int* re, int* im) {
+    *re = decode_nibble((b >> 4) & 0xF);
+    *im = decode_nibble(b & 0xF);
+}
+
+/* OCP E4M3 encoding of a sign-magnitude nibble, derived from the format
+ * (bias 7, 3 mantissa bits): |v| = 2^(e-7) * (1 + m/8). */
+static uint8_t nibble_to_fp8(unsigned n) {
+    unsigned mag = n & 0x7, sign = (n & 0x8) ? 0x80u : 0u;
+    if (mag == 0) return (uint8_t)sign;
+    unsigned e = 0;
+    while ((mag >> (e + 1)) != 0) ++e;          /* floor(log2 mag) */
+    unsigned frac = (mag << 3 >> e) & 0x7;      /* 3-bit mantissa (exact for mag <= 7) */
+    return (uint8_t)(sign | ((e + 7) << 3) | frac);
+}
+
+uint16_t ftdo_qc_to_fp8_pair(uint8_t b) {
+    return (uint16_t)(nibble_to_fp8((b >> 4) & 0xF) | ((unsigned)nibble_to_fp8(b & 0xF) << 8));
+}
+
+float ftdo_fp8e4m3_to_float(uint8_t b) {
+    int sign = (b & 0x80) ? -1 : 1;
+    int e = (b >> 3) & 0xF, m = b & 0x7;
+    if (e == 0xF && m == 0x7) return NAN;        /* OCP E4M3: only S.1111.111 is NaN */
+    float v = (e == 0) ? ldexpf((float)m / 8.0f, -6) : ldexpf(1.0f + (float)m / 8.0f, e - 7);
+    return sign * v;
+}
+
+/* ------------------------------------------------------------------ */
+/* generators */
+void ftdo_gen_qc_uniform(uint64_t seed, int64_t n, uint8_t* out) {
+    ftdo_mt64 s;
+    ftdo_mt64_seed(&s, seed);
+    uint64_t word = 0;
+    int left = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int v[2];
+        for (int c = 0; c < 2; ++c) {
+            for (;;) {
+                if (left == 0) { word = ftdo_mt64_next(&s); left = 16; }
+                unsigned u = (unsigned)(word & 0xF);
+                word >>= 4;
+                --left;
+                if (u != 15) { v[c] = (int)u - 7; break; }
+            }
+        }
+        out[i] = ftdo_encode_qc(v[0], v[1]);
+    }
+}
+
+void ftdo_gen_bytes(uint64_t seed, int64_t n, uint8_t* out) {
+    ftdo_mt64 s;
+    ftdo_mt64_seed(&s, seed);
+    uint64_t word = 0;
+    int left = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (left == 0) { word = ftdo_mt64_next(&s); left = 8; }
+        out[i] = (uint8_t)(word & 0xFF);
+        word >>= 8;
+        --left;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* HERK */
+static void decode_planes(const uint8_t* A, int16_t* re, int16_t* im, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        int r, m;
+        ftdo_decode_qc(A[i], &r, &m);
+        re[i] = (int16_t)r;
+        im[i] = (int16_t)m;
+    }
+}
+
+/* One output row i of one batch: C_row[j] for j <= i. int32 partial sums over
+ * blocks of 2^20 samples (|term| <= 98) then int64, so any K is exact. */
+static void herk_row(const int16_t* re, const int16_t* im, int K, int i, int64_t* Crow) {
+    const int16_t* ar = re + (int64_t)i * K;
+    const int16_t* ai = im + (int64_t)i * K;
+    for (int j = 0; j <= i; ++j) {
+        const int16_t* br = re + (int64_t)j * K;
+        const int16_t* bi = im + (int64_t)j * K;
+        int64_t sr = 0, si = 0;
+        for (int k0 = 0; k0 < K; k0 += (1 << 20)) {
+            int k1 = (K - k0 > (1 << 20)) ? k0 + (1 << 20) : K;
+            int32_t pr = 0, pi = 0;
+            for (int k = k0; k < k1; ++k) {
+                /* a * conj(b) = (ar br + ai bi) + i (ai br - ar bi) */
+                pr += (int32_t)ar[k] * br[k] + (int32_t)ai[k] * bi[k];
+                pi += (int32_t)ai[k] * br[k] - (int32_t)ar[k] * bi[k];
+            }
+            sr += pr;
+            si += pi;
+        }
+        Crow[2 * j] = sr;
+        Crow[2 * j + 1] = (i == j) ? 0 : si;
+    }
+}
+
+void ftdo_herk_int64_rows(const uint8_t* A, int64_t* C, int N, int K, int batch,
+                          int row0, int row1, int threads) {
+    if (N <= 0 || K < 0 || batch <= 0) return;
+    if (row0 < 0) row0 = 0;
+    if (row1 > N) row1 = N;
+    int64_t nk = (int64_t)N * K;
+    int16_t* re = (int16_t*)malloc(sizeof(int16_t) * (size_t)(nk * batch > 0 ? nk * batch : 1));
+    int16_t* im = (int16_t*)malloc(sizeof(int16_t) * (size_t)(nk * batch > 0 ? nk * batch : 1));
+    decode_planes(A, re, im, nk * batch);
+    int64_t packed2 = (int64_t)N * (N + 1); /* scalars per batch */
+    int64_t rows = (int64_t)(row1 - row0);
+    int64_t total = rows * batch;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+#endif
+    for (int64_t t = 0; t < total; ++t) {
+        int b = (int)(t / rows);
+        /* heavy rows first: better dynamic balance */
+        int i = row1 - 1 - (int)(t % rows);
+        int64_t* Crow = C + b * packed2 + (int64_t)i * (i + 1);
+        herk_row(re + b * nk, im + b * nk, K, i, Crow);
+    }
+    (void)threads;
+    free(re);
+    free(im);
+}
+
+void ftdo_herk_int64(const uint8_t* A, int64_t* C, int N, int K, int batch, int threads) {
+    ftdo_herk_int64_rows(A, C, N, K, batch, 0, N, threads);
+}
+
+static float epilogue(int64_t acc, int diag_im, float alpha, float beta, float old) {
+    float v = diag_im ? 0.0f : (float)acc;
+    v = v * alpha;
+    if (beta != 0.0f) v = fmaf(beta, old, v);
+    return v;
+}
+
+void ftdo_herk_f32(const uint8_t* A, float* C, int N, int K, int batch,
+                   float alpha, float beta, int threads) {
+    int64_t packed2 = (int64_t)N * (N + 1);
+    int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * (size_t)(packed2 * batch));
+    ftdo_herk_int64(A, acc, N, K, batch, threads);
+    for (int b = 0; b < batch; ++b)
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j <= i; ++j)
+                for (int c = 0; c < 2; ++c) {
+                    int64_t o = b * packed2 + ((int64_t)i * (i + 1) / 2 + j) * 2 + c;
+                    C[o] = epilogue(acc[o], i == j && c == 1, alpha, beta, C[o]);
+                }
+    free(acc);
+}
+
+/* IEEE binary32 -> binary16, round to nearest even, overflow to inf */
+uint16_t ftdo_f32_to_f16_rne(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    uint16_t sign = (uint16_t)((x >> 16) & 0x8000u);
+    uint32_t ax = x & 0x7FFFFFFFu;
+    if (ax >= 0x7F800000u) /* inf / nan */
+        return (uint16_t)(sign | 0x7C00u | (ax > 0x7F800000u ? 0x200u : 0u));
+    if (ax >= 0x477FF000u) /* >= 65520 rounds to inf */
+        return (uint16_t)(sign | 0x7C00u);
+    if (ax < 0x33000001u) /* < 2^-25 (or exactly): rounds to 0 */
+        return sign;
+    int e = (int)(ax >> 23) - 127;
+    uint32_t mant = (ax & 0x7FFFFFu) | 0x800000u;
+    if (e < -14) { /* subnormal half */
+        int shift = -e - 14 + 13;  /* total right shift of the 24-bit significand */
+        uint32_t q = mant >> shift;
+        uint32_t rem = mant & ((1u << shift) - 1u);
+        uint32_t half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (q & 1u))) ++q;
+        return (uint16_t)(sign | q);
+    }
+    uint32_t q = mant >> 13, rem = mant & 0x1FFFu;
+    uint32_t h = ((uint32_t)(e + 15) << 10) | (q & 0x3FFu);
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++h; /* may carry into exponent: correct */
+    return (uint16_t)(sign | h);
+}
+
+float ftdo_f16_to_f32(uint16_t h) {
+    int sign = (h & 0x8000) ? -1 : 1;
+    int e = (h >> 10) & 0x1F, m = h & 0x3FF;
+    if (e == 0x1F) return m ? NAN : sign * INFINITY;
+    float v = (e == 0) ? ldexpf((float)m, -24) : ldexpf((float)(m | 0x400), e - 25);
+    return sign * v;
+}
+
+void ftdo_herk_f16(const uint8_t* A, uint16_t* C, int N, int K, int batch,
+                   float alpha, float beta, int threads) {
+    int64_t packed2 = (int64_t)N * (N + 1);
+    int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * (size_t)(packed2 * batch));
+    ftdo_herk_int64(A, acc, N, K, batch, threads);
+    for (int b = 0; b < batch; ++b)
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j <= i; ++j)
+                for (int c = 0; c < 2; ++c) {
+                    int64_t o = b * packed2 + ((int64_t)i * (i + 1) / 2 + j) * 2 + c;
+                    float old = ftdo_f16_to_f32(C[o]);
+                    C[o] = ftdo_f32_to_f16_rne(epilogue(acc[o], i == j && c == 1, alpha, beta, old));
+                }
+    free(acc);
+}

@@ -1,0 +1,67 @@
+/*
+ * herk_oracle.h -- CPU oracle for the X-engine HERK (test infrastructure only).
+ *
+ * This library is the checker: only tests, the smoke check and the CPU-baseline
+ * leg of bench.py load it.  The product path (libftdx.so) never links or calls it.
+ *
+ * Conventions restated from the reference (cutlass_gemm_api side):
+ *   - INT4 "QC" byte per complex sample: high nibble = Re, low nibble = Im,
+ *     each nibble sign-magnitude (bit 3 = sign, bits 2:0 = |v|), range [-7, 7],
+ *     -0 legal.  (cutlass_interface/test_herk_int4.cu:36-45,
+ *     tests/visibility_pipeline_test.cpp:42-53)
+ *   - Input layout [batch][N][K], K fastest.
+ *   - C[i][j] = sum_k A[i][k] * conj(A[j][k]) for j <= i; Im(C[i][i]) forced 0;
+ *     packed lower triangle, tri = i(i+1)/2 + j, interleaved [re, im];
+ *     batch stride N(N+1) scalars.  (cutlass_interface/test_herk_int4.cu:51-70,
+ *     cutlass_interface/shared/herk_kernel_common.hpp:353-424)
+ *   - Epilogue: v = acc; if diag && imag: v = 0; v *= alpha;
+ *     if beta != 0: v = fma(beta, old, v).  (herk_kernel_common.hpp:405-420)
+ */
+#ifndef FTD_HERK_ORACLE_H
+#define FTD_HERK_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- portable generator: mt19937_64 (64-bit Mersenne twister) ---- */
+typedef struct { uint64_t mt[312]; int idx; } ftdo_mt64;
+void     ftdo_mt64_seed(ftdo_mt64* s, uint64_t seed);
+uint64_t ftdo_mt64_next(ftdo_mt64* s);
+/* n-th (1-based) output of mt19937_64 seeded with `seed` */
+uint64_t ftdo_mt64_nth(uint64_t seed, uint64_t n);
+
+/* ---- QC sign-magnitude codec ---- */
+uint8_t ftdo_encode_qc(int re, int im);
+void    ftdo_decode_qc(uint8_t b, int* re, int* im);
+/* QC byte -> [fp8(Re), fp8(Im)] as (Re byte) | (Im byte << 8), OCP E4M3 */
+uint16_t ftdo_qc_to_fp8_pair(uint8_t b);
+/* OCP E4M3 byte -> float (independent decoder, all 256 codes) */
+float   ftdo_fp8e4m3_to_float(uint8_t b);
+
+/* Uniform [-7,7] complex int4 samples, encoded QC, from 4-bit fields of the
+ * mt19937_64 stream (fields LSB first, value 15 rejected, v = u - 7; Re then Im). */
+void ftdo_gen_qc_uniform(uint64_t seed, int64_t n, uint8_t* out);
+/* Raw random bytes (includes -0 nibbles): 8 bytes per word, LSB first. */
+void ftdo_gen_bytes(uint64_t seed, int64_t n, uint8_t* out);
+
+/* ---- HERK oracles (plain path, cutlass_gemm_api convention) ---- */
+/* C: [batch][N(N+1)/2][2] int64 (re, im), Im(diag) = 0. threads<=0: OpenMP default */
+void ftdo_herk_int64(const uint8_t* A, int64_t* C, int N, int K, int batch, int threads);
+/* Same, rows [row0, row1) of each batch only (bounded CPU-baseline sample). */
+void ftdo_herk_int64_rows(const uint8_t* A, int64_t* C, int N, int K, int batch,
+                          int row0, int row1, int threads);
+/* FP32 output with the kernel epilogue (alpha, beta; C read when beta != 0). */
+void ftdo_herk_f32(const uint8_t* A, float* C, int N, int K, int batch,
+                   float alpha, float beta, int threads);
+/* FP16 output (IEEE binary16 bits, round-to-nearest-even from the FP32 value). */
+void ftdo_herk_f16(const uint8_t* A, uint16_t* C, int N, int K, int batch,
+                   float alpha, float beta, int threads);
+
+uint16_t ftdo_f32_to_f16_rne(float f);
+float    ftdo_f16_to_f32(uint16_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,135 @@
+// test_herk_api.cpp -- end-to-end test of the C++ drop-in HERK API on the GPU,
+// in the manner of the reference's cutlass_interface/test_herk_int4.cu:290-460:
+// generate int4 voltages, call CutlassComplexGemm::herk (INT4 -> FP32 and FP16)
+// and ggp::herkBatchedCutlassQC, compare with the CPU oracle.  The reference test
+// allows max_err < max(1, 0.02K); this one requires bit-exact FP32 output.
+//
+// usage: test_herk_api N K batch [seed]
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "cutlass_gemm_api.h"
+#include "ggp_cutlass_interface.h"
+#include "herk_oracle.h"
+
+#define HIP_OK(x)                                                                       \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) {                                                         \
+            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_),      \
+                         __FILE__, __LINE__);                                           \
+            std::exit(2);                                                               \
+        }                                                                               \
+    } while (0)
+
+using namespace cutlass_gemm_api;
+
+int main(int argc, char** argv) {
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s N K batch [seed]\n", argv[0]);
+        return 2;
+    }
+    const int N = std::atoi(argv[1]), K = std::atoi(argv[2]), batch = std::atoi(argv[3]);
+    const uint64_t seed = argc > 4 ? std::strtoull(argv[4], nullptr, 10) : 1;
+    const int64_t nA = (int64_t)N * K * batch;
+    const int64_t nC = (int64_t)N * (N + 1) * batch;  // scalars
+    std::printf("test_herk_api N=%d K=%d batch=%d seed=%llu\n", N, K, batch,
+                (unsigned long long)seed);
+    std::fflush(stdout);
+
+    std::vector<uint8_t> hA(nA);
+    ftdo_gen_qc_uniform(seed, nA, hA.data());
+    std::vector<float> ref(nC, 0.0f);
+    ftdo_herk_f32(hA.data(), ref.data(), N, K, batch, 1.0f, 0.0f, 0);
+    std::vector<uint16_t> ref16(nC, 0);
+    ftdo_herk_f16(hA.data(), ref16.data(), N, K, batch, 1.0f, 0.0f, 0);
+
+    uint8_t* dA = nullptr;
+    float* dC = nullptr;
+    uint16_t* dC16 = nullptr;
+    HIP_OK(hipMalloc(&dA, nA > 0 ? nA : 1));
+    HIP_OK(hipMalloc(&dC, nC * sizeof(float)));
+    HIP_OK(hipMalloc(&dC16, nC * sizeof(uint16_t)));
+    HIP_OK(hipMemcpy(dA, hA.data(), nA, hipMemcpyHostToDevice));
+
+    int failures = 0;
+    CutlassComplexGemm api;
+    api.set_strategy_tune_verbosity(0);
+
+    auto compare32 = [&](const char* label) {
+        std::vector<float> got(nC);
+        HIP_OK(hipMemcpy(got.data(), dC, nC * sizeof(float), hipMemcpyDeviceToHost));
+        int64_t bad = 0;
+        for (int64_t i = 0; i < nC; ++i)
+            if (std::memcmp(&got[i], &ref[i], 4) != 0) {
+                if (bad < 5)
+                    std::printf("    [%lld] got %.1f ref %.1f\n", (long long)i, got[i], ref[i]);
+                ++bad;
+            }
+        std::printf("  %-40s %s (%lld mismatches)\n", label, bad ? "FAIL" : "PASS",
+                    (long long)bad);
+        failures += bad ? 1 : 0;
+    };
+
+    // Test 1: herk INT4 -> FP32 via the class
+    HIP_OK(hipMemset(dC, 0xFF, nC * sizeof(float)));
+    int rc = api.herk(dA, dC, N, K, batch, InputPrecision::INT4, ComputePrecision::FP8,
+                      OutputPrecision::FP32, OutputFormat::PackedTriangle, 1.0f, 0.0f, nullptr,
+                      false);
+    HIP_OK(hipDeviceSynchronize());
+    if (rc != 0) {
+        std::printf("  herk returned %d\n", rc);
+        ++failures;
+    } else {
+        compare32("CutlassComplexGemm::herk INT4->FP32");
+    }
+
+    // Test 2: herk INT4 -> FP16 via the class
+    HIP_OK(hipMemset(dC16, 0xFF, nC * sizeof(uint16_t)));
+    rc = api.herk(dA, dC16, N, K, batch, InputPrecision::INT4, ComputePrecision::FP8,
+                  OutputPrecision::FP16, OutputFormat::PackedTriangle, 1.0f, 0.0f, nullptr, false);
+    HIP_OK(hipDeviceSynchronize());
+    {
+        std::vector<uint16_t> got(nC);
+        HIP_OK(hipMemcpy(got.data(), dC16, nC * 2, hipMemcpyDeviceToHost));
+        int64_t bad = 0;
+        for (int64_t i = 0; i < nC; ++i) bad += got[i] != ref16[i];
+        std::printf("  %-40s %s (%lld mismatches)\n", "CutlassComplexGemm::herk INT4->FP16",
+                    (rc || bad) ? "FAIL" : "PASS", (long long)bad);
+        failures += (rc || bad) ? 1 : 0;
+    }
+
+    // Test 3: GGP glue
+    ggp::initCutlassApi(0, nullptr, nullptr, false, 0);
+    HIP_OK(hipMemset(dC, 0xFF, nC * sizeof(float)));
+    rc = ggp::herkBatchedCutlassQC(dA, dC, N, K, batch, nullptr);
+    HIP_OK(hipDeviceSynchronize());
+    if (rc != 0) {
+        std::printf("  herkBatchedCutlassQC returned %d\n", rc);
+        ++failures;
+    } else {
+        compare32("ggp::herkBatchedCutlassQC");
+    }
+
+    // Test 4: unsupported combination throws, as the reference
+    bool threw = false;
+    try {
+        api.herk(dA, dC, N, K, batch, InputPrecision::INT4, ComputePrecision::FP4,
+                 OutputPrecision::FP32);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    std::printf("  %-40s %s\n", "FP4 + FP32 output throws", threw ? "PASS" : "FAIL");
+    failures += threw ? 0 : 1;
+
+    HIP_OK(hipFree(dA));
+    HIP_OK(hipFree(dC));
+    HIP_OK(hipFree(dC16));
+    std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
+    return failures ? 1 : 0;
+}
