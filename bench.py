@@ -40,6 +40,68 @@ def synth_qc(torch, n: int, seed: int, device) -> "torch.Tensor":
     return (nib[0] << 4) | nib[1]
 
 
+def batched_vs_plain(torch, ftdx, dev, count: int, seed: int, reps: int) -> dict:
+    """Batched uneven HERK: `count` problems, N ~ U{256..4096 (powers of 2)},
+    K ~ U{256, 320, ..., 4096}; one grouped persistent launch (hipGraph replay,
+    autotuned tile configuration) vs one plain launch per problem on one stream.
+    Output bytes of the two must be identical."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    Ns = rng.choice([256, 512, 1024, 2048, 4096], size=count).tolist()
+    Ks = (rng.integers(4, 65, size=count) * 64).tolist()
+    a_off, c_off, oa, oc = [], [], 0, 0
+    for N, K in zip(Ns, Ks):
+        a_off.append(oa)
+        c_off.append(oc)
+        oa += (N * K + 255) // 256 * 256
+        oc += (N * (N + 1) + 63) // 64 * 64
+    A = synth_qc(torch, oa, seed, dev)
+    Cp = torch.empty(oc, dtype=torch.float32, device=dev)
+    Cg = torch.empty(oc, dtype=torch.float32, device=dev)
+    pp = [(A[a:a + N * K], Cp[c:c + N * (N + 1)], N, K) for a, c, N, K in zip(a_off, c_off, Ns, Ks)]
+    pg = [(A[a:a + N * K], Cg[c:c + N * (N + 1)], N, K) for a, c, N, K in zip(a_off, c_off, Ns, Ks)]
+    flop = sum(8.0 * N * N * K for N, K in zip(Ns, Ks))
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_strategy_tune_verbosity(0)
+    arr = ftdx.problem_array(pp)
+    t_plan = time.perf_counter()
+    plan = ftdx.GroupedPlan(ctx, pg, tile_config=ftdx.TILE_AUTO, use_graph=True)
+    t_plan = time.perf_counter() - t_plan
+    stream = torch.cuda.current_stream()
+    ftdx.herk_each(ctx, pp, stream=stream, arr=arr)
+    plan.run(stream)
+    torch.cuda.synchronize()
+    identical = bool(torch.equal(Cp.view(torch.int32), Cg.view(torch.int32)))
+    tp, tg = [], []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(reps):
+        e0.record(stream)
+        ftdx.herk_each(ctx, pp, stream=stream, arr=arr)
+        e1.record(stream)
+        e1.synchronize()
+        tp.append(e0.elapsed_time(e1))
+        e0.record(stream)
+        plan.run(stream)
+        e1.record(stream)
+        e1.synchronize()
+        tg.append(e0.elapsed_time(e1))
+    tp.sort()
+    tg.sort()
+    ms_p, ms_g = tp[len(tp) // 2], tg[len(tg) // 2]
+    info = plan.info()
+    plan.close()
+    return {
+        "problems": count, "seed": seed, "flop_4m_equiv": flop,
+        "grouped_ms": round(ms_g, 4), "plain_loop_ms": round(ms_p, 4),
+        "grouped_tflops": round(flop / (ms_g * 1e-3) / 1e12, 2),
+        "plain_loop_tflops": round(flop / (ms_p * 1e-3) / 1e12, 2),
+        "speedup": round(ms_p / ms_g, 3), "identical_output": identical,
+        "tile_config": info["tile_config"], "workgroups": info["workgroups"],
+        "items": info["items"], "schedule_imbalance": round(info["imbalance"], 4),
+        "plan_s": round(t_plan, 3), "graph": True,
+    }
+
+
 def cpu_baseline(N: int, K: int, seed: int, target_s: float) -> dict:
     """Oracle HERK (int64, OpenMP) on rows of one channel; rate in 4M-equiv TFLOP/s."""
     import numpy as np
@@ -116,6 +178,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-problems", type=int, default=512)
+    ap.add_argument("--tile-config", type=int, default=-1,
+                    help="-1: autotune the headline shape; else a fixed FTDX_TILE_* id")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -132,9 +198,21 @@ def main() -> int:
     A = synth_qc(torch, N * K * B, args.seed, dev)
     C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_strategy_tune_verbosity(0)
+    if args.tile_config >= 0:
+        ctx.set_launch_config(args.tile_config, 1)
 
+    def step(tune=False):
+        rc = ctx.herk(A, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                      ftdx.OutputPrecision.FP32, stream=stream, tune=tune)
+        if rc != 0:
+            raise RuntimeError(f"herk failed: {rc} {ftdx.last_error()}")
+
+    step(tune=True)  # autotune (cached in the context) outside the timed region
+    cfg, remap = ctx.query_config(N, K, B)
     for _ in range(args.warmup):
-        ftdx.herk_int4(A, N, K, B, C=C, stream=stream)
+        step()
     torch.cuda.synchronize()
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -143,7 +221,7 @@ def main() -> int:
     t0 = time.perf_counter()
     for s in range(args.steps):
         starts[s].record(stream)
-        ftdx.herk_int4(A, N, K, B, C=C, stream=stream)
+        step()
         ends[s].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -157,6 +235,16 @@ def main() -> int:
     achieved = flop / (kern_ms * 1e-3) / 1e12
     log(f"bench: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms "
         f"(median {per[len(per) // 2]:.3f}), {value:.1f} TFLOP/s 4M-equiv")
+
+    batched = None
+    if not args.no_batched:
+        del C
+        torch.cuda.empty_cache()
+        log(f"bench: batched uneven HERK, {args.batched_problems} problems ...")
+        batched = batched_vs_plain(torch, ftdx, dev, args.batched_problems, 3, 5)
+        log(f"bench: grouped {batched['grouped_ms']:.3f} ms vs plain loop "
+            f"{batched['plain_loop_ms']:.3f} ms -> {batched['speedup']:.2f}x, "
+            f"identical={batched['identical_output']}")
 
     cpu = None
     if not args.no_cpu:
@@ -190,9 +278,11 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": round(achieved / FP8_DENSE_PEAK_TFLOPS, 4),
             "traffic": None,
-            "kernel": "herk_int4_fp8_tile_kernel",
+            "kernel": "herk_int4_fp8_plain_kernel",
             "kernel_ms": round(kern_ms, 4),
+            "tile_config": cfg, "xcd_remap": remap,
         },
+        "batched_vs_plain": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1,28 +1,36 @@
 // ftdx_capi.cpp -- extern "C" entry points of libftdx.so (declared in include/ftdx.h).
 //
-// Validation, dispatch and per-context settings for the HERK kernels; the
-// reference counterpart of each entry point is cited in ftdx.h.
+// Validation, launch-configuration choice (explicit setting, tuning cache or the
+// autotuner), and dispatch of the plain HERK; the reference counterpart of each
+// entry point is cited in ftdx.h.
 #include "ftdx.h"
 
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "ftdx_internal.h"
+#include "ftdx_tune.h"
 #include "herk_fp8.h"
 
 struct ftdx_context {
     int device = 0;
     int herk_mode = FTDX_HERK_AUTO;
     int direct_tile = FTDX_DIRECT_TILE_AUTO;
+    int forced_cfg = -1;     // ftdx_set_launch_config
+    int forced_remap = -1;
     int graph = 0;
     int strategy_verbosity = 1;
     int kernel_verbosity = 0;
     std::string tune_cache_path;
     std::string kernel_cache_path;
+    bool tune_loaded = false;
+    ftdx::TuneCache tune;
     int init_N = 0, init_K = 0, init_batch = 0;
 };
 
@@ -31,6 +39,29 @@ namespace ftdx {
 namespace {
 thread_local std::string g_last_error;
 thread_local ftdx_context* g_default_ctx = nullptr;
+
+// Default launch configuration when nothing is tuned or set.
+constexpr int kDefaultCfg = TILE_128x128_K64;
+constexpr int kDefaultRemap = 1;
+
+// DirectTileConfig (reference names, cutlass_gemm_api.h:113-119) -> gfx950 tile
+// configuration: the reference enum picks K chunk x pipeline depth; here it
+// picks among the compiled tile shapes.
+int direct_tile_to_cfg(int d) {
+    switch (d) {
+        case FTDX_DIRECT_TILE_K32_B3: return TILE_64x64_K64;
+        case FTDX_DIRECT_TILE_K64_B2: return TILE_128x128_K64;
+        case FTDX_DIRECT_TILE_K64_B3: return TILE_256x128_K64;
+        case FTDX_DIRECT_TILE_K128_B2: return TILE_128x128_K128;
+        default: return -1;
+    }
+}
+
+std::string default_tune_path() {
+    const char* e = std::getenv("FTDX_TUNE_CACHE_PATH");
+    return e ? std::string(e) : std::string();
+}
+
 }  // namespace
 
 int fail(int code, const std::string& msg) {
@@ -42,6 +73,105 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(FTDX_ERR_HIP, std::string(what) + ": " + hipGetErrorName(e) + " (" +
                                   hipGetErrorString(e) + ")");
 }
+
+namespace {
+
+void ensure_tune_loaded(ftdx_context* ctx) {
+    if (ctx->tune_loaded) return;
+    ctx->tune_loaded = true;
+    if (ctx->tune_cache_path.empty()) ctx->tune_cache_path = default_tune_path();
+    if (!ctx->tune_cache_path.empty()) {
+        int skipped = 0;
+        int n = ctx->tune.load(ctx->tune_cache_path, &skipped);
+        if (ctx->strategy_verbosity >= 2 && n >= 0)
+            std::printf("ftdx tune: loaded %d entries from %s (%d lines skipped)\n", n,
+                        ctx->tune_cache_path.c_str(), skipped);
+    }
+}
+
+// Explicit choice (launch config / direct tile), else cache, else default.
+void pick_config(ftdx_context* ctx, const TuneKey& key, int& cfg, int& remap, bool& from_cache) {
+    from_cache = false;
+    cfg = ctx->forced_cfg >= 0 ? ctx->forced_cfg : direct_tile_to_cfg(ctx->direct_tile);
+    remap = ctx->forced_remap;
+    TuneEntry e;
+    if (cfg < 0 && ctx->tune.find(key, e)) {
+        cfg = e.cfg;
+        if (remap < 0) remap = e.xcd_remap;
+        from_cache = true;
+    }
+    if (cfg < 0) cfg = kDefaultCfg;
+    if (remap < 0) remap = kDefaultRemap;
+}
+
+// Times every (tile configuration, workgroup order) on this call's buffers and
+// stores the fastest (cutlass_interface/strategy_cache.hpp:504-800: warm-up, then
+// event-timed repeats, failures skipped).  With beta != 0 the timing runs use a
+// scratch output so the caller's C is not accumulated into.
+int autotune(ftdx_context* ctx, const HerkPlainArgs& a0, bool out_fp16, hipStream_t s,
+             const TuneKey& key, TuneEntry& best) {
+    HerkPlainArgs a = a0;
+    void* scratch = nullptr;
+    hipError_t e;
+    if (a.beta != 0.0f) {
+        size_t bytes = (size_t)a.c_bstride * a.batch * (out_fp16 ? 2 : 4);
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > free_b * 8 / 10) {
+            if (ctx->strategy_verbosity >= 1)
+                std::printf("ftdx tune: skipped (scratch %.1f GB exceeds 80%% of free memory)\n",
+                            bytes / 1e9);
+            return FTDX_ERR_NO_MEMORY;
+        }
+        if ((e = hipMalloc(&scratch, bytes)) != hipSuccess) return hip_fail(e, "autotune: scratch");
+        a.C = scratch;
+    }
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    best.ms = 1e30f;
+    best.cfg = kDefaultCfg;
+    best.xcd_remap = kDefaultRemap;
+    for (int cfg = 0; cfg < NUM_TILE_CONFIGS; ++cfg) {
+        for (int remap = 0; remap <= 1; ++remap) {
+            if (launch_herk_int4_plain(a, cfg, out_fp16, remap, s) != hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
+            const int reps = 3;
+            (void)hipEventRecord(t0, s);
+            for (int r = 0; r < reps; ++r) (void)launch_herk_int4_plain(a, cfg, out_fp16, remap, s);
+            (void)hipEventRecord(t1, s);
+            if (hipEventSynchronize(t1) != hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            ms /= reps;
+            if (ctx->strategy_verbosity >= 2)
+                std::printf("ftdx tune:   %-14s xcd_remap=%d  %.4f ms\n", tile_config_name(cfg),
+                            remap, ms);
+            if (ms < best.ms) {
+                best.ms = ms;
+                best.cfg = cfg;
+                best.xcd_remap = remap;
+            }
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (scratch) (void)hipFree(scratch);
+    ctx->tune.put(key, best);
+    if (ctx->strategy_verbosity >= 1)
+        std::printf("ftdx tune: N=%d K=%d batch=%d -> %s xcd_remap=%d (%.4f ms)\n", key.N, key.K,
+                    key.batch, tile_config_name(best.cfg), best.xcd_remap, best.ms);
+    if (!ctx->tune_cache_path.empty() && !ctx->tune.save(ctx->tune_cache_path) &&
+        ctx->strategy_verbosity >= 1)
+        std::printf("ftdx tune: could not write %s\n", ctx->tune_cache_path.c_str());
+    return FTDX_OK;
+}
+
+}  // namespace
 
 }  // namespace ftdx
 
@@ -58,10 +188,11 @@ int ftdx_create(ftdx_context** out) {
     if (!out) return fail(FTDX_ERR_INVALID_ARG, "ftdx_create: out is NULL");
     ftdx_context* c = new (std::nothrow) ftdx_context();
     if (!c) return fail(FTDX_ERR_NO_MEMORY, "ftdx_create: out of host memory");
-    hipError_t e = hipGetDevice(&c->device);
-    if (e != hipSuccess) {
-        delete c;
-        return hip_fail(e, "ftdx_create: hipGetDevice");
+    // Without a visible device the context still holds settings and the tuning
+    // cache; calls that launch work then fail with a device error.
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        (void)hipGetLastError();
+        c->device = -1;
     }
     *out = c;
     return FTDX_OK;
@@ -90,6 +221,16 @@ int ftdx_set_direct_tile_config(ftdx_context* ctx, int config) {
     return FTDX_OK;
 }
 
+int ftdx_set_launch_config(ftdx_context* ctx, int tile_config, int xcd_remap) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_launch_config: NULL context");
+    if (tile_config < -1 || tile_config >= ftdx::NUM_TILE_CONFIGS || xcd_remap < -1 ||
+        xcd_remap > 1)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_launch_config: out of range");
+    ctx->forced_cfg = tile_config;
+    ctx->forced_remap = xcd_remap;
+    return FTDX_OK;
+}
+
 int ftdx_set_direct_herk_graph(ftdx_context* ctx, int enable) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_direct_herk_graph: NULL context");
     ctx->graph = enable ? 1 : 0;
@@ -99,6 +240,7 @@ int ftdx_set_direct_herk_graph(ftdx_context* ctx, int enable) {
 int ftdx_set_tune_cache_path(ftdx_context* ctx, const char* path) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_tune_cache_path: NULL context");
     ctx->tune_cache_path = path ? path : "";
+    ctx->tune_loaded = false;
     return FTDX_OK;
 }
 
@@ -120,6 +262,36 @@ int ftdx_set_kernel_tune_cache_path(ftdx_context* ctx, const char* path) {
     return FTDX_OK;
 }
 
+int ftdx_tune_load(ftdx_context* ctx, const char* path, int* loaded, int* skipped) {
+    if (!ctx || !path) return fail(FTDX_ERR_INVALID_ARG, "ftdx_tune_load: NULL argument");
+    int sk = 0;
+    int n = ctx->tune.load(path, &sk);
+    ctx->tune_loaded = true;
+    if (loaded) *loaded = n < 0 ? 0 : n;
+    if (skipped) *skipped = sk;
+    if (n < 0) return fail(FTDX_ERR_IO, std::string("ftdx_tune_load: cannot read ") + path);
+    return FTDX_OK;
+}
+
+int ftdx_tune_save(ftdx_context* ctx, const char* path) {
+    if (!ctx || !path) return fail(FTDX_ERR_INVALID_ARG, "ftdx_tune_save: NULL argument");
+    if (!ctx->tune.save(path)) return fail(FTDX_ERR_IO, std::string("ftdx_tune_save: cannot write ") + path);
+    return FTDX_OK;
+}
+
+int ftdx_herk_query_config(ftdx_context* ctx, int N, int K, int batch_count, int output,
+                           int* tile_config, int* xcd_remap) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_query_config: NULL context");
+    ftdx::ensure_tune_loaded(ctx);
+    ftdx::TuneKey key{N, K, batch_count, output == FTDX_OUTPUT_FP16 ? 1 : 0, K % 16 == 0 ? 1 : 0};
+    int cfg, remap;
+    bool cached;
+    ftdx::pick_config(ctx, key, cfg, remap, cached);
+    if (tile_config) *tile_config = cfg;
+    if (xcd_remap) *xcd_remap = remap;
+    return FTDX_OK;
+}
+
 int ftdx_init_herk(ftdx_context* ctx, int N, int K, int batch_count, void* stream) {
     (void)stream;
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_init_herk: NULL context");
@@ -129,6 +301,7 @@ int ftdx_init_herk(ftdx_context* ctx, int N, int K, int batch_count, void* strea
     ctx->init_N = N;
     ctx->init_K = K;
     ctx->init_batch = batch_count;
+    ftdx::ensure_tune_loaded(ctx);
     return FTDX_OK;
 }
 
@@ -141,7 +314,6 @@ int ftdx_end_herk(ftdx_context* ctx) {
 int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch_count,
               int input, int compute, int output, int format, float alpha, float beta,
               void* stream, int tune) {
-    (void)tune;
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: NULL context");
     if (N < 0 || K < 0 || batch_count < 0)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: negative size");
@@ -172,7 +344,7 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, "ftdx_herk: hipGetDevice");
     if (dev != ctx->device)
-        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: context belongs to another device");
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: context belongs to another device (or none)");
 
     ftdx::HerkPlainArgs a{};
     a.A = static_cast<const uint8_t*>(A);
@@ -180,14 +352,43 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     a.N = N;
     a.K = K;
     a.batch = batch_count;
-    a.tiles = ftdx::herk_plain_tiles(N);
     a.alpha = alpha;
     a.beta = beta;
     a.a_bstride = (int64_t)N * K;
     a.c_bstride = (int64_t)N * (N + 1);
-    e = ftdx::launch_herk_int4_plain(a, output == FTDX_OUTPUT_FP16,
-                                     static_cast<hipStream_t>(stream));
+    const bool fp16 = output == FTDX_OUTPUT_FP16;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    ftdx::ensure_tune_loaded(ctx);
+    const int vec = (K % 16 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15u) == 0);
+    ftdx::TuneKey key{N, K, batch_count, fp16 ? 1 : 0, vec};
+    int cfg, remap;
+    bool cached;
+    ftdx::pick_config(ctx, key, cfg, remap, cached);
+    const bool explicit_cfg = ctx->forced_cfg >= 0 || ctx->direct_tile != FTDX_DIRECT_TILE_AUTO;
+    if (tune && !cached && !explicit_cfg) {
+        ftdx::TuneEntry best;
+        if (ftdx::autotune(ctx, a, fp16, s, key, best) == FTDX_OK) {
+            cfg = best.cfg;
+            if (ctx->forced_remap < 0) remap = best.xcd_remap;
+        }
+    }
+    e = ftdx::launch_herk_int4_plain(a, cfg, fp16, remap, s);
     if (e != hipSuccess) return hip_fail(e, "ftdx_herk: kernel launch");
+    return FTDX_OK;
+}
+
+int ftdx_herk_each(ftdx_context* ctx, const ftdx_herk_problem* problems, int count, int output,
+                   void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_each: NULL context");
+    if (count < 0 || (count > 0 && !problems))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_each: bad problem list");
+    for (int p = 0; p < count; ++p) {
+        const ftdx_herk_problem& q = problems[p];
+        int rc = ftdx_herk(ctx, q.A, q.C, q.N, q.K, 1, FTDX_INPUT_INT4, FTDX_COMPUTE_FP8, output,
+                           FTDX_FORMAT_PACKED_TRIANGLE, q.alpha, q.beta, stream, 0);
+        if (rc != FTDX_OK) return rc;
+    }
     return FTDX_OK;
 }
 
@@ -212,6 +413,9 @@ int ftdx_herk_int4(const void* A, void* C, int N, int K, int batch_count, int ou
 int ftdx_print_build_info(void) {
     std::printf("ftdx: MI355X X-engine HERK library, ABI %d\n", FTDX_ABI_VERSION);
     std::printf("  target: gfx950 (CDNA4), FP8 E4M3 MFMA v_mfma_f32_32x32x64_f8f6f4\n");
+    std::printf("  tile configurations:");
+    for (int c = 0; c < ftdx::NUM_TILE_CONFIGS; ++c) std::printf(" %s", ftdx::tile_config_name(c));
+    std::printf("\n  tune fingerprint: %s\n", ftdx::TuneCache::fingerprint().c_str());
     int n = 0;
     if (hipGetDeviceCount(&n) == hipSuccess && n > 0) {
         int dev = 0;

@@ -1,0 +1,315 @@
+// ftdx_grouped.cpp -- grouped HERK over many problems of uneven N and K in one
+// persistent launch (optionally replayed from a captured hipGraph).
+//
+// The reference runs one uniform (N, K, batch) HERK per call
+// (cutlass_interface/cutlass_gemm_api.cu:1707-1878); its callers loop over
+// sub-arrays.  Here a plan holds, in device memory, the problem descriptors and
+// a static schedule: every lower-triangle tile of every problem is one item;
+// items are sorted by cost (K rounded to the stage, less for diagonal tiles)
+// and assigned longest-first to the least-loaded persistent workgroup.  Ties go
+// to workgroups of the same XCD first, so items that share input rows run on
+// one XCD and meet in its L2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "ftdx.h"
+#include "ftdx_internal.h"
+#include "herk_fp8.h"
+
+struct ftdx_grouped_plan {
+    int device = 0;
+    int cfg = 0;
+    int out_fp16 = 0;
+    int num_wg = 0;
+    int count = 0;
+    int64_t items = 0;
+    double max_load = 0, mean_load = 0;
+    ftdx::GroupProblemDev* d_probs = nullptr;
+    ftdx::GroupItemDev* d_items = nullptr;
+    int* d_wg_begin = nullptr;
+    bool use_graph = false;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace ftdx {
+
+namespace {
+
+int stage_samples(int cfg) { return cfg == TILE_128x128_K128 ? 128 : 64; }
+
+// Relative cost of one tile: K rounded up to the stage, plus a fixed epilogue
+// share; tiles on the diagonal do about three quarters of the work.
+double tile_cost(int cfg, int K, bool diag) {
+    const int kc = stage_samples(cfg);
+    const double k = (double)((K + kc - 1) / kc) * kc + 192.0;
+    return diag ? 0.75 * k : k;
+}
+
+}  // namespace
+
+int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int count, int cfg,
+                  int out_fp16, int cu_count) {
+    const int TM = tile_config_rows(cfg), TN = tile_config_cols(cfg), r = TM / TN;
+    std::vector<GroupProblemDev> probs(count);
+    struct Item {
+        double cost;
+        int prob, tile;
+    };
+    std::vector<Item> items;
+    for (int p = 0; p < count; ++p) {
+        const ftdx_herk_problem& q = problems[p];
+        GroupProblemDev& d = probs[p];
+        d.A = static_cast<const uint8_t*>(q.A);
+        d.C = q.C;
+        d.N = q.N;
+        d.K = q.K;
+        d.alpha = q.alpha;
+        d.beta = q.beta;
+        d.vec = (q.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(q.A) & 15u) == 0);
+        d.tiles = herk_tiles(cfg, q.N);
+        int t = 0;
+        const int nbM = (q.N + TM - 1) / TM, nbN = (q.N + TN - 1) / TN;
+        for (int I = 0; I < nbM; ++I) {
+            const int cnt = std::min(nbN, r * (I + 1));
+            for (int J = 0; J < cnt; ++J, ++t) {
+                const bool diag = (J * TN >= I * TM);
+                items.push_back({tile_cost(cfg, q.K, diag), p, t});
+            }
+        }
+    }
+    P->items = (int64_t)items.size();
+    if (items.size() > (size_t)0x7FFFFFFF) return fail(FTDX_ERR_INVALID_ARG, "grouped: too many tiles");
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item& a, const Item& b) { return a.cost > b.cost; });
+
+    const int wg_per_cu = tile_config_wg_per_cu(cfg);
+    int G = (int)std::min<int64_t>((int64_t)cu_count * wg_per_cu, (int64_t)items.size());
+    if (G < 1) G = 1;
+    // rank: workgroups of XCD 0 first, then XCD 1, ... (hardware ids go round-robin)
+    std::vector<int> rank(G);
+    {
+        const int q = G / 8, rem = G % 8;
+        for (int g = 0; g < G; ++g) {
+            const int x = g % 8, local = g / 8;
+            rank[g] = x * q + std::min(x, rem) + local;
+        }
+    }
+    using Bin = std::pair<double, int>;  // (load, rank)
+    std::priority_queue<Bin, std::vector<Bin>, std::greater<Bin>> heap;
+    std::vector<int> by_rank(G);
+    for (int g = 0; g < G; ++g) {
+        heap.push({0.0, rank[g]});
+        by_rank[rank[g]] = g;
+    }
+    std::vector<std::vector<GroupItemDev>> lists(G);
+    std::vector<double> load(G, 0.0);
+    for (const Item& it : items) {
+        Bin b = heap.top();
+        heap.pop();
+        const int g = by_rank[b.second];
+        lists[g].push_back({it.prob, it.tile});
+        load[g] = b.first + it.cost;
+        heap.push({load[g], b.second});
+    }
+    std::vector<int> begin(G + 1, 0);
+    std::vector<GroupItemDev> flat;
+    flat.reserve(items.size());
+    for (int g = 0; g < G; ++g) {
+        begin[g] = (int)flat.size();
+        flat.insert(flat.end(), lists[g].begin(), lists[g].end());
+    }
+    begin[G] = (int)flat.size();
+    P->max_load = *std::max_element(load.begin(), load.end());
+    P->mean_load = std::accumulate(load.begin(), load.end(), 0.0) / G;
+
+    hipError_t e;
+    if ((e = hipMalloc(&P->d_probs, sizeof(GroupProblemDev) * std::max(count, 1))) != hipSuccess)
+        return hip_fail(e, "grouped plan: hipMalloc");
+    if ((e = hipMalloc(&P->d_items, sizeof(GroupItemDev) * std::max<size_t>(flat.size(), 1))) !=
+        hipSuccess)
+        return hip_fail(e, "grouped plan: hipMalloc");
+    if ((e = hipMalloc(&P->d_wg_begin, sizeof(int) * (G + 1))) != hipSuccess)
+        return hip_fail(e, "grouped plan: hipMalloc");
+    if ((e = hipMemcpy(P->d_probs, probs.data(), sizeof(GroupProblemDev) * count,
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "grouped plan: upload");
+    if (!flat.empty() &&
+        (e = hipMemcpy(P->d_items, flat.data(), sizeof(GroupItemDev) * flat.size(),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "grouped plan: upload");
+    if ((e = hipMemcpy(P->d_wg_begin, begin.data(), sizeof(int) * (G + 1),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "grouped plan: upload");
+    P->num_wg = G;
+    P->cfg = cfg;
+    P->out_fp16 = out_fp16;
+    P->count = count;
+    return FTDX_OK;
+}
+
+hipError_t grouped_launch(ftdx_grouped_plan* P, hipStream_t s) {
+    return launch_herk_int4_grouped(P->d_probs, P->d_items, P->d_wg_begin, P->num_wg, P->cfg,
+                                    P->out_fp16 != 0, s);
+}
+
+void grouped_release(ftdx_grouped_plan* P) {
+    if (!P) return;
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
+    if (P->d_probs) (void)hipFree(P->d_probs);
+    if (P->d_items) (void)hipFree(P->d_items);
+    if (P->d_wg_begin) (void)hipFree(P->d_wg_begin);
+    P->exec = nullptr;
+    P->cap_stream = nullptr;
+    P->d_probs = nullptr;
+    P->d_items = nullptr;
+    P->d_wg_begin = nullptr;
+}
+
+int grouped_capture(ftdx_grouped_plan* P) {
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "grouped plan: hipStreamCreate");
+    if ((e = hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeRelaxed)) != hipSuccess)
+        return hip_fail(e, "grouped plan: hipStreamBeginCapture");
+    hipError_t le = grouped_launch(P, P->cap_stream);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(P->cap_stream, &graph);
+    if (le != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(le, "grouped plan: captured launch");
+    }
+    if (e != hipSuccess) return hip_fail(e, "grouped plan: hipStreamEndCapture");
+    e = hipGraphInstantiate(&P->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "grouped plan: hipGraphInstantiate");
+    P->use_graph = true;
+    return FTDX_OK;
+}
+
+// Times each tile configuration's plan on the problems' own buffers (outputs are
+// overwritten: only used when every beta is 0) and returns the fastest.
+int grouped_autotune(const ftdx_herk_problem* problems, int count, int out16, int cus, int& best,
+                     float& best_ms) {
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    best_ms = 1e30f;
+    for (int cfg = 0; cfg < NUM_TILE_CONFIGS; ++cfg) {
+        ftdx_grouped_plan P;
+        int rc = grouped_build(&P, problems, count, cfg, out16, cus);
+        if (rc != FTDX_OK) {
+            grouped_release(&P);
+            continue;
+        }
+        float ms = 1e30f;
+        if (grouped_launch(&P, nullptr) == hipSuccess) {
+            (void)hipEventRecord(t0, nullptr);
+            const int reps = 2;
+            for (int r = 0; r < reps; ++r) (void)grouped_launch(&P, nullptr);
+            (void)hipEventRecord(t1, nullptr);
+            if (hipEventSynchronize(t1) == hipSuccess) {
+                (void)hipEventElapsedTime(&ms, t0, t1);
+                ms /= reps;
+            }
+        }
+        (void)hipGetLastError();
+        grouped_release(&P);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cfg;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (best_ms >= 1e30f) return fail(FTDX_ERR_HIP, "grouped autotune: no configuration ran");
+    return FTDX_OK;
+}
+
+}  // namespace ftdx
+
+using ftdx::fail;
+using ftdx::hip_fail;
+
+extern "C" {
+
+int ftdx_herk_grouped_plan_create(ftdx_context* ctx, const ftdx_herk_problem* problems, int count,
+                                  int output, int tile_config, int use_graph,
+                                  ftdx_grouped_plan** plan) {
+    if (!ctx || !plan) return fail(FTDX_ERR_INVALID_ARG, "grouped plan: NULL context or plan");
+    *plan = nullptr;
+    if (count < 0 || (count > 0 && !problems))
+        return fail(FTDX_ERR_INVALID_ARG, "grouped plan: bad problem list");
+    if (output != FTDX_OUTPUT_FP32 && output != FTDX_OUTPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "grouped plan: unknown output precision");
+    if (tile_config < -1 || tile_config >= ftdx::NUM_TILE_CONFIGS)
+        return fail(FTDX_ERR_INVALID_ARG, "grouped plan: unknown tile configuration");
+    for (int p = 0; p < count; ++p) {
+        const ftdx_herk_problem& q = problems[p];
+        if (q.N < 0 || q.K < 0) return fail(FTDX_ERR_INVALID_ARG, "grouped plan: negative size");
+        if (q.N > 0 && ((!q.A && q.K > 0) || !q.C))
+            return fail(FTDX_ERR_INVALID_ARG, "grouped plan: NULL data pointer");
+        if (q.C && reinterpret_cast<uintptr_t>(q.C) % (output == FTDX_OUTPUT_FP32 ? 8u : 4u))
+            return fail(FTDX_ERR_INVALID_ARG, "grouped plan: C must be aligned to one element");
+    }
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "grouped plan: hipGetDevice");
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+        return hip_fail(e, "grouped plan: device attribute");
+    const int out16 = output == FTDX_OUTPUT_FP16 ? 1 : 0;
+    int cfg = tile_config >= 0 ? tile_config : ftdx::TILE_128x128_K64;
+    bool all_beta0 = true;
+    for (int p = 0; p < count; ++p) all_beta0 = all_beta0 && problems[p].beta == 0.0f;
+    if (tile_config < 0 && all_beta0 && count > 0) {
+        float best_ms = 0.0f;
+        int rc = ftdx::grouped_autotune(problems, count, out16, cus, cfg, best_ms);
+        if (rc != FTDX_OK) return rc;
+    }
+    auto* P = new ftdx_grouped_plan();
+    P->device = dev;
+    int rc = ftdx::grouped_build(P, problems, count, cfg, out16, cus);
+    if (rc == FTDX_OK && use_graph && P->items > 0) rc = ftdx::grouped_capture(P);
+    if (rc != FTDX_OK) {
+        ftdx::grouped_release(P);
+        delete P;
+        return rc;
+    }
+    *plan = P;
+    return FTDX_OK;
+}
+
+int ftdx_herk_grouped_plan_run(ftdx_grouped_plan* plan, void* stream) {
+    if (!plan) return fail(FTDX_ERR_INVALID_ARG, "grouped run: NULL plan");
+    if (plan->items == 0) return FTDX_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = plan->use_graph ? hipGraphLaunch(plan->exec, s) : ftdx::grouped_launch(plan, s);
+    if (e != hipSuccess) return hip_fail(e, "grouped run: launch");
+    return FTDX_OK;
+}
+
+int ftdx_herk_grouped_plan_info(const ftdx_grouped_plan* plan, int* num_workgroups,
+                                int64_t* num_items, int* tile_config, double* imbalance) {
+    if (!plan) return fail(FTDX_ERR_INVALID_ARG, "grouped info: NULL plan");
+    if (num_workgroups) *num_workgroups = plan->num_wg;
+    if (num_items) *num_items = plan->items;
+    if (tile_config) *tile_config = plan->cfg;
+    if (imbalance) *imbalance = plan->mean_load > 0 ? plan->max_load / plan->mean_load : 1.0;
+    return FTDX_OK;
+}
+
+int ftdx_herk_grouped_plan_destroy(ftdx_grouped_plan* plan) {
+    if (!plan) return FTDX_OK;
+    ftdx::grouped_release(plan);
+    delete plan;
+    return FTDX_OK;
+}
+
+}  // extern "C"

@@ -1,21 +1,60 @@
-// herk_fp8.h -- host-side launch interface of the plain FP8 HERK kernels (internal).
+// herk_fp8.h -- host-side launch interface of the FP8 HERK kernels (internal).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace ftdx {
 
+// Tile configurations compiled into the library (ids are stable: they appear in
+// the tuning cache file).
+enum TileConfigId {
+    TILE_128x128_K64 = 0,   // 4 waves (2x2, 64x64 each), 2 workgroups / CU
+    TILE_128x128_K128 = 1,  // 4 waves, 128-sample stages, 1 workgroup / CU
+    TILE_64x64_K64 = 2,     // 1 wave, 4 workgroups / CU: small N, fine-grained balance
+    TILE_256x128_K64 = 3,   // 8 waves (4x2, 64x64 each), 1 workgroup / CU
+    NUM_TILE_CONFIGS = 4
+};
+
+const char* tile_config_name(int cfg);
+int tile_config_rows(int cfg);   // TM
+int tile_config_cols(int cfg);   // TN
+int tile_config_wg_per_cu(int cfg);
+
 struct HerkPlainArgs {
     const uint8_t* A;   // [batch][N][K] QC bytes
     void* C;            // [batch][N(N+1)] scalars (float or half), packed lower triangle
     int N, K, batch;
-    int tiles;          // lower-triangle tiles per batch
     float alpha, beta;
     int64_t a_bstride;  // bytes of A per batch (N*K)
     int64_t c_bstride;  // scalars of C per batch (N*(N+1))
 };
 
-int herk_plain_tiles(int N);
-hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, bool out_fp16, hipStream_t stream);
+// Lower-triangle tiles per problem for a configuration.
+int herk_tiles(int cfg, int N);
+
+// Plain batched launch: grid = tiles x batch, XCD-aware workgroup order when
+// xcd_remap != 0.
+hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, int cfg, bool out_fp16, int xcd_remap,
+                                  hipStream_t stream);
+
+// ---- grouped persistent launch ----
+struct GroupProblemDev {
+    const uint8_t* A;
+    void* C;
+    int N, K;
+    float alpha, beta;
+    int vec;     // 1: 16-byte loads legal (K % 16 == 0, A 16-byte aligned)
+    int tiles;
+};
+
+struct GroupItemDev {
+    int prob;
+    int tile;
+};
+
+// Persistent kernel: workgroup g processes items[wg_begin[g] .. wg_begin[g+1]).
+hipError_t launch_herk_int4_grouped(const GroupProblemDev* probs, const GroupItemDev* items,
+                                    const int* wg_begin, int num_wg, int cfg, bool out_fp16,
+                                    hipStream_t stream);
 
 }  // namespace ftdx

@@ -1,259 +1,197 @@
-// herk_fp8.hip -- plain batched HERK on gfx950 FP8 MFMA, INT4 (QC) input.
+// herk_fp8.hip -- batched HERK on gfx950 FP8 MFMA with INT4 (QC) input.
 //
 //   C_b = alpha * A_b A_b^H + beta * C_b,  A_b in C^{N x K}, packed lower triangle.
 //
-// Replaces the reference's three-stage direct path
+// One kernel replaces the reference's three-stage direct path
 //   cast_int4_to_fp8_interleaved (cutlass_interface/cutlass_gemm_api.cu:1566-1623)
 //   herk_direct_fp8_precast_kernel (cutlass_interface/shared/herk_kernel_common.hpp:175-425)
 //   packed-triangle store           (herk_kernel_common.hpp:353-424)
-// with one kernel:
-//   * load stage: 16-byte global loads of QC bytes, unpacked in registers to OCP E4M3
-//     (v_perm_b32 table) and staged in LDS, double-buffered, one barrier per stage;
-//   * math: each wave owns a 64x64 complex block of a 128x128 lower-triangle tile.
-//     Rows use two MFMA A-operands per 32 rows: u_i = [re, im, ...] gives Re(C) and
-//     v_i = [im, -re, ...] (register byte-permute of u_i) gives Im(C); columns use u_j.
-//     4 real products per complex MAC, FP32 accumulate: exact for |sum| < 2^24,
-//     i.e. 98*K < 2^24 for inputs in [-7, 7];
-//   * epilogue: the 32x32 accumulator layout puts Re and Im of one (i, j) in the same
-//     lane and register, so each lane stores interleaved [re, im] straight into the
-//     packed triangle (Im(C_ii) = 0, alpha/beta as the reference epilogue).
-#include "herk_common.h"
+// The per-tile work is in herk_tile.h.  Two launch shapes:
+//   * plain:   one workgroup per (tile, batch), optional XCD-aware order so the
+//              workgroups one XCD runs together share input rows in its L2;
+//   * grouped: persistent workgroups walking a host-built static schedule over the
+//              tiles of many problems of different N and K (one launch).
+// FP32 accumulation is exact while |sum| < 2^24, i.e. 98*K < 2^24 for [-7, 7] input.
 #include "herk_fp8.h"
+#include "herk_tile.h"
 
 namespace ftdx {
 
 namespace {
 
-constexpr int TILE = 128;             // complex rows (and cols) per workgroup tile
-constexpr int KC = 64;                // complex samples per LDS stage
-constexpr int PITCH = 2 * KC + 16;    // LDS bytes per staged row (fp8 [re,im] + pad)
-constexpr int REGION = TILE * PITCH;  // one operand block (rows or cols) per stage
-constexpr int STAGE = 2 * REGION;     // rows + cols
-constexpr int THREADS = 256;
+using Cfg0 = TileCfg<128, 128, 2, 2, 64, 2, 2>;
+using Cfg1 = TileCfg<128, 128, 2, 2, 128, 1, 1>;
+using Cfg2 = TileCfg<64, 64, 1, 1, 64, 1, 4>;
+using Cfg3 = TileCfg<256, 128, 4, 2, 64, 2, 1>;
 
-template <typename OutT>
-struct Store;
+// logical workgroup id for hardware id w when consecutive ids go round-robin over
+// 8 XCDs: XCD x gets one contiguous range of logical ids.
+__device__ __forceinline__ int xcd_logical(int w, int total) {
+    const int x = w & 7, local = w >> 3;
+    const int q = total >> 3, r = total & 7;
+    return x * q + (x < r ? x : r) + local;
+}
 
-template <>
-struct Store<float> {
-    static __device__ __forceinline__ void load(const void* C, int64_t idx, float& re, float& im) {
-        float2 v = reinterpret_cast<const float2*>(C)[idx];
-        re = v.x;
-        im = v.y;
-    }
-    static __device__ __forceinline__ void store(void* C, int64_t idx, float re, float im) {
-        reinterpret_cast<float2*>(C)[idx] = make_float2(re, im);
-    }
-};
+template <class Cfg, typename OutT, int MODE>
+__global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
+herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
+    const int total = tiles * p.batch;
+    const int w = xcd_remap ? xcd_logical((int)blockIdx.x, total) : (int)blockIdx.x;
+    const int b = w / tiles;
+    const int t = w - b * tiles;
+    int I, J;
+    tri_decode_r(t, Cfg::TM / Cfg::TN, I, J);
+    TileJob j;
+    j.A = p.A + (int64_t)b * p.a_bstride;
+    j.C = static_cast<OutT*>(p.C) + (int64_t)b * p.c_bstride;
+    j.N = p.N;
+    j.K = p.K;
+    j.row0 = I * Cfg::TM;
+    j.col0 = J * Cfg::TN;
+    j.alpha = p.alpha;
+    j.beta = p.beta;
+    TileRunner<Cfg>::template run<OutT, MODE>(j, lds);
+}
 
-template <>
-struct Store<_Float16> {
-    static __device__ __forceinline__ void load(const void* C, int64_t idx, float& re, float& im) {
-        uint32_t w = reinterpret_cast<const uint32_t*>(C)[idx];
-        _Float16 h0, h1;
-        uint16_t lo = (uint16_t)(w & 0xFFFFu), hi = (uint16_t)(w >> 16);
-        __builtin_memcpy(&h0, &lo, 2);
-        __builtin_memcpy(&h1, &hi, 2);
-        re = (float)h0;
-        im = (float)h1;
-    }
-    static __device__ __forceinline__ void store(void* C, int64_t idx, float re, float im) {
-        _Float16 h0 = (_Float16)re, h1 = (_Float16)im;  // v_cvt_f16_f32: round to nearest even
-        uint16_t lo, hi;
-        __builtin_memcpy(&lo, &h0, 2);
-        __builtin_memcpy(&hi, &h1, 2);
-        reinterpret_cast<uint32_t*>(C)[idx] = (uint32_t)lo | ((uint32_t)hi << 16);
-    }
-};
-
-// Global -> registers for one stage. Each thread owns four 16-byte segments:
-// q = 0,1 -> row block, q = 2,3 -> column block (skipped on diagonal tiles).
-// Segment index s = tid + 256*(q&1): staged row r = s>>2, 16-byte segment s&3.
-template <bool VEC>
-__device__ __forceinline__ void load_stage(const uint8_t* __restrict__ Ab, int N, int K, int k0,
-                                           int row0, int col0, bool diag, int tid, uint4 (&g)[4]) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int s = tid + THREADS * (q & 1);
-        const int r = s >> 2, seg = s & 3;
-        const int grow = ((q >> 1) ? col0 : row0) + r;
-        const int kk = k0 + seg * 16;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (!((q >> 1) && diag) && grow < N) {
-            const uint8_t* src = Ab + (int64_t)grow * K + kk;
-            if (VEC) {
-                if (kk < K) v = *reinterpret_cast<const uint4*>(src);
-            } else {
-                uint32_t w[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-                for (int e = 0; e < 16; ++e)
-                    if (kk + e < K) w[e >> 2] |= (uint32_t)src[e] << (8 * (e & 3));
-                v = make_uint4(w[0], w[1], w[2], w[3]);
-            }
-        }
-        g[q] = v;
+template <class Cfg, typename OutT>
+__global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
+herk_int4_fp8_grouped_kernel(const GroupProblemDev* __restrict__ probs,
+                             const GroupItemDev* __restrict__ items,
+                             const int* __restrict__ wg_begin) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
+    const int beg = wg_begin[blockIdx.x], end = wg_begin[blockIdx.x + 1];
+    for (int it = beg; it < end; ++it) {
+        const GroupItemDev item = items[it];
+        const GroupProblemDev pr = probs[item.prob];
+        int I, J;
+        tri_decode_r(item.tile, Cfg::TM / Cfg::TN, I, J);
+        TileJob j;
+        j.A = pr.A;
+        j.C = pr.C;
+        j.N = pr.N;
+        j.K = pr.K;
+        j.row0 = I * Cfg::TM;
+        j.col0 = J * Cfg::TN;
+        j.alpha = pr.alpha;
+        j.beta = pr.beta;
+        if (pr.vec)
+            TileRunner<Cfg>::template run<OutT, LOAD_VEC16>(j, lds);
+        else
+            TileRunner<Cfg>::template run<OutT, LOAD_BYTES>(j, lds);
     }
 }
 
-__device__ __forceinline__ void store_stage(uint8_t* __restrict__ st, bool diag, int tid,
-                                            const uint4 (&g)[4]) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if ((q >> 1) && diag) continue;
-        const int s = tid + THREADS * (q & 1);
-        const int r = s >> 2, seg = s & 3;
-        uint4 lo, hi;
-        qc4_to_fp8x8(g[q].x, lo.x, lo.y);
-        qc4_to_fp8x8(g[q].y, lo.z, lo.w);
-        qc4_to_fp8x8(g[q].z, hi.x, hi.y);
-        qc4_to_fp8x8(g[q].w, hi.z, hi.w);
-        uint8_t* dst = st + (q >> 1) * REGION + r * PITCH + seg * 32;
-        *reinterpret_cast<uint4*>(dst) = lo;
-        *reinterpret_cast<uint4*>(dst + 16) = hi;
+template <class Cfg>
+hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap,
+                            hipStream_t stream) {
+    const int tiles = tri_tiles(a.N, Cfg::TM, Cfg::TN);
+    const int64_t blocks = (int64_t)tiles * a.batch;
+    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+    const bool vec = (a.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15u) == 0);
+    dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    if (out_fp16) {
+        if (vec)
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_VEC16>), grid, block,
+                               0, stream, a, tiles, xcd_remap);
+        else
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_BYTES>), grid, block,
+                               0, stream, a, tiles, xcd_remap);
+    } else {
+        if (vec)
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_VEC16>), grid, block, 0,
+                               stream, a, tiles, xcd_remap);
+        else
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_BYTES>), grid, block, 0,
+                               stream, a, tiles, xcd_remap);
     }
+    return hipGetLastError();
 }
 
-__device__ __forceinline__ v8i lds_read32(const uint8_t* p) {
-    uint4 a = *reinterpret_cast<const uint4*>(p);
-    uint4 b = *reinterpret_cast<const uint4*>(p + 16);
-    v8i r;
-    r[0] = (int)a.x; r[1] = (int)a.y; r[2] = (int)a.z; r[3] = (int)a.w;
-    r[4] = (int)b.x; r[5] = (int)b.y; r[6] = (int)b.z; r[7] = (int)b.w;
-    return r;
-}
-
-template <typename OutT, bool VEC>
-__global__ void __launch_bounds__(THREADS, 2) herk_int4_fp8_tile_kernel(HerkPlainArgs p) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[2 * STAGE];
-
-    const int wg = blockIdx.x;
-    const int b = wg / p.tiles;
-    const int t = wg - b * p.tiles;
-    int bi, bj;
-    tri_decode(t, bi, bj);
-    const bool diag = (bi == bj);
-    const int row0 = bi * TILE, col0 = bj * TILE;
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    // On a diagonal tile the wave above the diagonal has no output.
-    const bool active = !(diag && wr < wc);
-
-    const uint8_t* __restrict__ Ab = p.A + (int64_t)b * p.a_bstride;
-    const int N = p.N, K = p.K;
-
-    v16f acc_re[2][2], acc_im[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                acc_re[i][j][r] = 0.0f;
-                acc_im[i][j][r] = 0.0f;
-            }
-
-    const int nst = (K + KC - 1) / KC;
-    uint4 g[4];
-    if (nst > 0) {
-        load_stage<VEC>(Ab, N, K, 0, row0, col0, diag, tid, g);
-        store_stage(lds, diag, tid, g);
-    }
-    __syncthreads();
-
-    // per-lane LDS operand offsets: row (lane & 31) of a 32-row block, 32-byte half (lane >> 5)
-    const int lrow = lane & 31, lhalf = (lane >> 5) * 32;
-    const int a_off = (wr * 64 + lrow) * PITCH + lhalf;
-    const int b_off = (diag ? 0 : REGION) + (wc * 64 + lrow) * PITCH + lhalf;
-
-    for (int s = 0; s < nst; ++s) {
-        const bool more = (s + 1 < nst);
-        if (more) load_stage<VEC>(Ab, N, K, (s + 1) * KC, row0, col0, diag, tid, g);
-
-        const uint8_t* st = lds + (s & 1) * STAGE;
-        if (active) {
-#pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-                v8i au[2], av[2], bu[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    au[i] = lds_read32(st + a_off + i * 32 * PITCH + ks * 64);
-                    bu[i] = lds_read32(st + b_off + i * 32 * PITCH + ks * 64);
-                }
-#pragma unroll
-                for (int i = 0; i < 2; ++i) av[i] = conj_swap_fp8x32(au[i]);
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc_re[i][j] = mfma_32x32x64_fp8(au[i], bu[j], acc_re[i][j]);
-                        acc_im[i][j] = mfma_32x32x64_fp8(av[i], bu[j], acc_im[i][j]);
-                    }
-            }
-        }
-        if (more) store_stage(lds + ((s + 1) & 1) * STAGE, diag, tid, g);
-        __syncthreads();
-    }
-
-    if (!active) return;
-
-    // Epilogue. 32x32 accumulator layout: lane holds column (lane & 31);
-    // register r holds row 8*(r/4) + 4*(lane/32) + (r%4).
-    const int64_t cb = (int64_t)b * p.c_bstride / 2;  // complex offset of this batch
-    const float alpha = p.alpha, beta = p.beta;
-    const int n = lane & 31, g4 = (lane >> 5) * 4;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int gj = col0 + wc * 64 + j * 32 + n;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int gi = row0 + wr * 64 + i * 32 + (r >> 2) * 8 + g4 + (r & 3);
-                if (gi < N && gj <= gi) {
-                    const int64_t idx = cb + tri_index(gi, gj);
-                    float re = acc_re[i][j][r];
-                    float im = (gi == gj) ? 0.0f : acc_im[i][j][r];
-                    re = re * alpha;
-                    im = im * alpha;
-                    if (beta != 0.0f) {
-                        float ore, oim;
-                        Store<OutT>::load(p.C, idx, ore, oim);
-                        re = __builtin_fmaf(beta, ore, re);
-                        im = __builtin_fmaf(beta, oim, im);
-                    }
-                    Store<OutT>::store(p.C, idx, re, im);
-                }
-            }
-        }
+template <class Cfg>
+hipError_t launch_grouped_cfg(const GroupProblemDev* probs, const GroupItemDev* items,
+                              const int* wg_begin, int num_wg, bool out_fp16, hipStream_t stream) {
+    dim3 grid((unsigned)num_wg), block(Cfg::THREADS);
+    if (out_fp16)
+        hipLaunchKernelGGL((herk_int4_fp8_grouped_kernel<Cfg, _Float16>), grid, block, 0, stream,
+                           probs, items, wg_begin);
+    else
+        hipLaunchKernelGGL((herk_int4_fp8_grouped_kernel<Cfg, float>), grid, block, 0, stream,
+                           probs, items, wg_begin);
+    return hipGetLastError();
 }
 
 }  // namespace
 
-int herk_plain_tiles(int N) {
-    const int nb = (N + TILE - 1) / TILE;
-    return nb * (nb + 1) / 2;
+const char* tile_config_name(int cfg) {
+    switch (cfg) {
+        case TILE_128x128_K64: return "t128x128_k64";
+        case TILE_128x128_K128: return "t128x128_k128";
+        case TILE_64x64_K64: return "t64x64_k64";
+        case TILE_256x128_K64: return "t256x128_k64";
+        default: return "unknown";
+    }
 }
 
-hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, bool out_fp16, hipStream_t stream) {
-    if (a.N <= 0 || a.batch <= 0) return hipSuccess;
-    const int64_t blocks = (int64_t)a.tiles * a.batch;
-    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-    const bool vec = (a.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15u) == 0);
-    dim3 grid((unsigned)blocks), block(THREADS);
-    if (out_fp16) {
-        if (vec)
-            hipLaunchKernelGGL((herk_int4_fp8_tile_kernel<_Float16, true>), grid, block, 0, stream, a);
-        else
-            hipLaunchKernelGGL((herk_int4_fp8_tile_kernel<_Float16, false>), grid, block, 0, stream, a);
-    } else {
-        if (vec)
-            hipLaunchKernelGGL((herk_int4_fp8_tile_kernel<float, true>), grid, block, 0, stream, a);
-        else
-            hipLaunchKernelGGL((herk_int4_fp8_tile_kernel<float, false>), grid, block, 0, stream, a);
+int tile_config_rows(int cfg) {
+    switch (cfg) {
+        case TILE_128x128_K64: return Cfg0::TM;
+        case TILE_128x128_K128: return Cfg1::TM;
+        case TILE_64x64_K64: return Cfg2::TM;
+        case TILE_256x128_K64: return Cfg3::TM;
+        default: return 0;
     }
-    return hipGetLastError();
+}
+
+int tile_config_cols(int cfg) {
+    switch (cfg) {
+        case TILE_128x128_K64: return Cfg0::TN;
+        case TILE_128x128_K128: return Cfg1::TN;
+        case TILE_64x64_K64: return Cfg2::TN;
+        case TILE_256x128_K64: return Cfg3::TN;
+        default: return 0;
+    }
+}
+
+int tile_config_wg_per_cu(int cfg) {
+    switch (cfg) {
+        case TILE_128x128_K64: return Cfg0::WG_PER_CU;
+        case TILE_128x128_K128: return Cfg1::WG_PER_CU;
+        case TILE_64x64_K64: return Cfg2::WG_PER_CU;
+        case TILE_256x128_K64: return Cfg3::WG_PER_CU;
+        default: return 1;
+    }
+}
+
+int herk_tiles(int cfg, int N) { return tri_tiles(N, tile_config_rows(cfg), tile_config_cols(cfg)); }
+
+hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, int cfg, bool out_fp16, int xcd_remap,
+                                  hipStream_t stream) {
+    if (a.N <= 0 || a.batch <= 0) return hipSuccess;
+    switch (cfg) {
+        case TILE_128x128_K64: return launch_plain_cfg<Cfg0>(a, out_fp16, xcd_remap, stream);
+        case TILE_128x128_K128: return launch_plain_cfg<Cfg1>(a, out_fp16, xcd_remap, stream);
+        case TILE_64x64_K64: return launch_plain_cfg<Cfg2>(a, out_fp16, xcd_remap, stream);
+        case TILE_256x128_K64: return launch_plain_cfg<Cfg3>(a, out_fp16, xcd_remap, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_herk_int4_grouped(const GroupProblemDev* probs, const GroupItemDev* items,
+                                    const int* wg_begin, int num_wg, int cfg, bool out_fp16,
+                                    hipStream_t stream) {
+    if (num_wg <= 0) return hipSuccess;
+    switch (cfg) {
+        case TILE_128x128_K64:
+            return launch_grouped_cfg<Cfg0>(probs, items, wg_begin, num_wg, out_fp16, stream);
+        case TILE_128x128_K128:
+            return launch_grouped_cfg<Cfg1>(probs, items, wg_begin, num_wg, out_fp16, stream);
+        case TILE_64x64_K64:
+            return launch_grouped_cfg<Cfg2>(probs, items, wg_begin, num_wg, out_fp16, stream);
+        case TILE_256x128_K64:
+            return launch_grouped_cfg<Cfg3>(probs, items, wg_begin, num_wg, out_fp16, stream);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace ftdx

@@ -1,0 +1,310 @@
+// herk_tile.h -- one lower-triangle tile of C = alpha * A A^H + beta * C on gfx950.
+//
+// Shared by the plain batched kernel and the grouped persistent kernel.
+//
+// Operand formulation (4 real products per complex MAC, the "4M" count the
+// reference's direct kernel also executes, herk_kernel_common.hpp:29-32):
+//   staged row a_i over 4 samples = [fp8 re0..re3 | fp8 im0..im3]  (u_i, two dwords)
+//   conjugate partner             = [im0..im3 | -re0..-re3]        (v_i: dword swap + sign)
+//   Re C_ij = u_i . u_j,   Im C_ij = v_i . u_j
+// The MFMA sums over all 64 bytes of a 32x32x64 step, so any fixed byte order
+// inside a lane is valid as long as A- and B-operands use the same one; the
+// dword-planar order makes v_i four XORs per operand instead of byte permutes.
+//
+// Data flow per stage of KC complex samples:
+//   global QC bytes (16 B / thread / segment) -> VGPR -> unpack to OCP E4M3 ->
+//   LDS (double-buffered, one barrier per stage) -> ds_read_b128 -> MFMA.
+#pragma once
+#include "herk_common.h"
+
+namespace ftdx {
+
+// Tile configuration: TM x TN complex tile, WAVES_M x WAVES_N waves, each wave a
+// (TM/WAVES_M) x (TN/WAVES_N) block of 32x32 MFMA sub-tiles; KC samples per stage.
+// WAVES_PER_EU is the occupancy target given to the compiler (caps VGPRs at
+// 512 / WAVES_PER_EU); WG_PER_CU is how many workgroups fit one CU (LDS- and
+// register-limited), used to size persistent grids.
+template <int TM_, int TN_, int WAVES_M_, int WAVES_N_, int KC_, int WAVES_PER_EU_, int WG_PER_CU_>
+struct TileCfg {
+    static constexpr int TM = TM_, TN = TN_;
+    static constexpr int WAVES_M = WAVES_M_, WAVES_N = WAVES_N_;
+    static constexpr int KC = KC_;
+    static constexpr int WAVES_PER_EU = WAVES_PER_EU_;
+    static constexpr int WG_PER_CU = WG_PER_CU_;
+    static constexpr int WM = TM / WAVES_M, WN = TN / WAVES_N;
+    static constexpr int FM = WM / 32, FN = WN / 32;
+    static constexpr int THREADS = 64 * WAVES_M * WAVES_N;
+    static constexpr int PITCH = 2 * KC + 16;         // LDS bytes per staged row (+16 B pad)
+    static constexpr int STAGE = (TM + TN) * PITCH;   // row block then column block
+    static constexpr int LDS_BYTES = 2 * STAGE;
+    static constexpr int SEGS_PER_ROW = KC / 16;      // 16-byte global segments per row
+    static constexpr int SEGS = (TM + TN) * SEGS_PER_ROW;
+    static constexpr int SEGS_PER_THREAD = SEGS / THREADS;
+    static_assert(WM % 32 == 0 && WN % 32 == 0, "wave block must be a multiple of 32");
+    static_assert(KC % 32 == 0, "stage must hold whole 32-sample MFMA steps");
+    static_assert(SEGS % THREADS == 0, "segments must divide evenly over threads");
+    static_assert(TM % TN == 0, "row tile must be a multiple of the column tile");
+    static_assert(LDS_BYTES * WG_PER_CU <= 160 * 1024, "LDS of resident workgroups exceeds 160 KB");
+};
+
+enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1 };
+
+struct TileJob {
+    const uint8_t* A;  // this problem's [N][K] QC bytes
+    void* C;           // this problem's packed triangle (scalars)
+    int N, K;
+    int row0, col0;    // first row / column of the tile
+    float alpha, beta;
+};
+
+// 4 QC bytes (4 samples) -> u dwords: re = fp8(Re0..Re3), im = fp8(Im0..Im3).
+__device__ __forceinline__ void qc4_to_fp8_planar(uint32_t x, uint32_t& re, uint32_t& im) {
+    const uint32_t TLO = 0x44403800u;  // fp8(0..3)
+    const uint32_t THI = 0x4E4C4A48u;  // fp8(4..7)
+    uint32_t m_im = x & 0x07070707u;
+    uint32_t m_re = (x >> 4) & 0x07070707u;
+    im = __builtin_amdgcn_perm(THI, TLO, m_im) | ((x << 4) & 0x80808080u);
+    re = __builtin_amdgcn_perm(THI, TLO, m_re) | (x & 0x80808080u);
+}
+
+// v from u for one 32-byte operand: [re | im] -> [im | -re] per dword pair.
+__device__ __forceinline__ v8i conj_planar(v8i a) {
+    v8i r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        r[2 * q] = a[2 * q + 1];
+        r[2 * q + 1] = a[2 * q] ^ (int)0x80808080u;
+    }
+    return r;
+}
+
+template <typename OutT>
+struct OutIO;
+
+typedef float v2f __attribute__((ext_vector_type(2)));
+
+// Output is written once and never re-read by the kernel: stores are
+// non-temporal so the L2 keeps the re-read input rows instead.
+template <>
+struct OutIO<float> {
+    static __device__ __forceinline__ void load(const void* C, int64_t idx, float& re, float& im) {
+        v2f v = reinterpret_cast<const v2f*>(C)[idx];
+        re = v[0];
+        im = v[1];
+    }
+    static __device__ __forceinline__ void store(void* C, int64_t idx, float re, float im) {
+        v2f v;
+        v[0] = re;
+        v[1] = im;
+        __builtin_nontemporal_store(v, reinterpret_cast<v2f*>(C) + idx);
+    }
+};
+
+template <>
+struct OutIO<_Float16> {
+    static __device__ __forceinline__ void load(const void* C, int64_t idx, float& re, float& im) {
+        uint32_t w = reinterpret_cast<const uint32_t*>(C)[idx];
+        uint16_t lo = (uint16_t)(w & 0xFFFFu), hi = (uint16_t)(w >> 16);
+        _Float16 h0, h1;
+        __builtin_memcpy(&h0, &lo, 2);
+        __builtin_memcpy(&h1, &hi, 2);
+        re = (float)h0;
+        im = (float)h1;
+    }
+    static __device__ __forceinline__ void store(void* C, int64_t idx, float re, float im) {
+        _Float16 h0 = (_Float16)re, h1 = (_Float16)im;  // round to nearest even
+        uint16_t lo, hi;
+        __builtin_memcpy(&lo, &h0, 2);
+        __builtin_memcpy(&hi, &h1, 2);
+        __builtin_nontemporal_store((uint32_t)lo | ((uint32_t)hi << 16),
+                                    reinterpret_cast<uint32_t*>(C) + idx);
+    }
+};
+
+template <class Cfg>
+struct TileRunner {
+    // Staged-row source for segment slot q of thread tid: global row, LDS row.
+    static __device__ __forceinline__ void seg_coords(int q, int tid, int& lrow, int& seg) {
+        const int s = tid + Cfg::THREADS * q;
+        lrow = s / Cfg::SEGS_PER_ROW;  // 0 .. TM+TN-1 (rows first, then columns)
+        seg = s % Cfg::SEGS_PER_ROW;
+    }
+
+    template <int MODE>
+    static __device__ __forceinline__ void load_stage(const TileJob& j, int k0, bool share_cols,
+                                                      int tid, uint4 (&g)[Cfg::SEGS_PER_THREAD]) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS_PER_THREAD; ++q) {
+            int lrow, seg;
+            seg_coords(q, tid, lrow, seg);
+            const bool is_col = lrow >= Cfg::TM;
+            const int grow = is_col ? j.col0 + (lrow - Cfg::TM) : j.row0 + lrow;
+            const int kk = k0 + seg * 16;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (!(is_col && share_cols) && grow < j.N) {
+                const uint8_t* src = j.A + (int64_t)grow * j.K + kk;
+                if (MODE == LOAD_VEC16) {
+                    if (kk < j.K) v = *reinterpret_cast<const uint4*>(src);
+                } else {
+                    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                    for (int e = 0; e < 16; ++e)
+                        if (kk + e < j.K) w[e >> 2] |= (uint32_t)src[e] << (8 * (e & 3));
+                    v = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+            }
+            g[q] = v;
+        }
+    }
+
+    static __device__ __forceinline__ void store_stage(uint8_t* st, bool share_cols, int tid,
+                                                       const uint4 (&g)[Cfg::SEGS_PER_THREAD]) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS_PER_THREAD; ++q) {
+            int lrow, seg;
+            seg_coords(q, tid, lrow, seg);
+            if (lrow >= Cfg::TM && share_cols) continue;
+            uint4 lo, hi;
+            qc4_to_fp8_planar(g[q].x, lo.x, lo.y);
+            qc4_to_fp8_planar(g[q].y, lo.z, lo.w);
+            qc4_to_fp8_planar(g[q].z, hi.x, hi.y);
+            qc4_to_fp8_planar(g[q].w, hi.z, hi.w);
+            uint8_t* dst = st + lrow * Cfg::PITCH + seg * 32;
+            *reinterpret_cast<uint4*>(dst) = lo;
+            *reinterpret_cast<uint4*>(dst + 16) = hi;
+        }
+    }
+
+    static __device__ __forceinline__ v8i lds_read32(const uint8_t* p) {
+        uint4 a = *reinterpret_cast<const uint4*>(p);
+        uint4 b = *reinterpret_cast<const uint4*>(p + 16);
+        v8i r;
+        r[0] = (int)a.x; r[1] = (int)a.y; r[2] = (int)a.z; r[3] = (int)a.w;
+        r[4] = (int)b.x; r[5] = (int)b.y; r[6] = (int)b.z; r[7] = (int)b.w;
+        return r;
+    }
+
+    // Runs one tile; all threads of the workgroup must call it (it uses barriers).
+    // Ends with a barrier, so the caller may reuse `lds` right away.
+    template <typename OutT, int MODE>
+    static __device__ __forceinline__ void run(const TileJob& j, uint8_t* __restrict__ lds) {
+        const int tid = threadIdx.x;
+        const int lane = tid & 63, wave = tid >> 6;
+        const int wm = wave / Cfg::WAVES_N, wn = wave % Cfg::WAVES_N;
+        const int wrow0 = j.row0 + wm * Cfg::WM;  // first row of this wave's block
+        const int wcol0 = j.col0 + wn * Cfg::WN;
+        // The column block lies inside the row block (diagonal tiles): read it from the
+        // row region instead of loading it twice.
+        const bool share_cols = (j.col0 >= j.row0) && (j.col0 + Cfg::TN <= j.row0 + Cfg::TM);
+        // Wave has output iff some (i >= j) with i < N in its block.
+        const bool active = (wrow0 < j.N) && (wrow0 + Cfg::WM - 1 >= wcol0);
+
+        v16f acc_re[Cfg::FM][Cfg::FN], acc_im[Cfg::FM][Cfg::FN];
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    acc_re[a][b][r] = 0.0f;
+                    acc_im[a][b][r] = 0.0f;
+                }
+
+        const int nst = (j.K + Cfg::KC - 1) / Cfg::KC;
+        uint4 g[Cfg::SEGS_PER_THREAD];
+        if (nst > 0) {
+            load_stage<MODE>(j, 0, share_cols, tid, g);
+            store_stage(lds, share_cols, tid, g);
+        }
+        __syncthreads();
+
+        const int lr = lane & 31, lh = (lane >> 5) * 32;
+        const int a_off = (wm * Cfg::WM + lr) * Cfg::PITCH + lh;
+        const int b_off = (share_cols ? (j.col0 - j.row0) : Cfg::TM) * Cfg::PITCH +
+                          (wn * Cfg::WN + lr) * Cfg::PITCH + lh;
+
+        for (int s = 0; s < nst; ++s) {
+            const bool more = (s + 1 < nst);
+            if (more) load_stage<MODE>(j, (s + 1) * Cfg::KC, share_cols, tid, g);
+            const uint8_t* st = lds + (s & 1) * Cfg::STAGE;
+            if (active) {
+#pragma unroll
+                for (int ks = 0; ks < Cfg::KC / 32; ++ks) {
+                    v8i au[Cfg::FM], bu[Cfg::FN];
+#pragma unroll
+                    for (int a = 0; a < Cfg::FM; ++a)
+                        au[a] = lds_read32(st + a_off + a * 32 * Cfg::PITCH + ks * 64);
+#pragma unroll
+                    for (int b = 0; b < Cfg::FN; ++b)
+                        bu[b] = lds_read32(st + b_off + b * 32 * Cfg::PITCH + ks * 64);
+#pragma unroll
+                    for (int a = 0; a < Cfg::FM; ++a) {
+                        const v8i av = conj_planar(au[a]);
+#pragma unroll
+                        for (int b = 0; b < Cfg::FN; ++b) {
+                            acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
+                            acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
+                        }
+                    }
+                }
+            }
+            if (more) store_stage(lds + ((s + 1) & 1) * Cfg::STAGE, share_cols, tid, g);
+            __syncthreads();
+        }
+
+        if (!active) return;
+
+        // Epilogue. 32x32 accumulator layout: lane holds column (lane & 31); register
+        // r holds row 8*(r/4) + 4*(lane/32) + (r%4).  Re and Im of one (i, j) sit in
+        // the same lane and register, so each lane writes interleaved [re, im].
+        const float alpha = j.alpha, beta = j.beta;
+        const int n = lane & 31, g4 = (lane >> 5) * 4;
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b) {
+                const int gj = wcol0 + b * 32 + n;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int gi = wrow0 + a * 32 + (r >> 2) * 8 + g4 + (r & 3);
+                    if (gi < j.N && gj <= gi) {
+                        const int64_t idx = tri_index(gi, gj);
+                        float re = acc_re[a][b][r];
+                        float im = (gi == gj) ? 0.0f : acc_im[a][b][r];
+                        re = re * alpha;
+                        im = im * alpha;
+                        if (beta != 0.0f) {
+                            float ore, oim;
+                            OutIO<OutT>::load(j.C, idx, ore, oim);
+                            re = __builtin_fmaf(beta, ore, re);
+                            im = __builtin_fmaf(beta, oim, im);
+                        }
+                        OutIO<OutT>::store(j.C, idx, re, im);
+                    }
+                }
+            }
+    }
+};
+
+// Lower-triangle tiles of an N x N problem for TM x TN tiles (TM = r * TN):
+// row block I holds column blocks J with J*TN <= (I+1)*TM - 1, capped at nbN.
+__host__ __device__ inline int tri_tiles(int N, int TM, int TN) {
+    const int nbM = (N + TM - 1) / TM, nbN = (N + TN - 1) / TN, r = TM / TN;
+    // sum over I of min(nbN, r*(I+1))
+    int64_t t = 0;
+    for (int I = 0; I < nbM; ++I) t += (r * (I + 1) < nbN) ? r * (I + 1) : nbN;
+    return (int)t;
+}
+
+// tile index t -> (I, J) for TM = r * TN (all row blocks but the last hold r*(I+1)).
+__device__ __forceinline__ void tri_decode_r(int t, int r, int& I, int& J) {
+    // uncapped count before row block I: r * I (I + 1) / 2
+    const float q = (float)t / (float)r;
+    int i = (int)((__builtin_sqrtf(8.0f * q + 1.0f) - 1.0f) * 0.5f);
+    while (i > 0 && r * i * (i + 1) / 2 > t) --i;
+    while (r * (i + 1) * (i + 2) / 2 <= t) ++i;
+    I = i;
+    J = t - r * i * (i + 1) / 2;
+}
+
+}  // namespace ftdx

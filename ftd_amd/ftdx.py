@@ -98,7 +98,30 @@ _SIGNATURES = {
     "ftdx_set_kernel_tune_cache_path": (_I, [_VP, _S]),
     "ftdx_print_build_info": (_I, []),
     "ftdx_herk_int4": (_I, [_VP, _VP, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_set_launch_config": (_I, [_VP, _I, _I]),
+    "ftdx_tune_load": (_I, [_VP, _S, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ftdx_tune_save": (_I, [_VP, _S]),
+    "ftdx_herk_query_config": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ftdx_herk_grouped_plan_create": (_I, [_VP, _VP, _I, _I, _I, _I, ctypes.POINTER(_VP)]),
+    "ftdx_herk_grouped_plan_run": (_I, [_VP, _VP]),
+    "ftdx_herk_grouped_plan_info": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
+    "ftdx_herk_grouped_plan_destroy": (_I, [_VP]),
+    "ftdx_herk_each": (_I, [_VP, _VP, _I, _I, _VP]),
 }
+
+TILE_AUTO = -1
+TILE_128x128_K64 = 0
+TILE_128x128_K128 = 1
+TILE_64x64_K64 = 2
+TILE_256x128_K64 = 3
+TILE_CONFIGS = (TILE_128x128_K64, TILE_128x128_K128, TILE_64x64_K64, TILE_256x128_K64)
+
+
+class HerkProblem(ctypes.Structure):
+    """ftdx_herk_problem (include/ftdx.h)."""
+    _fields_ = [("A", ctypes.c_void_p), ("C", ctypes.c_void_p), ("N", ctypes.c_int),
+                ("K", ctypes.c_int), ("alpha", ctypes.c_float), ("beta", ctypes.c_float)]
 
 
 def declared_functions(header: str = HEADER_PATH) -> list[str]:
@@ -237,6 +260,81 @@ class CutlassComplexGemm:
     @staticmethod
     def print_build_info() -> None:
         lib().ftdx_print_build_info()
+
+    # ---- gfx950 launch configuration and tuning cache ----
+    def set_launch_config(self, tile_config: int = -1, xcd_remap: int = -1) -> None:
+        _check(lib().ftdx_set_launch_config(self._h, tile_config, xcd_remap), "set_launch_config")
+
+    def query_config(self, N: int, K: int, batch: int, output=OutputPrecision.FP32):
+        c, r = ctypes.c_int(), ctypes.c_int()
+        _check(lib().ftdx_herk_query_config(self._h, N, K, batch, int(output), ctypes.byref(c),
+                                            ctypes.byref(r)), "query_config")
+        return c.value, r.value
+
+    def tune_load(self, path: str):
+        n, sk = ctypes.c_int(), ctypes.c_int()
+        _check(lib().ftdx_tune_load(self._h, path.encode(), ctypes.byref(n), ctypes.byref(sk)),
+               "tune_load")
+        return n.value, sk.value
+
+    def tune_save(self, path: str) -> None:
+        _check(lib().ftdx_tune_save(self._h, path.encode()), "tune_save")
+
+
+def problem_array(problems):
+    """ctypes array of ftdx_herk_problem from (A, C, N, K[, alpha, beta]) tuples."""
+    arr = (HerkProblem * max(1, len(problems)))()
+    for i, p in enumerate(problems):
+        A, C, N, K = p[0], p[1], p[2], p[3]
+        alpha = p[4] if len(p) > 4 else 1.0
+        beta = p[5] if len(p) > 5 else 0.0
+        arr[i] = HerkProblem(_ptr(A) if A is not None else None, _ptr(C), N, K, alpha, beta)
+    return arr
+
+
+def herk_each(ctx: "CutlassComplexGemm", problems, output=None, stream=None, arr=None) -> None:
+    """One plain HERK launch per problem, in order (ftdx_herk_each)."""
+    output = OutputPrecision.FP32 if output is None else output
+    arr = problem_array(problems) if arr is None else arr
+    _check(lib().ftdx_herk_each(ctx.handle, arr, len(problems), int(output),
+                                _stream_ptr(stream)), "herk_each")
+
+
+class GroupedPlan:
+    """Grouped HERK plan over many (A, C, N, K, alpha, beta) problems
+    (ftdx_herk_grouped_plan_*).  Keeps references to the tensors it points at."""
+
+    def __init__(self, ctx: CutlassComplexGemm, problems, output=OutputPrecision.FP32,
+                 tile_config: int = TILE_AUTO, use_graph: bool = True):
+        arr = problem_array(problems)
+        self._keep = [t for p in problems for t in (p[0], p[1])]
+        h = ctypes.c_void_p()
+        _check(lib().ftdx_herk_grouped_plan_create(ctx.handle, arr, len(problems), int(output),
+                                                    tile_config, int(bool(use_graph)),
+                                                    ctypes.byref(h)), "grouped_plan_create")
+        self._h = h
+
+    def run(self, stream=None) -> None:
+        _check(lib().ftdx_herk_grouped_plan_run(self._h, _stream_ptr(stream)), "grouped_plan_run")
+
+    def info(self) -> dict:
+        g, n, c, imb = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().ftdx_herk_grouped_plan_info(self._h, ctypes.byref(g), ctypes.byref(n),
+                                                  ctypes.byref(c), ctypes.byref(imb)),
+               "grouped_plan_info")
+        return {"workgroups": g.value, "items": n.value, "tile_config": c.value,
+                "imbalance": imb.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ftdx_herk_grouped_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def herk_int4(A, N: int, K: int, batch: int, C=None, output: str = "fp32",

@@ -67,7 +67,27 @@ extern "C" {
 #define FTDX_DIRECT_TILE_K64_B3 3
 #define FTDX_DIRECT_TILE_K128_B2 4
 
+/* Tile configurations of the gfx950 HERK kernel (rows x cols complex per
+ * workgroup tile, samples per LDS stage); FTDX_TILE_AUTO lets the library choose. */
+#define FTDX_TILE_AUTO (-1)
+#define FTDX_TILE_128x128_K64 0  /* 4 waves, 2 workgroups per CU */
+#define FTDX_TILE_128x128_K128 1 /* 4 waves, 1 workgroup per CU */
+#define FTDX_TILE_64x64_K64 2    /* 1 wave, 4 workgroups per CU */
+#define FTDX_TILE_256x128_K64 3  /* 8 waves, 1 workgroup per CU */
+
 typedef struct ftdx_context ftdx_context;
+typedef struct ftdx_grouped_plan ftdx_grouped_plan;
+
+/* One problem of a grouped HERK: C = alpha * A A^H + beta * C with A [N][K] QC
+ * bytes and C the packed lower triangle (N(N+1) scalars). */
+typedef struct ftdx_herk_problem {
+    const void* A;
+    void* C;
+    int N;
+    int K;
+    float alpha;
+    float beta;
+} ftdx_herk_problem;
 
 /* Message of the last failure on this thread ("" if none). */
 const char* ftdx_last_error(void);
@@ -100,6 +120,45 @@ int ftdx_set_tune_cache_path(ftdx_context* ctx, const char* path);
 int ftdx_set_strategy_tune_verbosity(ftdx_context* ctx, int level);
 int ftdx_set_kernel_tune_verbosity(ftdx_context* ctx, int level);
 int ftdx_set_kernel_tune_cache_path(ftdx_context* ctx, const char* path);
+
+/* Explicit launch configuration for ftdx_herk (tile_config = FTDX_TILE_*,
+ * xcd_remap = 0/1, either -1 = choose automatically).  Overrides
+ * set_direct_tile_config and the tuning cache while set. */
+int ftdx_set_launch_config(ftdx_context* ctx, int tile_config, int xcd_remap);
+
+/* Tuning cache I/O (replaces the reference's strategy/kernel cache files,
+ * cutlass_interface/strategy_cache.hpp:13-27, tune_cache.hpp:22-37).  Entries
+ * carry a build fingerprint; lines of another build or malformed lines are
+ * skipped.  `loaded` / `skipped` may be NULL. */
+int ftdx_tune_load(ftdx_context* ctx, const char* path, int* loaded, int* skipped);
+int ftdx_tune_save(ftdx_context* ctx, const char* path);
+/* Launch configuration ftdx_herk would use now for this shape without tuning
+ * (explicit setting, else cache entry, else the default). */
+int ftdx_herk_query_config(ftdx_context* ctx, int N, int K, int batch_count, int output,
+                           int* tile_config, int* xcd_remap);
+
+/* Grouped HERK: many problems of different N and K in one persistent launch.
+ * No reference counterpart (the reference loops over uniform batched calls,
+ * lib/visibility_pipeline.cu:484-540); it serves sub-arrays and channel groups
+ * of uneven size.  A plan uploads the problem list and a static tile schedule;
+ * with use_graph != 0 the launch is captured once in a hipGraph and replayed.
+ * tile_config = FTDX_TILE_AUTO times the configurations on the problems' own
+ * buffers when every beta is 0 (C is overwritten), else uses the default.
+ * All outputs share one precision.  The plan keeps the pointers: buffers must
+ * outlive it. */
+int ftdx_herk_grouped_plan_create(ftdx_context* ctx, const ftdx_herk_problem* problems,
+                                  int count, int output, int tile_config, int use_graph,
+                                  ftdx_grouped_plan** plan);
+int ftdx_herk_grouped_plan_run(ftdx_grouped_plan* plan, void* stream);
+int ftdx_herk_grouped_plan_info(const ftdx_grouped_plan* plan, int* num_workgroups,
+                                int64_t* num_items, int* tile_config, double* imbalance);
+int ftdx_herk_grouped_plan_destroy(ftdx_grouped_plan* plan);
+
+/* The same problem list run the plain way: one ftdx_herk launch per problem, in
+ * order, on one stream (the per-sub-array loop of the reference's callers,
+ * lib/visibility_pipeline.cu:484-540).  Output bytes equal the grouped plan's. */
+int ftdx_herk_each(ftdx_context* ctx, const ftdx_herk_problem* problems, int count, int output,
+                   void* stream);
 
 /* Replaces CutlassComplexGemm::print_build_info (cutlass_gemm_api.h:773-775). */
 int ftdx_print_build_info(void);

@@ -64,3 +64,42 @@ def test_cpp_wrapper_exports():
                 "ggp::initCutlassApi(", "ggp::herkBatchedCutlassQC(",
                 "ggp::herkBatchedCutlassQC_FP16("]:
         assert sym in out, sym
+
+
+def test_tune_cache_file_parsing(tmp_path):
+    """Malformed lines and lines of another build fingerprint are skipped
+    (the reference's cache parser, strategy_cache.hpp:143-420, skips bad lines)."""
+    from ftd_amd import ftdx
+    ctx = ftdx.CutlassComplexGemm()  # no device needed for settings / cache I/O
+    good = tmp_path / "good.txt"
+    ctx.tune_save(str(good))
+    header = open(good).read().strip()
+    assert header.startswith("# ftdx-tune gfx950-abi1")
+    f = tmp_path / "cache.txt"
+    f.write_text("\n".join([
+        header,
+        "herk\t4096\t2048\t96\t0\t1\t3\t1\t7.5",
+        "herk\t128\t1024\t16\t0\t1\t0\t0\t0.1",
+        "herk\t1\t2",                                  # too short
+        "herk\t10\t10\t1\t0\t1\t9\t0\t1.0",            # unknown tile config
+        "gemm\t10\t10\t1\t0\t1\t0\t0\t1.0",            # other kind
+        "herk\t10\t10\t1\t0\t1\t0\t0\t1.0\textra",     # trailing field
+        "# ftdx-tune some-other-build",
+        "herk\t64\t64\t1\t0\t1\t1\t0\t0.01",           # foreign fingerprint
+    ]) + "\n")
+    n, skipped = ctx.tune_load(str(f))
+    assert (n, skipped) == (2, 5)
+    assert ctx.query_config(4096, 2048, 96) == (3, 1)
+    assert ctx.query_config(128, 1024, 16) == (0, 0)
+    assert ctx.query_config(64, 64, 1) == (0, 1)       # default
+    ctx.set_launch_config(2, 0)
+    assert ctx.query_config(4096, 2048, 96) == (2, 0)  # explicit setting wins
+    ctx.set_launch_config(-1, -1)
+    ctx.set_direct_tile_config(ftdx.DirectTileConfig.K128_B2)
+    assert ctx.query_config(4096, 2048, 96)[0] == 1
+    out = tmp_path / "round.txt"
+    ctx.tune_save(str(out))
+    ctx2 = ftdx.CutlassComplexGemm()
+    assert ctx2.tune_load(str(out)) == (2, 0)
+    with pytest.raises(ftdx.FtdxError):
+        ctx2.tune_load(str(tmp_path / "missing.txt"))

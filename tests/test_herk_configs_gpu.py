@@ -1,0 +1,169 @@
+"""GPU parity of every compiled tile configuration and workgroup order, of the
+grouped persistent launch (with and without hipGraph replay) and of the
+autotuner path; all bit-exact against the CPU oracle."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32 if a.dtype == np.float32 else np.uint16)
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("remap", [0, 1])
+@pytest.mark.parametrize("N,K,B", [(1, 1, 2), (63, 70, 3), (129, 64, 2), (257, 100, 1),
+                                   (300, 256, 2), (520, 128, 1)])
+def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B):
+    import torch
+    from ftd_amd import ftdx
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, remap)
+    A_np = oracle.gen_qc_uniform(31 * N + K + cfg, N * K * B)
+    A = torch.from_numpy(A_np).to(dev)
+    C = torch.full((B, N * (N + 1) // 2, 2), float("nan"), dtype=torch.float32, device=dev)
+    assert g.herk(A, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                  ftdx.OutputPrecision.FP32, tune=False) == 0
+    torch.cuda.synchronize()
+    ref = oracle.herk_f32(A_np, N, K, B)
+    assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref))
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3])
+def test_tile_configs_fp16_beta(oracle, dev, cfg):
+    import torch
+    from ftd_amd import ftdx
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, 1)
+    N, K, B = 200, 96, 2
+    A_np = oracle.gen_qc_uniform(5 + cfg, N * K * B)
+    rng = np.random.default_rng(cfg)
+    C0 = rng.integers(-100, 100, size=(B, N * (N + 1) // 2, 2)).astype(np.float16)
+    A = torch.from_numpy(A_np).to(dev)
+    C = torch.from_numpy(C0).to(dev)
+    assert g.herk(A, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                  ftdx.OutputPrecision.FP16, alpha=0.5, beta=2.0, tune=False) == 0
+    torch.cuda.synchronize()
+    ref = oracle.herk_f16(A_np, N, K, B, alpha=0.5, beta=2.0, C=C0)
+    assert np.array_equal(_bits(C.cpu().numpy()), ref)
+
+
+def _grouped_case(oracle, dev, shapes, seed):
+    import torch
+    probs, refs, inputs = [], [], []
+    for i, (N, K) in enumerate(shapes):
+        A_np = oracle.gen_qc_uniform(seed * 1000 + i, N * K)
+        A = torch.from_numpy(A_np).to(dev)
+        C = torch.full((N * (N + 1) // 2, 2), float("nan"), dtype=torch.float32, device=dev)
+        probs.append((A, C, N, K, 1.0, 0.0))
+        inputs.append(A_np)
+        refs.append(oracle.herk_f32(A_np, N, K, 1)[0])
+    return probs, refs
+
+
+@pytest.mark.parametrize("cfg", [-1, 0, 1, 2, 3])
+@pytest.mark.parametrize("graph", [False, True])
+def test_grouped_uneven_bit_exact(oracle, dev, cfg, graph):
+    import torch
+    from ftd_amd import ftdx
+    shapes = [(256, 320), (64, 64), (130, 1000), (512, 256), (33, 17), (1, 5), (300, 64),
+              (128, 4096), (200, 2048), (96, 100)]
+    probs, refs = _grouped_case(oracle, dev, shapes, 7)
+    ctx = ftdx.CutlassComplexGemm()
+    plan = ftdx.GroupedPlan(ctx, probs, tile_config=cfg, use_graph=graph)
+    info = plan.info()
+    assert info["items"] > 0 and info["workgroups"] > 0
+    for rep in range(3):  # replays give the same bytes
+        for p in probs:
+            p[1].fill_(float("nan"))
+        plan.run()
+        torch.cuda.synchronize()
+        for (A, C, N, K, _, _), ref in zip(probs, refs):
+            assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (rep, N, K)
+    plan.close()
+
+
+def test_grouped_alpha_beta_and_unaligned(oracle, dev):
+    import torch
+    from ftd_amd import ftdx
+    probs, refs = [], []
+    rng = np.random.default_rng(2)
+    for i, (N, K, alpha, beta, off) in enumerate([(100, 64, 0.5, 1.0, 0), (70, 33, -1.0, 0.25, 3),
+                                                  (130, 48, 2.0, 0.0, 1)]):
+        A_np = oracle.gen_qc_uniform(900 + i, N * K)
+        buf = torch.zeros(N * K + 16, dtype=torch.uint8, device=dev)
+        A = buf[off:off + N * K]
+        A.copy_(torch.from_numpy(A_np))
+        C0 = rng.integers(-50, 50, size=(N * (N + 1) // 2, 2)).astype(np.float32)
+        C = torch.from_numpy(C0).to(dev)
+        probs.append((A, C, N, K, alpha, beta))
+        refs.append(oracle.herk_f32(A_np, N, K, 1, alpha=alpha, beta=beta, C=C0[None])[0])
+    ctx = ftdx.CutlassComplexGemm()
+    plan = ftdx.GroupedPlan(ctx, probs, tile_config=-1, use_graph=True)
+    plan.run()
+    torch.cuda.synchronize()
+    for (A, C, N, K, a, b), ref in zip(probs, refs):
+        assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (N, K, a, b)
+
+
+def test_grouped_matches_plain_loop_bytes(oracle, dev):
+    """A reduced batched-uneven set: grouped output bytes == per-problem plain loop."""
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(3)
+    Ns = rng.choice([256, 512, 1024], size=24)
+    Ks = rng.integers(4, 65, size=24) * 64
+    plain, grouped, probs = [], [], []
+    ctx = ftdx.CutlassComplexGemm()
+    for i, (N, K) in enumerate(zip(Ns.tolist(), Ks.tolist())):
+        A = torch.from_numpy(oracle.gen_qc_uniform(5000 + i, N * K)).to(dev)
+        Cp = torch.empty((N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+        Cg = torch.full_like(Cp, float("nan"))
+        assert ctx.herk(A, Cp, N, K, 1, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                        ftdx.OutputPrecision.FP32, tune=False) == 0
+        plain.append(Cp)
+        grouped.append(Cg)
+        probs.append((A, Cg, N, K))
+    plan = ftdx.GroupedPlan(ctx, probs, use_graph=True)
+    plan.run()
+    torch.cuda.synchronize()
+    for Cp, Cg in zip(plain, grouped):
+        assert torch.equal(Cp.view(torch.int32), Cg.view(torch.int32))
+
+
+def test_autotune_and_cache_file(oracle, dev, tmp_path):
+    import torch
+    from ftd_amd import ftdx
+    N, K, B = 256, 512, 3
+    A_np = oracle.gen_qc_uniform(77, N * K * B)
+    A = torch.from_numpy(A_np).to(dev)
+    C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    path = str(tmp_path / "tune.txt")
+    g = ftdx.CutlassComplexGemm()
+    g.set_strategy_tune_verbosity(0)
+    g.set_tune_cache_path(path)
+    assert g.herk(A, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                  ftdx.OutputPrecision.FP32, tune=True) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(_bits(C.cpu().numpy()), _bits(oracle.herk_f32(A_np, N, K, B)))
+    text = open(path).read()
+    assert text.startswith("# ftdx-tune ") and "\nherk\t256\t512\t3\t0\t1\t" in text
+    cfg, remap = g.query_config(N, K, B)
+    g2 = ftdx.CutlassComplexGemm()
+    n, sk = g2.tune_load(path)
+    assert n == 1 and sk == 0
+    assert g2.query_config(N, K, B) == (cfg, remap)
+    # beta != 0 with tuning: the caller's C is accumulated exactly once
+    C0 = np.ones((B, N * (N + 1) // 2, 2), dtype=np.float32)
+    C2 = torch.from_numpy(C0).to(dev)
+    g3 = ftdx.CutlassComplexGemm()
+    g3.set_strategy_tune_verbosity(0)
+    assert g3.herk(A, C2, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                   ftdx.OutputPrecision.FP32, alpha=1.0, beta=1.0, tune=True) == 0
+    torch.cuda.synchronize()
+    ref = oracle.herk_f32(A_np, N, K, B, alpha=1.0, beta=1.0, C=C0)
+    assert np.array_equal(_bits(C2.cpu().numpy()), _bits(ref))
