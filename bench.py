@@ -25,6 +25,7 @@ if ROOT not in sys.path:
 
 FP8_DENSE_PEAK_TFLOPS = 5033.0   # 256 CU x 2.4 GHz x 8192 FLOP/CU/clk (AMD published, dense)
 HBM_PEAK_GBS = 8000.0
+MX_DENSE_PEAK_TFLOPS = 10066.0   # MXFP6/MXFP4 dense (AMD published), 2x the FP8 rate
 METRIC = "HERK TFLOP/s (4M-equiv) & % FP8 MFMA peak, 4096 inputs; batched vs plain"
 
 
@@ -99,6 +100,137 @@ def batched_vs_plain(torch, ftdx, dev, count: int, seed: int, reps: int) -> dict
         "tile_config": info["tile_config"], "workgroups": info["workgroups"],
         "items": info["items"], "schedule_imbalance": round(info["imbalance"], 4),
         "plan_s": round(t_plan, 3), "graph": True,
+    }
+
+
+def mx_study(torch, ftdx, dev, N: int, K: int, B: int, seed: int, reps: int) -> dict:
+    """MX block-scaled HERK at 4096 inputs x 32 channels x K=4096: complex
+    Gaussian voltages (E|x|^2 = 1) with 1% impulsive 30-sigma outliers of uniform
+    phase, as FP16; FP6 E3M2 and FP4 E2M1 with 32-sample E8M0 scales.  Error per
+    channel vs the FP64 HERK of the same FP16 voltages (complex128 matmul)."""
+    g = torch.Generator(device=dev).manual_seed(seed)
+    x = torch.complex(torch.randn((B, N, K), generator=g, device=dev),
+                      torch.randn((B, N, K), generator=g, device=dev)) * (0.5 ** 0.5)
+    mask = torch.rand((B, N, K), generator=g, device=dev) < 0.01
+    phase = (torch.rand((B, N, K), generator=g, device=dev) * 2 - 1) * torch.pi
+    x = torch.where(mask, 30.0 * torch.exp(1j * phase), x)
+    X16 = torch.view_as_real(x).to(torch.float16).contiguous()
+    del x, mask, phase
+    ii, jj = torch.tril_indices(N, N, device=dev)
+    ctx = ftdx.CutlassComplexGemm()
+    flop = 8.0 * N * N * K * B
+    out = {"inputs": N, "channels": B, "K": K, "seed": seed,
+           "data": "complex Gaussian sigma=1 + 1% 30-sigma outliers, FP16"}
+    for name, comp in (("fp6_e3m2", ftdx.ComputePrecision.FP6), ("fp4_e2m1", ftdx.ComputePrecision.FP4)):
+        C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+        ftdx.herk_mx(ctx, X16, N, K, B, ftdx.InputPrecision.FP16, comp, output="fp32", C=C)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            ftdx.herk_mx(ctx, X16, N, K, B, ftdx.InputPrecision.FP16, comp, output="fp32", C=C)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        fro, mxe = [], []
+        for b in range(B):
+            xb = X16[b].double()
+            xc = torch.complex(xb[..., 0], xb[..., 1])
+            R = (xc @ xc.conj().transpose(0, 1).contiguous())[ii, jj]
+            R = torch.view_as_real(R).clone()
+            R[ii == jj, 1] = 0.0
+            d = C[b].double() - R
+            fro.append(float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(R)))
+            mxe.append(float(d.abs().max() / R.abs().max()))
+            del xb, xc, R, d
+        out[name] = {"ms": round(ms, 4), "tflops_4m_equiv": round(flop / (ms * 1e-3) / 1e12, 2),
+                     "pct_mx_dense_peak": round(100 * flop / (ms * 1e-3) / 1e12 / MX_DENSE_PEAK_TFLOPS, 2),
+                     "rel_frobenius": {"min": min(fro), "median": sorted(fro)[B // 2], "max": max(fro)},
+                     "rel_max_element": {"min": min(mxe), "median": sorted(mxe)[B // 2], "max": max(mxe)}}
+        del C
+    torch.cuda.empty_cache()
+    return out
+
+
+def calibration_study(torch, ftdx, dev, N: int, K: int, B: int, seed: int, reps: int) -> dict:
+    """Differentiable gain calibration at 1024 inputs x 16 channels x K=2048:
+    forward HERK (FP8, exact), gain loss + gradients, HERK adjoint dX = (G+G^H) X
+    (FP32 MFMA), timed; errors vs an FP64 reference computed on the device."""
+    import numpy as np
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    v = torch.randint(-7, 8, (2, B * N * K), generator=gen, device=dev, dtype=torch.int16)
+    nib = torch.where(v < 0, 8 + (-v), v).to(torch.uint8)
+    A = ((nib[0] << 4) | nib[1]).contiguous()
+    Xc = torch.complex(v[0].double(), v[1].double()).reshape(B, N, K)
+    del v, nib
+    ctx = ftdx.CutlassComplexGemm()
+    M = ftdx.herk_int4(A, N, K, B)
+    ii, jj = torch.tril_indices(N, N, device=dev)
+    Mc = torch.view_as_complex(M.double())
+    rng = np.random.default_rng(seed)
+
+    def gains():
+        return (1 + 0.1 * rng.standard_normal((B, N))) * np.exp(1j * np.pi * rng.uniform(-1, 1, (B, N)))
+
+    gt, g = torch.from_numpy(gains()).to(dev), torch.from_numpy(gains()).to(dev)
+    noise = torch.complex(torch.randn(Mc.shape, generator=gen, device=dev, dtype=torch.float64),
+                          torch.randn(Mc.shape, generator=gen, device=dev, dtype=torch.float64))
+    V = gt[:, ii] * gt[:, jj].conj() * Mc + 50.0 * noise
+    V[:, ii == jj] = V[:, ii == jj].real.to(V.dtype)
+    V32 = torch.view_as_real(V).float().contiguous()
+    g32 = torch.view_as_real(g).float().contiguous()
+    V, g = torch.view_as_complex(V32.double()), torch.view_as_complex(g32.double())
+    loss, grad, Gamma = ftdx.gain_loss_grad(ctx, V32, M, g32, N, B)
+    dX = ftdx.herk_adjoint(ctx, Gamma, A, N, K, B)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    for _ in range(reps):
+        ftdx.herk_int4(A, N, K, B, C=M)
+    e[1].record()
+    for _ in range(reps):
+        ftdx.gain_loss_grad(ctx, V32, M, g32, N, B)
+    e[2].record()
+    for _ in range(reps):
+        ftdx.herk_adjoint(ctx, Gamma, A, N, K, B)
+    e[3].record()
+    e[3].synchronize()
+    t_fwd, t_gain, t_adj = (e[i].elapsed_time(e[i + 1]) / reps for i in range(3))
+    R = V - g[:, ii] * g[:, jj].conj() * Mc
+    G64 = -2.0 * R * g[:, ii].conj() * g[:, jj]
+    acc = torch.zeros((B, N, 2), dtype=torch.float64, device=dev)
+    acc.index_add_(1, ii, torch.view_as_real(-2.0 * R * Mc.conj() * g[:, jj]))
+    acc.index_add_(1, jj, torch.view_as_real(-2.0 * R.conj() * Mc * g[:, ii]))
+    grad64 = torch.view_as_complex(acc)
+    H = torch.zeros((B, N, N), dtype=torch.complex128, device=dev)
+    H[:, ii, jj] = G64
+    H = H + H.conj().transpose(1, 2)
+    dX64 = torch.matmul(H, Xc)
+    h = 1e-4
+    fd_err = 0.0
+    for k in (0, N // 2, N - 1):
+        for unit in (1.0, 1j):
+            def Lg(delta):
+                gg = g[0].clone()
+                gg[k] += delta
+                r = V[0] - gg[ii] * gg[jj].conj() * Mc[0]
+                return float((r.abs() ** 2).sum())
+            fd = (Lg(h * unit) - Lg(-h * unit)) / (2 * h)
+            an = float(grad[0, k, 0] if unit == 1.0 else grad[0, k, 1])
+            fd_err = max(fd_err, abs(fd - an) / float(grad64.abs().max()))
+    adj_flop = 8.0 * N * N * K * B
+    return {
+        "inputs": N, "channels": B, "K": K, "seed": seed,
+        "forward_herk_ms": round(t_fwd, 4), "gain_loss_grad_ms": round(t_gain, 4),
+        "herk_adjoint_ms": round(t_adj, 4),
+        "herk_adjoint_tflops": round(adj_flop / (t_adj * 1e-3) / 1e12, 2),
+        "loss_rel_err_vs_fp64": float(((loss - (R.abs() ** 2).sum(-1)).abs()
+                                       / (R.abs() ** 2).sum(-1)).max()),
+        "grad_g_max_rel_err_vs_fp64": float((torch.view_as_complex(grad.double()) - grad64).abs().max()
+                                            / grad64.abs().max()),
+        "dX_max_rel_err_vs_fp64": float((torch.view_as_complex(dX.double()) - dX64).abs().max()
+                                        / dX64.abs().max()),
+        "grad_g_max_err_vs_fp64_central_diff_h1e-4": fd_err,
     }
 
 
@@ -179,7 +311,12 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-mx", action="store_true")
+    ap.add_argument("--no-calib", action="store_true")
     ap.add_argument("--batched-problems", type=int, default=512)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the timed steps, one plain device copy of this many MiB "
+                         "(reference kernel for scaling rocprof byte counters)")
     ap.add_argument("--tile-config", type=int, default=-1,
                     help="-1: autotune the headline shape; else a fixed FTDX_TILE_* id")
     args = ap.parse_args()
@@ -209,7 +346,7 @@ def main() -> int:
         if rc != 0:
             raise RuntimeError(f"herk failed: {rc} {ftdx.last_error()}")
 
-    step(tune=True)  # autotune (cached in the context) outside the timed region
+    step(tune=args.tile_config < 0)  # autotune (cached in the context) outside the timed region
     cfg, remap = ctx.query_config(N, K, B)
     for _ in range(args.warmup):
         step()
@@ -236,6 +373,15 @@ def main() -> int:
     log(f"bench: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms "
         f"(median {per[len(per) // 2]:.3f}), {value:.1f} TFLOP/s 4M-equiv")
 
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20)
+        src = torch.ones(n, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {args.calib_copy_mib} MiB done")
+        del src, dst
+
     batched = None
     if not args.no_batched:
         del C
@@ -245,6 +391,21 @@ def main() -> int:
         log(f"bench: grouped {batched['grouped_ms']:.3f} ms vs plain loop "
             f"{batched['plain_loop_ms']:.3f} ms -> {batched['speedup']:.2f}x, "
             f"identical={batched['identical_output']}")
+
+    mx = None
+    if not args.no_mx:
+        log("bench: MX block-scaled HERK (FP6 / FP4) 4096 x 32 x K=4096 ...")
+        mx = mx_study(torch, ftdx, dev, 4096, 4096, 32, 4, 3)
+        log(f"bench: MX fp6 {mx['fp6_e3m2']['ms']:.3f} ms rel-fro {mx['fp6_e3m2']['rel_frobenius']['max']:.3e}; "
+            f"fp4 {mx['fp4_e2m1']['ms']:.3f} ms rel-fro {mx['fp4_e2m1']['rel_frobenius']['max']:.3e}")
+
+    calib = None
+    if not args.no_calib:
+        log("bench: gain calibration + HERK adjoint 1024 x 16 x K=2048 ...")
+        calib = calibration_study(torch, ftdx, dev, 1024, 2048, 16, 5, 5)
+        log(f"bench: calib fwd {calib['forward_herk_ms']:.3f} ms, gain {calib['gain_loss_grad_ms']:.3f} ms, "
+            f"adjoint {calib['herk_adjoint_ms']:.3f} ms; dX err {calib['dX_max_rel_err_vs_fp64']:.2e}")
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -283,6 +444,8 @@ def main() -> int:
             "tile_config": cfg, "xcd_remap": remap,
         },
         "batched_vs_plain": batched,
+        "mx_block_scaled": mx,
+        "gain_calibration": calib,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

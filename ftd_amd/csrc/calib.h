@@ -1,0 +1,25 @@
+// calib.h -- host-side interface of the gain-calibration and HERK-adjoint kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ftdx {
+
+// Triangle tiles (32x32) per channel and 32-blocks per dimension of the gain kernel.
+int gain_tiles(int N);
+int gain_blocks(int N);
+
+// V, M: packed lower triangles [batch][N(N+1)/2] complex; g: [batch][N].
+// Scratch: rowpart, colpart [batch][gain_blocks(N)][N] complex; losspart
+// [batch][gain_tiles(N)] double.  Outputs: loss [batch] (double), grad [batch][N],
+// Gamma (packed, may be null).
+hipError_t launch_gain_loss_grad(const float2* V, const float2* M, const float2* g, int N, int batch,
+                                 float2* Gamma, float2* rowpart, float2* colpart, double* losspart,
+                                 double* loss, float2* grad, hipStream_t s);
+
+// dX [batch][N][K] complex = (Gamma + Gamma^H) X with X QC bytes [batch][N][K];
+// H is scratch [batch][N][N] complex.
+hipError_t launch_herk_adjoint(const float2* Gamma, const uint8_t* X, float2* H, float2* dX, int N,
+                               int K, int batch, hipStream_t s);
+
+}  // namespace ftdx

@@ -17,6 +17,8 @@
 #include "ftdx_internal.h"
 #include "ftdx_tune.h"
 #include "herk_fp8.h"
+#include "herk_mx.h"
+#include "calib.h"
 
 struct ftdx_context {
     int device = 0;
@@ -32,6 +34,16 @@ struct ftdx_context {
     bool tune_loaded = false;
     ftdx::TuneCache tune;
     int init_N = 0, init_K = 0, init_batch = 0;
+    // MX path scratch (quantized codes and scales), grown on demand
+    uint8_t* mx_codes = nullptr;
+    size_t mx_codes_cap = 0;
+    uint8_t* mx_scales = nullptr;
+    size_t mx_scales_cap = 0;
+    // calibration scratch: gradient partials, loss partials, dense Hermitian matrix
+    uint8_t* cal_part = nullptr;
+    size_t cal_part_cap = 0;
+    uint8_t* cal_herm = nullptr;
+    size_t cal_herm_cap = 0;
 };
 
 namespace ftdx {
@@ -171,6 +183,25 @@ int autotune(ftdx_context* ctx, const HerkPlainArgs& a0, bool out_fp16, hipStrea
     return FTDX_OK;
 }
 
+// Grow-only device scratch; the old buffer is released after the device is idle.
+int ensure_scratch(uint8_t*& p, size_t& cap, size_t need, const char* what) {
+    if (need <= cap && p) return FTDX_OK;
+    hipError_t e;
+    if (p) {
+        if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_fail(e, what);
+        (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    if ((e = hipMalloc(&p, need > 0 ? need : 1)) != hipSuccess) return hip_fail(e, what);
+    cap = need;
+    return FTDX_OK;
+}
+
+int mx_format(int compute) {
+    return compute == FTDX_COMPUTE_FP4 ? MX_FP4_E2M1 : MX_FP6_E3M2;
+}
+
 }  // namespace
 
 }  // namespace ftdx
@@ -308,6 +339,157 @@ int ftdx_init_herk(ftdx_context* ctx, int N, int K, int batch_count, void* strea
 int ftdx_end_herk(ftdx_context* ctx) {
     if (!ctx) return FTDX_OK;
     ctx->init_N = ctx->init_K = ctx->init_batch = 0;
+    if (ctx->mx_codes || ctx->mx_scales || ctx->cal_part || ctx->cal_herm) {
+        (void)hipDeviceSynchronize();
+        if (ctx->mx_codes) (void)hipFree(ctx->mx_codes);
+        if (ctx->mx_scales) (void)hipFree(ctx->mx_scales);
+        if (ctx->cal_part) (void)hipFree(ctx->cal_part);
+        if (ctx->cal_herm) (void)hipFree(ctx->cal_herm);
+    }
+    ctx->mx_codes = ctx->mx_scales = nullptr;
+    ctx->mx_codes_cap = ctx->mx_scales_cap = 0;
+    ctx->cal_part = ctx->cal_herm = nullptr;
+    ctx->cal_part_cap = ctx->cal_herm_cap = 0;
+    return FTDX_OK;
+}
+
+int ftdx_gain_loss_grad(ftdx_context* ctx, const void* V, const void* M, const void* g, int N,
+                        int batch_count, void* loss, void* grad_g, void* Gamma, void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: NULL context");
+    if (N < 0 || batch_count < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: negative size");
+    if (N == 0 || batch_count == 0) return FTDX_OK;
+    if (!V || !M || !g || !loss || !grad_g)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: NULL pointer");
+    const uintptr_t al = reinterpret_cast<uintptr_t>(V) | reinterpret_cast<uintptr_t>(M) |
+                         reinterpret_cast<uintptr_t>(g) | reinterpret_cast<uintptr_t>(loss) |
+                         reinterpret_cast<uintptr_t>(grad_g) | reinterpret_cast<uintptr_t>(Gamma);
+    if (al & 7u) return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: buffers must be 8-byte aligned");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "ftdx_gain_loss_grad: hipGetDevice");
+    if (dev != ctx->device)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: context belongs to another device (or none)");
+    const int nb = ftdx::gain_blocks(N), tiles = ftdx::gain_tiles(N);
+    const size_t part = (size_t)batch_count * nb * N * sizeof(float2);
+    const size_t lossp = (size_t)batch_count * tiles * sizeof(double);
+    int rc = ftdx::ensure_scratch(ctx->cal_part, ctx->cal_part_cap, 2 * part + lossp,
+                                  "ftdx_gain_loss_grad: scratch");
+    if (rc != FTDX_OK) return rc;
+    float2* rowpart = reinterpret_cast<float2*>(ctx->cal_part);
+    float2* colpart = reinterpret_cast<float2*>(ctx->cal_part + part);
+    double* losspart = reinterpret_cast<double*>(ctx->cal_part + 2 * part);
+    e = ftdx::launch_gain_loss_grad(static_cast<const float2*>(V), static_cast<const float2*>(M),
+                                    static_cast<const float2*>(g), N, batch_count,
+                                    static_cast<float2*>(Gamma), rowpart, colpart, losspart,
+                                    static_cast<double*>(loss), static_cast<float2*>(grad_g),
+                                    static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "ftdx_gain_loss_grad: launch");
+    return FTDX_OK;
+}
+
+int ftdx_herk_adjoint(ftdx_context* ctx, const void* Gamma, const void* X, void* dX, int N, int K,
+                      int batch_count, int input, void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_adjoint: NULL context");
+    if (input != FTDX_INPUT_INT4)
+        return fail(FTDX_ERR_UNSUPPORTED, "ftdx_herk_adjoint: only INT4 (QC) voltages are supported");
+    if (N < 0 || K < 0 || batch_count < 0)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_adjoint: negative size");
+    if (N == 0 || K == 0 || batch_count == 0) return FTDX_OK;
+    if (!Gamma || !X || !dX) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_adjoint: NULL pointer");
+    if ((reinterpret_cast<uintptr_t>(Gamma) | reinterpret_cast<uintptr_t>(dX)) & 7u)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_adjoint: Gamma and dX must be 8-byte aligned");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "ftdx_herk_adjoint: hipGetDevice");
+    if (dev != ctx->device)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_adjoint: context belongs to another device (or none)");
+    const size_t hb = (size_t)batch_count * N * N * sizeof(float2);
+    int rc = ftdx::ensure_scratch(ctx->cal_herm, ctx->cal_herm_cap, hb, "ftdx_herk_adjoint: scratch");
+    if (rc != FTDX_OK) return rc;
+    e = ftdx::launch_herk_adjoint(static_cast<const float2*>(Gamma), static_cast<const uint8_t*>(X),
+                                  reinterpret_cast<float2*>(ctx->cal_herm), static_cast<float2*>(dX),
+                                  N, K, batch_count, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "ftdx_herk_adjoint: launch");
+    return FTDX_OK;
+}
+
+int ftdx_mx_layout(int compute, int K, int* blocks, int* code_bytes) {
+    if (compute != FTDX_COMPUTE_FP6 && compute != FTDX_COMPUTE_FP4)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_layout: compute must be FP6 or FP4");
+    if (K < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_layout: negative K");
+    if (blocks) *blocks = ftdx::mx_padded_blocks(K);
+    if (code_bytes) *code_bytes = ftdx::mx_code_bytes(ftdx::mx_format(compute));
+    return FTDX_OK;
+}
+
+int ftdx_mx_quantize(const void* X, int input, int compute, void* codes, void* scales,
+                     int64_t rows, int K, void* stream) {
+    if (compute != FTDX_COMPUTE_FP6 && compute != FTDX_COMPUTE_FP4)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: compute must be FP6 or FP4");
+    if (input != FTDX_INPUT_INT4 && input != FTDX_INPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: unknown input precision");
+    if (rows < 0 || K < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: negative size");
+    if (rows == 0 || K == 0) return FTDX_OK;
+    if (!X || !codes || !scales) return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: NULL pointer");
+    if ((reinterpret_cast<uintptr_t>(codes) & 7u) || (input == FTDX_INPUT_FP16 && (reinterpret_cast<uintptr_t>(X) & 3u)))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: misaligned buffer");
+    hipError_t e = ftdx::launch_mx_quantize(
+        X, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16, ftdx::mx_format(compute),
+        static_cast<uint8_t*>(codes), static_cast<uint8_t*>(scales), rows, K,
+        static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "ftdx_mx_quantize: launch");
+    return FTDX_OK;
+}
+
+int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch_count,
+                 int input, int compute, int output, float alpha, float beta, void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: NULL context");
+    if (N < 0 || K < 0 || batch_count < 0)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: negative size");
+    if (compute != FTDX_COMPUTE_FP6 && compute != FTDX_COMPUTE_FP4)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: compute must be FP6 or FP4");
+    if (input != FTDX_INPUT_INT4 && input != FTDX_INPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: unknown input precision");
+    if (output != FTDX_OUTPUT_FP32 && output != FTDX_OUTPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: unknown output precision");
+    if (N == 0 || batch_count == 0) return FTDX_OK;
+    if ((!A && K > 0) || !C) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: NULL data pointer");
+    if (reinterpret_cast<uintptr_t>(C) % (output == FTDX_OUTPUT_FP32 ? 8u : 4u) != 0 ||
+        (input == FTDX_INPUT_FP16 && (reinterpret_cast<uintptr_t>(A) & 3u)))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: misaligned buffer");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: hipGetDevice");
+    if (dev != ctx->device)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: context belongs to another device (or none)");
+
+    const int fmt = ftdx::mx_format(compute);
+    const int nblkp = ftdx::mx_padded_blocks(K);
+    const int64_t rows = (int64_t)N * batch_count;
+    const size_t code_bytes = (size_t)rows * nblkp * 2 * ftdx::mx_code_bytes(fmt);
+    const size_t scale_bytes = (size_t)rows * nblkp * 2;
+    int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap, code_bytes, "ftdx_herk_mx: scratch");
+    if (rc != FTDX_OK) return rc;
+    rc = ftdx::ensure_scratch(ctx->mx_scales, ctx->mx_scales_cap, scale_bytes, "ftdx_herk_mx: scratch");
+    if (rc != FTDX_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (nblkp > 0) {
+        e = ftdx::launch_mx_quantize(A, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16,
+                                     fmt, ctx->mx_codes, ctx->mx_scales, rows, K, s);
+        if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: quantize launch");
+    }
+    ftdx::MxHerkArgs a{};
+    a.codes = ctx->mx_codes;
+    a.scales = ctx->mx_scales;
+    a.C = C;
+    a.N = N;
+    a.batch = batch_count;
+    a.nblkp = nblkp;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.c_bstride = (int64_t)N * (N + 1);
+    e = ftdx::launch_herk_mx(a, fmt, output == FTDX_OUTPUT_FP16, s);
+    if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: kernel launch");
     return FTDX_OK;
 }
 
@@ -335,10 +517,12 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: C must be aligned to one complex element");
     if ((int64_t)N * (N + 1) / 2 > (int64_t)1 << 40)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: N too large");
+    // FP6/FP4 compute: the block-scaled MX path (FP16 output, as the reference)
+    if (compute != FTDX_COMPUTE_FP8)
+        return ftdx_herk_mx(ctx, A, C, N, K, batch_count, input, compute, output, alpha, beta,
+                            stream);
     if (input != FTDX_INPUT_INT4)
         return fail(FTDX_ERR_UNSUPPORTED, "ftdx_herk: FP16 input is not built yet");
-    if (compute != FTDX_COMPUTE_FP8)
-        return fail(FTDX_ERR_UNSUPPORTED, "ftdx_herk: FP6/FP4 compute is not built yet");
 
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);

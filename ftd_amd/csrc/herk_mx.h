@@ -1,0 +1,32 @@
+// herk_mx.h -- host-side interface of the MX block-scaled HERK kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ftdx {
+
+enum MxFormat { MX_FP4_E2M1 = 0, MX_FP6_E3M2 = 1 };
+enum MxInput { MX_IN_INT4 = 0, MX_IN_FP16 = 1 };
+
+// Bytes of one 32-code block (FP4: 16, FP6: 24).
+int mx_code_bytes(int fmt);
+// 32-sample blocks per row, padded to whole LDS stages.
+int mx_padded_blocks(int K);
+
+// X: [rows][K] QC bytes (MX_IN_INT4) or [rows][K][2] FP16 (MX_IN_FP16).
+// codes: rows * nblkp * 2 * mx_code_bytes(fmt) bytes; scales: rows * nblkp * 2 bytes.
+hipError_t launch_mx_quantize(const void* X, int input, int fmt, uint8_t* codes, uint8_t* scales,
+                              int64_t rows, int K, hipStream_t s);
+
+struct MxHerkArgs {
+    const uint8_t* codes;   // [batch][N][nblkp][2][PB]
+    const uint8_t* scales;  // [batch][N][nblkp][2]
+    void* C;
+    int N, batch, nblkp;
+    float alpha, beta;
+    int64_t c_bstride;      // scalars of C per batch
+};
+
+hipError_t launch_herk_mx(const MxHerkArgs& a, int fmt, bool out_fp16, hipStream_t s);
+
+}  // namespace ftdx

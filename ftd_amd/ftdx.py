@@ -108,6 +108,11 @@ _SIGNATURES = {
                                          ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
     "ftdx_herk_grouped_plan_destroy": (_I, [_VP]),
     "ftdx_herk_each": (_I, [_VP, _VP, _I, _I, _VP]),
+    "ftdx_herk_mx": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_mx_quantize": (_I, [_VP, _I, _I, _VP, _VP, ctypes.c_int64, _I, _VP]),
+    "ftdx_mx_layout": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ftdx_gain_loss_grad": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+    "ftdx_herk_adjoint": (_I, [_VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP]),
 }
 
 TILE_AUTO = -1
@@ -279,6 +284,67 @@ class CutlassComplexGemm:
 
     def tune_save(self, path: str) -> None:
         _check(lib().ftdx_tune_save(self._h, path.encode()), "tune_save")
+
+
+def mx_layout(compute: ComputePrecision, K: int) -> tuple[int, int]:
+    """(padded 32-sample blocks per row, bytes per 32-code block)."""
+    b, c = ctypes.c_int(), ctypes.c_int()
+    _check(lib().ftdx_mx_layout(int(compute), K, ctypes.byref(b), ctypes.byref(c)), "mx_layout")
+    return b.value, c.value
+
+
+def mx_quantize(X, input: InputPrecision, compute: ComputePrecision, rows: int, K: int,
+                stream=None):
+    """MX quantization stage on the device: returns (codes, scales) uint8 tensors of
+    shapes [rows, blocks, 2, code_bytes] and [rows, blocks, 2]."""
+    import torch
+    blocks, cb = mx_layout(compute, K)
+    codes = torch.zeros((rows, blocks, 2, cb), dtype=torch.uint8, device=X.device)
+    scales = torch.zeros((rows, blocks, 2), dtype=torch.uint8, device=X.device)
+    _check(lib().ftdx_mx_quantize(_ptr(X), int(input), int(compute), _ptr(codes), _ptr(scales),
+                                  rows, K, _stream_ptr(stream)), "mx_quantize")
+    return codes, scales
+
+
+def herk_mx(ctx: "CutlassComplexGemm", A, N: int, K: int, batch: int,
+            input: InputPrecision = InputPrecision.FP16,
+            compute: ComputePrecision = ComputePrecision.FP6, output: str = "fp32", C=None,
+            alpha: float = 1.0, beta: float = 0.0, stream=None):
+    """MX block-scaled HERK (ftdx_herk_mx); returns C [batch, N(N+1)/2, 2]."""
+    import torch
+    dt = torch.float32 if output == "fp32" else torch.float16
+    if C is None:
+        C = torch.empty((batch, packed_len(N), 2), dtype=dt, device=A.device)
+    out = OutputPrecision.FP32 if output == "fp32" else OutputPrecision.FP16
+    _check(lib().ftdx_herk_mx(ctx.handle, _ptr(A), _ptr(C), N, K, batch, int(input),
+                              int(compute), int(out), alpha, beta, _stream_ptr(stream)),
+           "herk_mx")
+    return C
+
+
+def gain_loss_grad(ctx: "CutlassComplexGemm", V, M, g, N: int, batch: int, want_gamma=True,
+                   stream=None):
+    """Gain-calibration loss and gradients (ftdx_gain_loss_grad).  V, M: packed
+    [batch, N(N+1)/2, 2] float32; g: [batch, N, 2] float32.  Returns
+    (loss [batch] float64, grad_g [batch, N, 2] float32, Gamma or None)."""
+    import torch
+    loss = torch.empty(batch, dtype=torch.float64, device=V.device)
+    grad = torch.empty((batch, N, 2), dtype=torch.float32, device=V.device)
+    Gamma = torch.empty_like(M) if want_gamma else None
+    _check(lib().ftdx_gain_loss_grad(ctx.handle, _ptr(V), _ptr(M), _ptr(g), N, batch, _ptr(loss),
+                                     _ptr(grad), _ptr(Gamma) if Gamma is not None else None,
+                                     _stream_ptr(stream)), "gain_loss_grad")
+    return loss, grad, Gamma
+
+
+def herk_adjoint(ctx: "CutlassComplexGemm", Gamma, X, N: int, K: int, batch: int, stream=None):
+    """dX = (Gamma + Gamma^H) X (ftdx_herk_adjoint); X QC bytes; returns
+    dX [batch, N, K, 2] float32."""
+    import torch
+    dX = torch.empty((batch, N, K, 2), dtype=torch.float32, device=X.device)
+    _check(lib().ftdx_herk_adjoint(ctx.handle, _ptr(Gamma), _ptr(X), _ptr(dX), N, K, batch,
+                                   int(InputPrecision.INT4), _stream_ptr(stream)), "herk_adjoint")
+    return dX
 
 
 def problem_array(problems):

@@ -160,6 +160,48 @@ int ftdx_herk_grouped_plan_destroy(ftdx_grouped_plan* plan);
 int ftdx_herk_each(ftdx_context* ctx, const ftdx_herk_problem* problems, int count, int output,
                    void* stream);
 
+/* MX block-scaled HERK (the reference's FP6/FP4 compute path,
+ * cutlass_interface/gemm_complex_sm100/herk_impl.hpp:1067-1340, reached from
+ * CutlassComplexGemm::herk with ComputePrecision::FP6/FP4).  A is QC bytes
+ * (FTDX_INPUT_INT4) or interleaved complex FP16 [batch][N][K][2]
+ * (FTDX_INPUT_FP16).  Re and Im are quantized separately per (row, 32-sample
+ * block) with an E8M0 scale 2^ceil(log2(amax / max)) (max 28 for FP6 E3M2,
+ * 6 for FP4 E2M1; mxfp_kernels.hpp:147-166) and nearest-even element
+ * conversion, then multiplied on scaled FP6/FP4 MFMA with FP32 accumulation.
+ * Unlike the reference (FP16 output only) both FP16 and FP32 output work here;
+ * ftdx_herk keeps the reference's rule and rejects FP32 output with FP6/FP4.
+ * The context owns the quantized scratch (grown on demand, freed by
+ * ftdx_end_herk / ftdx_destroy). */
+int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch_count,
+                 int input, int compute, int output, float alpha, float beta, void* stream);
+
+/* The MX quantization stage alone (fused_mxfp_preprocess_kernel_sm100,
+ * mxfp_kernels.hpp:265-360).  X as for ftdx_herk_mx with `rows` rows of K
+ * samples.  codes: rows * blocks * 2 * code_bytes bytes laid out
+ * [row][block][Re|Im][code_bytes], scales: rows * blocks * 2 E8M0 bytes
+ * [row][block][Re|Im], with blocks and code_bytes from ftdx_mx_layout. */
+int ftdx_mx_quantize(const void* X, int input, int compute, void* codes, void* scales,
+                     int64_t rows, int K, void* stream);
+int ftdx_mx_layout(int compute, int K, int* blocks, int* code_bytes);
+
+/* Differentiable gain calibration on packed visibilities (no reference
+ * counterpart: the reference has no backward pass).  Per channel b, with M the
+ * model visibilities (a HERK output), V the observed ones, both packed lower
+ * triangles [batch][N(N+1)/2][2] FP32, and gains g [batch][N][2] FP32:
+ *   R_ij = V_ij - g_i conj(g_j) M_ij,   loss_b = sum_{i>=j} |R_ij|^2   (FP64, device)
+ *   grad_g [batch][N][2] = dL/dRe g + i dL/dIm g                        (FP32, device)
+ *   Gamma (optional, packed like M) = dL/dRe M + i dL/dIm M = -2 R_ij conj(g_i) g_j
+ * Sums are reduced in a fixed order (bitwise reproducible). */
+int ftdx_gain_loss_grad(ftdx_context* ctx, const void* V, const void* M, const void* g, int N,
+                        int batch_count, void* loss, void* grad_g, void* Gamma, void* stream);
+
+/* HERK adjoint: for M = X X^H (packed) and an upstream gradient Gamma (packed,
+ * the convention above), dX = (Gamma + Gamma^H) X, FP32 [batch][N][K][2].
+ * X is QC bytes [batch][N][K] (input = FTDX_INPUT_INT4).  FP32 MFMA complex GEMM;
+ * the context owns the dense Hermitian scratch (batch * N * N * 8 bytes). */
+int ftdx_herk_adjoint(ftdx_context* ctx, const void* Gamma, const void* X, void* dX, int N, int K,
+                      int batch_count, int input, void* stream);
+
 /* Replaces CutlassComplexGemm::print_build_info (cutlass_gemm_api.h:773-775). */
 int ftdx_print_build_info(void);
 

@@ -283,3 +283,205 @@ void ftdo_herk_f16(const uint8_t* A, uint16_t* C, int N, int K, int batch,
                 }
     free(acc);
 }
+
+/* ------------------------------------------------------------------ */
+/* MX block-scaled quantization */
+static void mx_params(int fmt, int* mbits, int* bias, float* maxv) {
+    switch (fmt) {
+        case 0: *mbits = 1; *bias = 1; *maxv = 6.0f; break;   /* FP4 E2M1 */
+        case 1: *mbits = 2; *bias = 3; *maxv = 28.0f; break;  /* FP6 E3M2 */
+        default: *mbits = 3; *bias = 1; *maxv = 7.5f; break;  /* FP6 E2M3 */
+    }
+}
+
+static int mx_sign_shift(int fmt) { return fmt == 0 ? 3 : 5; }
+
+uint8_t ftdo_mx_encode(float x, int fmt) {
+    int mbits, bias;
+    float maxv;
+    mx_params(fmt, &mbits, &bias, &maxv);
+    const unsigned sign = signbit(x) ? 1u : 0u;
+    const float a = fabsf(x);
+    const int ebits = (fmt == 1) ? 3 : 2;
+    const unsigned maxcode = (((1u << ebits) - 1u) << mbits) | ((1u << mbits) - 1u);
+    unsigned code;
+    if (a >= maxv) {
+        code = maxcode;
+    } else {
+        int e;
+        frexpf(a, &e);
+        e -= 1; /* a in [2^e, 2^(e+1)) */
+        const int emin = 1 - bias;
+        if (a == 0.0f || e < emin) {
+            /* subnormal grid step 2^(emin - mbits); q may reach 2^mbits = min normal */
+            code = (unsigned)rintf(ldexpf(a, mbits - emin));
+        } else {
+            unsigned q = (unsigned)rintf(ldexpf(a, mbits - e)); /* in [2^m, 2^(m+1)] */
+            if (q == (2u << mbits)) {
+                q >>= 1;
+                ++e;
+            }
+            code = ((unsigned)(e + bias) << mbits) | (q - (1u << mbits));
+        }
+        if (code > maxcode) code = maxcode;
+    }
+    return (uint8_t)((sign << mx_sign_shift(fmt)) | code);
+}
+
+double ftdo_mx_decode(uint8_t code, int fmt) {
+    int mbits, bias;
+    float maxv;
+    mx_params(fmt, &mbits, &bias, &maxv);
+    const int s = mx_sign_shift(fmt);
+    const unsigned mag = code & ((1u << s) - 1u);
+    const unsigned e = mag >> mbits, m = mag & ((1u << mbits) - 1u);
+    double v = (e == 0) ? ldexp((double)m, 1 - bias - mbits)
+                        : ldexp((double)(m | (1u << mbits)), (int)e - bias - mbits);
+    return ((code >> s) & 1u) ? -v : v;
+}
+
+uint8_t ftdo_mx_scale_exponent(float amax, int fmt) {
+    int mbits, bias;
+    float maxv;
+    mx_params(fmt, &mbits, &bias, &maxv);
+    if (amax == 0.0f) return 0;
+    const float ratio = amax / maxv;
+    if (ratio == 0.0f) return 0; /* denormal amax underflowing the ratio: smallest scale */
+    int e;
+    const float m = frexpf(ratio, &e); /* ratio = m * 2^e, m in [0.5, 1) */
+    int p = (m == 0.5f) ? e - 1 : e;   /* ceil(log2(ratio)) */
+    int b = p + 127;
+    if (b < 0) b = 0;
+    if (b > 254) b = 254;
+    return (uint8_t)b;
+}
+
+void ftdo_mx_quantize(const float* X, int rows, int K, int fmt, uint8_t* codes, uint8_t* scales) {
+    const int nblk = (K + 31) / 32;
+    for (int r = 0; r < rows; ++r)
+        for (int b = 0; b < nblk; ++b)
+            for (int part = 0; part < 2; ++part) {
+                float amax = 0.0f;
+                for (int i = 0; i < 32; ++i) {
+                    const int k = b * 32 + i;
+                    if (k < K) {
+                        const float v = fabsf(X[((int64_t)r * K + k) * 2 + part]);
+                        if (v > amax) amax = v;
+                    }
+                }
+                const uint8_t se = ftdo_mx_scale_exponent(amax, fmt);
+                scales[((int64_t)r * nblk + b) * 2 + part] = se;
+                const float inv = ldexpf(1.0f, 127 - (int)se);
+                for (int i = 0; i < 32; ++i) {
+                    const int k = b * 32 + i;
+                    const float v = (k < K) ? X[((int64_t)r * K + k) * 2 + part] : 0.0f;
+                    codes[(((int64_t)r * nblk + b) * 2 + part) * 32 + i] =
+                        ftdo_mx_encode(v * inv, fmt);
+                }
+            }
+}
+
+void ftdo_herk_c64(const double* X, double* C, int N, int K, int batch, int threads) {
+    const int64_t total = (int64_t)N * batch;
+    const int64_t p2 = (int64_t)N * (N + 1);
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+#endif
+    for (int64_t t = 0; t < total; ++t) {
+        const int b = (int)(t / N), i = N - 1 - (int)(t % N);
+        const double* xi = X + ((int64_t)b * N + i) * K * 2;
+        double* Crow = C + b * p2 + (int64_t)i * (i + 1);
+        for (int j = 0; j <= i; ++j) {
+            const double* xj = X + ((int64_t)b * N + j) * K * 2;
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const double ar = xi[2 * k], ai = xi[2 * k + 1];
+                const double br = xj[2 * k], bi = xj[2 * k + 1];
+                sr += ar * br + ai * bi;
+                si += ai * br - ar * bi;
+            }
+            Crow[2 * j] = sr;
+            Crow[2 * j + 1] = (i == j) ? 0.0 : si;
+        }
+    }
+    (void)threads;
+}
+
+/* ------------------------------------------------------------------ */
+/* Gain calibration and HERK adjoint (FP64).  Derivation: with z_ij =
+ * g_i conj(g_j) M_ij and R = V - z, dL = -2 Re sum conj(R) dz; collecting the
+ * coefficients of dg_k and conj(dg_k) gives the gradient below; dL/dM gives
+ * Gamma; through M = X X^H, dX = (Gamma + Gamma^H) X. */
+void ftdo_gain_loss_grad_f64(const double* V, const double* M, const double* g, int N, int batch,
+                             double* loss, double* grad, double* Gamma) {
+    const int64_t P = (int64_t)N * (N + 1) / 2;
+    for (int b = 0; b < batch; ++b) {
+        double L = 0.0;
+        double* gr = grad + (int64_t)b * N * 2;
+        for (int k = 0; k < 2 * N; ++k) gr[k] = 0.0;
+        const double* gb = g + (int64_t)b * N * 2;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j <= i; ++j) {
+                const int64_t o = ((int64_t)b * P + (int64_t)i * (i + 1) / 2 + j) * 2;
+                const double gir = gb[2 * i], gii = gb[2 * i + 1];
+                const double gjr = gb[2 * j], gji = gb[2 * j + 1];
+                /* p = g_i conj(g_j) */
+                const double pr = gir * gjr + gii * gji, pi = gii * gjr - gir * gji;
+                const double mr = M[o], mi = M[o + 1];
+                const double zr = pr * mr - pi * mi, zi = pr * mi + pi * mr;
+                const double rr = V[o] - zr, ri = V[o + 1] - zi;
+                L += rr * rr + ri * ri;
+                if (Gamma) { /* -2 R conj(p) */
+                    Gamma[o] = -2.0 * (rr * pr + ri * pi);
+                    Gamma[o + 1] = -2.0 * (ri * pr - rr * pi);
+                }
+                /* row i term: -2 R conj(M) g_j */
+                const double ar = rr * mr + ri * mi, ai = ri * mr - rr * mi;
+                gr[2 * i] += -2.0 * (ar * gjr - ai * gji);
+                gr[2 * i + 1] += -2.0 * (ar * gji + ai * gjr);
+                /* column j term: -2 conj(R) M g_i */
+                const double br = rr * mr + ri * mi, bi = rr * mi - ri * mr;
+                gr[2 * j] += -2.0 * (br * gir - bi * gii);
+                gr[2 * j + 1] += -2.0 * (br * gii + bi * gir);
+            }
+        loss[b] = L;
+    }
+}
+
+void ftdo_herk_adjoint_f64(const double* Gamma, const double* X, double* dX, int N, int K,
+                           int batch, int threads) {
+    const int64_t P = (int64_t)N * (N + 1) / 2;
+    const int64_t total = (int64_t)batch * N;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+#endif
+    for (int64_t t = 0; t < total; ++t) {
+        const int b = (int)(t / N), i = (int)(t % N);
+        double* out = dX + ((int64_t)b * N + i) * K * 2;
+        for (int k = 0; k < 2 * K; ++k) out[k] = 0.0;
+        for (int j = 0; j < N; ++j) {
+            double hr, hi;
+            if (i > j) {
+                const int64_t o = ((int64_t)b * P + (int64_t)i * (i + 1) / 2 + j) * 2;
+                hr = Gamma[o];
+                hi = Gamma[o + 1];
+            } else if (i < j) {
+                const int64_t o = ((int64_t)b * P + (int64_t)j * (j + 1) / 2 + i) * 2;
+                hr = Gamma[o];
+                hi = -Gamma[o + 1];
+            } else {
+                const int64_t o = ((int64_t)b * P + (int64_t)i * (i + 1) / 2 + i) * 2;
+                hr = 2.0 * Gamma[o];
+                hi = 0.0;
+            }
+            const double* xj = X + ((int64_t)b * N + j) * K * 2;
+            for (int k = 0; k < K; ++k) {
+                out[2 * k] += hr * xj[2 * k] - hi * xj[2 * k + 1];
+                out[2 * k + 1] += hr * xj[2 * k + 1] + hi * xj[2 * k];
+            }
+        }
+    }
+    (void)threads;
+}

@@ -61,6 +61,34 @@ void ftdo_herk_f16(const uint8_t* A, uint16_t* C, int N, int K, int batch,
 uint16_t ftdo_f32_to_f16_rne(float f);
 float    ftdo_f16_to_f32(uint16_t h);
 
+/* ---- MX block-scaled quantization (FP6 E3M2 / FP4 E2M1, 32-sample blocks) ----
+ * fmt: 0 = FP4 E2M1 (max 6), 1 = FP6 E3M2 (max 28), 2 = FP6 E2M3 (max 7.5).
+ * Scale per (row, 32-sample block, Re|Im part): E8M0 biased exponent
+ * e = clamp(ceil(log2(amax / max)) + 127, 0, 254), 0 when amax == 0
+ * (reference rule: cutlass_interface/gemm_complex_sm100/mxfp_kernels.hpp:147-166).
+ * Elements: x * 2^(127-e) rounded to the format, nearest-even, with subnormals,
+ * saturating at +-max (OCP Microscaling Formats v1.0 conversion).
+ * Codes are returned one per byte (sign in the format's top bit). */
+uint8_t ftdo_mx_encode(float x, int fmt);
+double  ftdo_mx_decode(uint8_t code, int fmt);
+uint8_t ftdo_mx_scale_exponent(float amax, int fmt);
+/* X: complex samples as float pairs [rows][K][2].  codes: [rows][nblk][2][32],
+ * scales: [rows][nblk][2] with nblk = ceil(K/32) (tail samples quantize as 0). */
+void ftdo_mx_quantize(const float* X, int rows, int K, int fmt, uint8_t* codes, uint8_t* scales);
+/* FP64 HERK of complex double input [batch][N][K][2], packed lower triangle
+ * [batch][N(N+1)/2][2], Im(diag) = 0. */
+void ftdo_herk_c64(const double* X, double* C, int N, int K, int batch, int threads);
+
+/* ---- gain calibration (FP64) ----
+ * V, M: packed lower triangles [batch][N(N+1)/2][2]; g: [batch][N][2].
+ * R_ij = V_ij - g_i conj(g_j) M_ij (i >= j), loss_b = sum |R_ij|^2,
+ * grad[b][k] = dL/dRe g_k + i dL/dIm g_k, Gamma_ij = dL/dRe M_ij + i dL/dIm M_ij. */
+void ftdo_gain_loss_grad_f64(const double* V, const double* M, const double* g, int N, int batch,
+                             double* loss, double* grad, double* Gamma);
+/* dX = (Gamma + Gamma^H) X; Gamma packed lower; X, dX: [batch][N][K][2]. */
+void ftdo_herk_adjoint_f64(const double* Gamma, const double* X, double* dX, int N, int K,
+                           int batch, int threads);
+
 #ifdef __cplusplus
 }
 #endif

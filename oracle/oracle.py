@@ -65,6 +65,25 @@ def lib():
                                     ctypes.c_float, ctypes.c_int]
         L.ftdo_f32_to_f16_rne.restype = ctypes.c_uint16
         L.ftdo_f32_to_f16_rne.argtypes = [ctypes.c_float]
+        L.ftdo_mx_encode.restype = ctypes.c_uint8
+        L.ftdo_mx_encode.argtypes = [ctypes.c_float, ctypes.c_int]
+        L.ftdo_mx_decode.restype = ctypes.c_double
+        L.ftdo_mx_decode.argtypes = [ctypes.c_uint8, ctypes.c_int]
+        L.ftdo_mx_scale_exponent.restype = ctypes.c_uint8
+        L.ftdo_mx_scale_exponent.argtypes = [ctypes.c_float, ctypes.c_int]
+        L.ftdo_mx_quantize.restype = None
+        L.ftdo_mx_quantize.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, u8p, u8p]
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.ftdo_gain_loss_grad_f64.restype = None
+        L.ftdo_gain_loss_grad_f64.argtypes = [dp, dp, dp, ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        L.ftdo_herk_adjoint_f64.restype = None
+        L.ftdo_herk_adjoint_f64.argtypes = [dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int]
+        L.ftdo_herk_c64.restype = None
+        L.ftdo_herk_c64.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -157,3 +176,92 @@ def herk_f16(A: np.ndarray, N: int, K: int, batch: int, alpha: float = 1.0, beta
 
 def f32_to_f16_rne(x: float) -> int:
     return int(lib().ftdo_f32_to_f16_rne(x))
+
+
+# ---- MX block-scaled quantization (fmt: 0 = FP4 E2M1, 1 = FP6 E3M2, 2 = FP6 E2M3) ----
+MX_FP4_E2M1, MX_FP6_E3M2, MX_FP6_E2M3 = 0, 1, 2
+
+
+def mx_encode(x: float, fmt: int) -> int:
+    return int(lib().ftdo_mx_encode(x, fmt))
+
+
+def mx_decode(code: int, fmt: int) -> float:
+    return float(lib().ftdo_mx_decode(code, fmt))
+
+
+def mx_scale_exponent(amax: float, fmt: int) -> int:
+    return int(lib().ftdo_mx_scale_exponent(amax, fmt))
+
+
+def mx_quantize(X: np.ndarray, fmt: int):
+    """X: float32 complex pairs [rows, K, 2].  Returns (codes [rows, nblk, 2, 32] uint8,
+    scales [rows, nblk, 2] uint8)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    rows, K = X.shape[0], X.shape[1]
+    nblk = (K + 31) // 32
+    codes = np.zeros((rows, nblk, 2, 32), dtype=np.uint8)
+    scales = np.zeros((rows, nblk, 2), dtype=np.uint8)
+    lib().ftdo_mx_quantize(_ptr(X, ctypes.c_float), rows, K, fmt, _ptr(codes, ctypes.c_uint8),
+                           _ptr(scales, ctypes.c_uint8))
+    return codes, scales
+
+
+def mx_dequantize(codes: np.ndarray, scales: np.ndarray, fmt: int, K: int) -> np.ndarray:
+    """Back to complex128 [rows, K] from mx_quantize's output."""
+    table = np.array([mx_decode(c, fmt) for c in range(64 if fmt else 16)])
+    vals = table[codes.astype(np.int64) & (63 if fmt else 15)]
+    vals = vals * np.ldexp(1.0, scales.astype(np.int64) - 127)[..., None]
+    rows, nblk = scales.shape[0], scales.shape[1]
+    re = vals[:, :, 0, :].reshape(rows, nblk * 32)[:, :K]
+    im = vals[:, :, 1, :].reshape(rows, nblk * 32)[:, :K]
+    return re + 1j * im
+
+
+def herk_c64(X: np.ndarray, N: int, K: int, batch: int, threads: int = 0) -> np.ndarray:
+    """FP64 HERK of complex128 X [batch, N, K] -> packed [batch, N(N+1)/2, 2] float64."""
+    Xf = np.ascontiguousarray(np.stack([X.real, X.imag], axis=-1), dtype=np.float64)
+    C = np.zeros((batch, N * (N + 1) // 2, 2), dtype=np.float64)
+    lib().ftdo_herk_c64(_ptr(Xf, ctypes.c_double), _ptr(C, ctypes.c_double), N, K, batch, threads)
+    return C
+
+
+# ---- gain calibration (FP64) ----
+def _c2(a):
+    a = np.asarray(a)
+    if np.iscomplexobj(a):
+        a = np.stack([a.real, a.imag], axis=-1)
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def gain_loss_grad_f64(V, M, g, N: int, batch: int):
+    """V, M: packed [batch, P] complex (or [..., 2] float); g: [batch, N] complex.
+    Returns (loss [batch], grad [batch, N] complex, Gamma [batch, P] complex)."""
+    Vf, Mf, gf = _c2(V), _c2(M), _c2(g)
+    P = N * (N + 1) // 2
+    assert Vf.size == batch * P * 2 and Mf.size == batch * P * 2 and gf.size == batch * N * 2
+    loss = np.zeros(batch)
+    grad = np.zeros((batch, N, 2))
+    Gam = np.zeros((batch, P, 2))
+    lib().ftdo_gain_loss_grad_f64(_ptr(Vf, ctypes.c_double), _ptr(Mf, ctypes.c_double),
+                                  _ptr(gf, ctypes.c_double), N, batch,
+                                  _ptr(loss, ctypes.c_double), _ptr(grad, ctypes.c_double),
+                                  _ptr(Gam, ctypes.c_double))
+    return loss, grad[..., 0] + 1j * grad[..., 1], Gam[..., 0] + 1j * Gam[..., 1]
+
+
+def herk_adjoint_f64(Gamma, X, N: int, K: int, batch: int, threads: int = 0):
+    """dX = (Gamma + Gamma^H) X in FP64; Gamma packed [batch, P] complex; X [batch, N, K]
+    complex.  Returns complex128 [batch, N, K]."""
+    Gf, Xf = _c2(Gamma), _c2(X)
+    dX = np.zeros((batch, N, K, 2))
+    lib().ftdo_herk_adjoint_f64(_ptr(Gf, ctypes.c_double), _ptr(Xf, ctypes.c_double),
+                                _ptr(dX, ctypes.c_double), N, K, batch, threads)
+    return dX[..., 0] + 1j * dX[..., 1]
+
+
+def decode_qc_array(A: np.ndarray) -> np.ndarray:
+    """QC bytes -> complex128 values."""
+    nib = np.array([0, 1, 2, 3, 4, 5, 6, 7, 0, -1, -2, -3, -4, -5, -6, -7], dtype=np.float64)
+    A = np.asarray(A, dtype=np.uint8)
+    return nib[A >> 4] + 1j * nib[A & 15]

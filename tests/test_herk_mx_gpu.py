@@ -1,0 +1,155 @@
+"""GPU tests of the MX block-scaled HERK (FP6 E3M2 / FP4 E2M1, 32-sample E8M0
+scales along K, FP32 accumulate).
+
+- The device quantizer's codes and scales equal the oracle's, byte for byte.
+- The MX HERK equals the FP64 HERK of the *dequantized* input to the MFMA's
+  accumulation precision (FP32 rounding for FP4; ~1e-4 of the largest entry for
+  FP6 E3M2, whose wide product range exceeds the MFMA's internal sum width).
+  This is a parity check of the scaled-MFMA path.
+- Against the FP64 HERK of the unquantized input the error is reported, not
+  pinned: relative Frobenius and max-element error per channel, as the
+  reference's precision study does (cutlass_interface/test_precision_accuracy.cu,
+  thresholds 25% / 55% of peak for FP6 / FP4 at :149-160).
+- INT4 input is exact in FP6 E3M2, so that case is bit-exact against the
+  integer oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _voltages(seed, rows, K, outlier_frac=0.01, outlier_mag=30.0):
+    """Complex Gaussian (E|x|^2 = 1) with a fraction of impulsive outliers of
+    magnitude `outlier_mag` and uniform phase; returned as FP16 pairs."""
+    rng = np.random.default_rng(seed)
+    x = (rng.standard_normal((rows, K)) + 1j * rng.standard_normal((rows, K))) / np.sqrt(2)
+    mask = rng.random((rows, K)) < outlier_frac
+    x[mask] = outlier_mag * np.exp(1j * rng.uniform(-np.pi, np.pi, mask.sum()))
+    return np.stack([x.real, x.imag], axis=-1).astype(np.float16)
+
+
+def _pack(codes, fmt, nblkp):
+    """Oracle codes [rows, nblk, 2, 32] (one per byte) -> packed device layout
+    [rows, nblkp, 2, PB]: FP4 nibble i at bits 4i, FP6 field i at bits 6i (LSB first)."""
+    rows, nblk = codes.shape[0], codes.shape[1]
+    bits = 4 if fmt == 0 else 6
+    pb = 32 * bits // 8
+    out = np.zeros((rows, nblkp, 2, pb), dtype=np.uint8)
+    c = codes.astype(np.uint64)
+    for r in range(rows):
+        for b in range(nblk):
+            for p in range(2):
+                v = 0
+                for i in range(32):
+                    v |= int(c[r, b, p, i]) << (bits * i)
+                out[r, b, p] = np.frombuffer(v.to_bytes(pb, "little"), dtype=np.uint8)
+    return out
+
+
+@pytest.mark.parametrize("compute", ["FP4", "FP6"])
+@pytest.mark.parametrize("inp", ["FP16", "INT4"])
+def test_quantizer_bytes_match_oracle(oracle, dev, compute, inp):
+    import torch
+    from ftd_amd import ftdx
+    fmt = 0 if compute == "FP4" else 1
+    rows, K = 37, 200
+    if inp == "FP16":
+        X16 = _voltages(11, rows, K)
+        X = torch.from_numpy(X16).to(dev)
+        Xf = X16.astype(np.float32)
+    else:
+        A = oracle.gen_bytes(12, rows * K)  # raw bytes: -0 nibbles included
+        X = torch.from_numpy(A).to(dev)
+        nib = np.array([0, 1, 2, 3, 4, 5, 6, 7, 0, -1, -2, -3, -4, -5, -6, -7], dtype=np.float32)
+        Xf = np.stack([nib[A >> 4], nib[A & 15]], axis=-1).reshape(rows, K, 2)
+    codes, scales = ftdx.mx_quantize(X, getattr(ftdx.InputPrecision, inp),
+                                     getattr(ftdx.ComputePrecision, compute), rows, K)
+    torch.cuda.synchronize()
+    ref_codes, ref_scales = oracle.mx_quantize(Xf, fmt)
+    nblkp = codes.shape[1]
+    assert np.array_equal(scales.cpu().numpy()[:, :ref_scales.shape[1]], ref_scales)
+    assert not np.any(scales.cpu().numpy()[:, ref_scales.shape[1]:])
+    assert np.array_equal(codes.cpu().numpy(), _pack(ref_codes, fmt, nblkp))
+
+
+@pytest.mark.parametrize("compute", ["FP4", "FP6"])
+@pytest.mark.parametrize("output", ["fp32", "fp16"])
+@pytest.mark.parametrize("N,K,B", [(200, 300, 2), (129, 64, 1), (33, 1000, 1)])
+def test_mx_herk_matches_fp64_of_quantized(oracle, dev, compute, output, N, K, B):
+    import torch
+    from ftd_amd import ftdx
+    fmt = 0 if compute == "FP4" else 1
+    X16 = _voltages(21 + N, B * N, K)
+    ctx = ftdx.CutlassComplexGemm()
+    C = ftdx.herk_mx(ctx, torch.from_numpy(X16).to(dev), N, K, B, ftdx.InputPrecision.FP16,
+                     getattr(ftdx.ComputePrecision, compute), output=output)
+    torch.cuda.synchronize()
+    got = C.cpu().numpy().astype(np.float64)
+    codes, scales = oracle.mx_quantize(X16.astype(np.float32), fmt)
+    Xq = oracle.mx_dequantize(codes, scales, fmt, K).reshape(B, N, K)
+    ref = oracle.herk_c64(Xq, N, K, B)
+    scale = np.abs(ref).max()
+    # FP4 products accumulate to FP32 rounding.  FP6 E3M2 spans 2^16 in product
+    # magnitude and the MFMA's internal sum keeps fewer bits: ~1e-4 of the largest
+    # entry, measured on MI355X (a single 32x32x64 step with random E3M2 codes
+    # already differs by 2e-5 of its largest output).  FP16 output adds 2^-11.
+    tol = {("fp32", "FP4"): 2e-6, ("fp32", "FP6"): 5e-4}.get((output, compute), 2e-3)
+    assert np.abs(got - ref).max() <= tol * scale, np.abs(got - ref).max() / scale
+
+
+def test_mx_fp6_int4_bit_exact(oracle, dev):
+    """INT4 voltages are exact in FP6 E3M2 (scale 2^-2): the MX HERK is bit-exact."""
+    import torch
+    from ftd_amd import ftdx
+    N, K, B = 130, 256, 2
+    A = oracle.gen_qc_uniform(31, N * K * B)
+    ctx = ftdx.CutlassComplexGemm()
+    C = ftdx.herk_mx(ctx, torch.from_numpy(A).to(dev), N, K, B, ftdx.InputPrecision.INT4,
+                     ftdx.ComputePrecision.FP6, output="fp32")
+    torch.cuda.synchronize()
+    ref = oracle.herk_f32(A, N, K, B)
+    assert np.array_equal(C.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("compute,bound_fro,bound_max", [("FP6", 0.08, 0.25), ("FP4", 0.30, 0.55)])
+def test_mx_error_vs_fp64_reported(oracle, dev, compute, bound_fro, bound_max):
+    """Gaussian voltages with 1% 30-sigma outliers: error vs the FP64 HERK of
+    the unquantized FP16 values, per channel (reported; loose sanity bounds)."""
+    import torch
+    from ftd_amd import ftdx
+    N, K, B = 96, 512, 3
+    X16 = _voltages(4, B * N, K)
+    ctx = ftdx.CutlassComplexGemm()
+    C = ftdx.herk_mx(ctx, torch.from_numpy(X16).to(dev), N, K, B, ftdx.InputPrecision.FP16,
+                     getattr(ftdx.ComputePrecision, compute), output="fp32")
+    torch.cuda.synchronize()
+    got = C.cpu().numpy().astype(np.float64)
+    X = (X16[..., 0].astype(np.float64) + 1j * X16[..., 1].astype(np.float64)).reshape(B, N, K)
+    ref = oracle.herk_c64(X, N, K, B)
+    for b in range(B):
+        d = got[b] - ref[b]
+        fro = np.sqrt((d ** 2).sum() / (ref[b] ** 2).sum())
+        mx = np.abs(d).max() / np.abs(ref[b]).max()
+        print(f"MX {compute} channel {b}: rel Frobenius {fro:.4e}, max-element {mx:.4e}")
+        assert fro < bound_fro and mx < bound_max
+
+
+def test_herk_dispatches_mx_and_keeps_reference_rules(oracle, dev):
+    import torch
+    from ftd_amd import ftdx
+    N, K, B = 64, 96, 2
+    X16 = _voltages(5, B * N, K)
+    X = torch.from_numpy(X16).to(dev)
+    g = ftdx.CutlassComplexGemm()
+    C16 = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float16, device=dev)
+    assert g.herk(X, C16, N, K, B, ftdx.InputPrecision.FP16, ftdx.ComputePrecision.FP6,
+                  ftdx.OutputPrecision.FP16, tune=False) == 0
+    ref = ftdx.herk_mx(g, X, N, K, B, ftdx.InputPrecision.FP16, ftdx.ComputePrecision.FP6,
+                       output="fp16")
+    torch.cuda.synchronize()
+    assert torch.equal(C16.view(torch.int16), ref.view(torch.int16))
+    C32 = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    with pytest.raises(ftdx.UnsupportedError):
+        g.herk(X, C32, N, K, B, ftdx.InputPrecision.FP16, ftdx.ComputePrecision.FP4,
+               ftdx.OutputPrecision.FP32)
