@@ -19,6 +19,7 @@
 #include "herk_fp8.h"
 #include "herk_mx.h"
 #include "calib.h"
+#include "stages.h"
 
 struct ftdx_context {
     int device = 0;
@@ -44,6 +45,22 @@ struct ftdx_context {
     size_t cal_part_cap = 0;
     uint8_t* cal_herm = nullptr;
     size_t cal_herm_cap = 0;
+    // pre-cast FP8 operand rows for the unfused / FP16-input paths
+    uint8_t* precast = nullptr;
+    size_t precast_cap = 0;
+    // plain-path graph replay (set_direct_herk_graph): small LRU of captured calls
+    struct GraphEntry {
+        bool used = false;
+        int N = 0, K = 0, batch = 0, input = 0, output = 0, mode = 0;
+        float alpha = 0.f, beta = 0.f;
+        const void* A = nullptr;
+        void* C = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long stamp = 0;
+    };
+    GraphEntry graphs[4];
+    unsigned long long graph_clock = 0;
+    hipStream_t cap_stream = nullptr;
 };
 
 namespace ftdx {
@@ -198,6 +215,21 @@ int ensure_scratch(uint8_t*& p, size_t& cap, size_t need, const char* what) {
     return FTDX_OK;
 }
 
+// Grows the pre-cast operand buffer; captured graphs of the pre-cast path point at
+// the old buffer, so they are dropped when it moves.
+int grow_precast(ftdx_context* ctx, size_t need, const char* what) {
+    const uint8_t* before = ctx->precast;
+    int rc = ensure_scratch(ctx->precast, ctx->precast_cap, need, what);
+    if (rc == FTDX_OK && before && before != ctx->precast) {
+        for (auto& gph : ctx->graphs)
+            if (gph.used && gph.mode == 1) {
+                (void)hipGraphExecDestroy(gph.exec);
+                gph = ftdx_context::GraphEntry();
+            }
+    }
+    return rc;
+}
+
 int mx_format(int compute) {
     return compute == FTDX_COMPUTE_FP4 ? MX_FP4_E2M1 : MX_FP6_E3M2;
 }
@@ -328,28 +360,49 @@ int ftdx_init_herk(ftdx_context* ctx, int N, int K, int batch_count, void* strea
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_init_herk: NULL context");
     if (N < 0 || K < 0 || batch_count < 0)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_init_herk: negative size");
-    // The fused path needs no scratch: only remember the reservation.
+    // The fused path needs no scratch.  The unfused (ForceBaseline) and FP16-input
+    // paths stage pre-cast FP8 rows: allocate them now so that later calls (and
+    // graph capture) allocate nothing (cutlass_gemm_api.h:601-617).
     ctx->init_N = N;
     ctx->init_K = K;
     ctx->init_batch = batch_count;
     ftdx::ensure_tune_loaded(ctx);
+    if (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE || ctx->graph) {
+        const size_t need = (size_t)N * batch_count * 2 * ftdx::precast_row_samples(K);
+        int rc = ftdx::grow_precast(ctx, need, "ftdx_init_herk: scratch");
+        if (rc != FTDX_OK) return rc;
+    }
     return FTDX_OK;
 }
 
 int ftdx_end_herk(ftdx_context* ctx) {
     if (!ctx) return FTDX_OK;
     ctx->init_N = ctx->init_K = ctx->init_batch = 0;
-    if (ctx->mx_codes || ctx->mx_scales || ctx->cal_part || ctx->cal_herm) {
+    bool any_graph = false;
+    for (auto& gph : ctx->graphs) any_graph = any_graph || gph.used;
+    if (ctx->mx_codes || ctx->mx_scales || ctx->cal_part || ctx->cal_herm || ctx->precast ||
+        any_graph) {
         (void)hipDeviceSynchronize();
         if (ctx->mx_codes) (void)hipFree(ctx->mx_codes);
         if (ctx->mx_scales) (void)hipFree(ctx->mx_scales);
         if (ctx->cal_part) (void)hipFree(ctx->cal_part);
         if (ctx->cal_herm) (void)hipFree(ctx->cal_herm);
+        if (ctx->precast) (void)hipFree(ctx->precast);
+        for (auto& gph : ctx->graphs) {
+            if (gph.used && gph.exec) (void)hipGraphExecDestroy(gph.exec);
+            gph = ftdx_context::GraphEntry();
+        }
+    }
+    if (ctx->cap_stream) {
+        (void)hipStreamDestroy(ctx->cap_stream);
+        ctx->cap_stream = nullptr;
     }
     ctx->mx_codes = ctx->mx_scales = nullptr;
     ctx->mx_codes_cap = ctx->mx_scales_cap = 0;
     ctx->cal_part = ctx->cal_herm = nullptr;
     ctx->cal_part_cap = ctx->cal_herm_cap = 0;
+    ctx->precast = nullptr;
+    ctx->precast_cap = 0;
     return FTDX_OK;
 }
 
@@ -521,8 +574,8 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     if (compute != FTDX_COMPUTE_FP8)
         return ftdx_herk_mx(ctx, A, C, N, K, batch_count, input, compute, output, alpha, beta,
                             stream);
-    if (input != FTDX_INPUT_INT4)
-        return fail(FTDX_ERR_UNSUPPORTED, "ftdx_herk: FP16 input is not built yet");
+    if (input == FTDX_INPUT_FP16 && (reinterpret_cast<uintptr_t>(A) & 3u))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: FP16 input must be 4-byte aligned");
 
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -544,22 +597,210 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     hipStream_t s = static_cast<hipStream_t>(stream);
 
     ftdx::ensure_tune_loaded(ctx);
-    const int vec = (K % 16 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15u) == 0);
+    // Pre-cast (unfused) path: FP16 input always; INT4 input under ForceBaseline.
+    const bool precast = (input == FTDX_INPUT_FP16) || (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE);
+    const int vec = precast ? 2 : ((K % 16 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15u) == 0));
     ftdx::TuneKey key{N, K, batch_count, fp16 ? 1 : 0, vec};
     int cfg, remap;
     bool cached;
     ftdx::pick_config(ctx, key, cfg, remap, cached);
     const bool explicit_cfg = ctx->forced_cfg >= 0 || ctx->direct_tile != FTDX_DIRECT_TILE_AUTO;
-    if (tune && !cached && !explicit_cfg) {
+    const int Kp = ftdx::precast_row_samples(K);
+    if (precast) {
+        const size_t need = (size_t)N * batch_count * 2 * Kp;
+        int rc = ftdx::grow_precast(ctx, need, "ftdx_herk: scratch");
+        if (rc != FTDX_OK) return rc;
+    }
+    auto run = [&](hipStream_t st) -> hipError_t {
+        ftdx::HerkPlainArgs b = a;
+        if (precast) {
+            hipError_t pe = input == FTDX_INPUT_FP16
+                ? ftdx::launch_promote_fp16_fp8_planar(static_cast<const uint16_t*>(A), ctx->precast,
+                                                       (int64_t)N * batch_count, K, st)
+                : ftdx::launch_promote_qc_fp8_planar(static_cast<const uint8_t*>(A), ctx->precast,
+                                                     (int64_t)N * batch_count, K, st);
+            if (pe != hipSuccess) return pe;
+            b.A = ctx->precast;
+            b.precast = 1;
+            b.lda = (int64_t)2 * Kp;
+            b.a_bstride = (int64_t)N * 2 * Kp;
+        }
+        return ftdx::launch_herk_int4_plain(b, cfg, fp16, remap, st);
+    };
+    if (tune && !cached && !explicit_cfg && !precast) {
         ftdx::TuneEntry best;
         if (ftdx::autotune(ctx, a, fp16, s, key, best) == FTDX_OK) {
             cfg = best.cfg;
             if (ctx->forced_remap < 0) remap = best.xcd_remap;
         }
     }
-    e = ftdx::launch_herk_int4_plain(a, cfg, fp16, remap, s);
+    // Graph replay of identical calls (cutlass_gemm_api.cu:1738-1807): capture the
+    // launch sequence once per (shape, scalars, pointers), then replay it.
+    if (ctx->graph && ctx->herk_mode != FTDX_HERK_AUTO) {
+        const int mode = precast ? 1 : 0;
+        ftdx_context::GraphEntry* hit = nullptr;
+        for (auto& gph : ctx->graphs)
+            if (gph.used && gph.N == N && gph.K == K && gph.batch == batch_count &&
+                gph.input == input && gph.output == output && gph.mode == mode &&
+                gph.alpha == alpha && gph.beta == beta && gph.A == A && gph.C == C)
+                hit = &gph;
+        if (!hit) {
+            if (!ctx->cap_stream &&
+                (e = hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking)) != hipSuccess)
+                return hip_fail(e, "ftdx_herk: capture stream");
+            if ((e = hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeRelaxed)) != hipSuccess)
+                return hip_fail(e, "ftdx_herk: begin capture");
+            hipError_t le = run(ctx->cap_stream);
+            hipGraph_t graph = nullptr;
+            e = hipStreamEndCapture(ctx->cap_stream, &graph);
+            if (le != hipSuccess || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return hip_fail(le != hipSuccess ? le : e, "ftdx_herk: capture");
+            }
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return hip_fail(e, "ftdx_herk: graph instantiate");
+            ftdx_context::GraphEntry* slot = &ctx->graphs[0];
+            for (auto& gph : ctx->graphs) {
+                if (!gph.used) { slot = &gph; break; }
+                if (gph.stamp < slot->stamp) slot = &gph;
+            }
+            if (slot->used && slot->exec) (void)hipGraphExecDestroy(slot->exec);
+            slot->used = true;
+            slot->N = N; slot->K = K; slot->batch = batch_count;
+            slot->input = input; slot->output = output; slot->mode = mode;
+            slot->alpha = alpha; slot->beta = beta; slot->A = A; slot->C = C;
+            slot->exec = exec;
+            hit = slot;
+        }
+        hit->stamp = ++ctx->graph_clock;
+        if ((e = hipGraphLaunch(hit->exec, s)) != hipSuccess) return hip_fail(e, "ftdx_herk: graph launch");
+        return FTDX_OK;
+    }
+    e = run(s);
     if (e != hipSuccess) return hip_fail(e, "ftdx_herk: kernel launch");
     return FTDX_OK;
+}
+
+int ftdx_promote_qc_to_fp8(const void* qc, void* fp8_pairs, int64_t n, void* stream) {
+    if (n < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp8: negative size");
+    if (n == 0) return FTDX_OK;
+    if (!qc || !fp8_pairs) return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp8: NULL pointer");
+    if ((reinterpret_cast<uintptr_t>(qc) & 15u) || (reinterpret_cast<uintptr_t>(fp8_pairs) & 15u))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp8: buffers must be 16-byte aligned");
+    hipError_t e = ftdx::launch_promote_qc_fp8_interleaved(static_cast<const uint8_t*>(qc),
+                                                           static_cast<uint8_t*>(fp8_pairs), n,
+                                                           static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_promote_qc_to_fp8: launch");
+}
+
+int ftdx_promote_qc_to_fp16_planar(const void* qc, void* re, void* im, int64_t n, void* stream) {
+    if (n < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp16_planar: negative size");
+    if (n == 0) return FTDX_OK;
+    if (!qc || !re || !im) return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp16_planar: NULL pointer");
+    if ((reinterpret_cast<uintptr_t>(re) | reinterpret_cast<uintptr_t>(im)) & 1u)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp16_planar: misaligned output");
+    hipError_t e = ftdx::launch_promote_qc_fp16_planar(static_cast<const uint8_t*>(qc),
+                                                       static_cast<uint16_t*>(re),
+                                                       static_cast<uint16_t*>(im), n,
+                                                       static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_promote_qc_to_fp16_planar: launch");
+}
+
+int ftdx_promote_data(const void* in, void* out, int64_t n, int prec, void* stream) {
+    if (prec != FTDX_PREC_FP32 && prec != FTDX_PREC_FP64)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_data: prec must be FTDX_PREC_FP32 or _FP64");
+    if (n < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_data: negative size");
+    if (n == 0) return FTDX_OK;
+    if (!in || !out) return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_data: NULL pointer");
+    if (reinterpret_cast<uintptr_t>(out) & (prec == FTDX_PREC_FP64 ? 7u : 3u))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_data: misaligned output");
+    hipError_t e = ftdx::launch_promote_data(static_cast<const uint8_t*>(in), out, n,
+                                             prec == FTDX_PREC_FP64, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_promote_data: launch");
+}
+
+int ftdx_pack_triangle(const void* full, void* packed, const void* old, int N, int batch_count,
+                       float beta, void* stream) {
+    if (N < 0 || batch_count < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_pack_triangle: negative size");
+    if (N == 0 || batch_count == 0) return FTDX_OK;
+    if (!full || !packed) return fail(FTDX_ERR_INVALID_ARG, "ftdx_pack_triangle: NULL pointer");
+    if ((reinterpret_cast<uintptr_t>(full) | reinterpret_cast<uintptr_t>(packed) |
+         reinterpret_cast<uintptr_t>(old)) & 7u)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_pack_triangle: buffers must be 8-byte aligned");
+    hipError_t e = ftdx::launch_pack_triangle(static_cast<const float2*>(full), static_cast<float2*>(packed),
+                                              static_cast<const float2*>(old), N, batch_count, beta,
+                                              static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_pack_triangle: launch");
+}
+
+int ftdx_tri_from_herm(const void* full, void* tri, int N, int batch_count, int prec, void* stream) {
+    if (prec != FTDX_PREC_FP32 && prec != FTDX_PREC_FP64)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_tri_from_herm: prec must be FTDX_PREC_FP32 or _FP64");
+    if (N < 0 || batch_count < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_tri_from_herm: negative size");
+    if (N == 0 || batch_count == 0) return FTDX_OK;
+    if (!full || !tri) return fail(FTDX_ERR_INVALID_ARG, "ftdx_tri_from_herm: NULL pointer");
+    hipError_t e = ftdx::launch_tri_from_herm(full, tri, N, batch_count, prec == FTDX_PREC_FP64,
+                                              static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_tri_from_herm: launch");
+}
+
+int ftdx_promote_qc_to_fp32(const void* qc, void* out, int64_t n, void* stream) {
+    if (n < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp32: negative size");
+    if (n == 0) return FTDX_OK;
+    if (!qc || !out) return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp32: NULL pointer");
+    if (reinterpret_cast<uintptr_t>(out) & 7u)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_promote_qc_to_fp32: output must be 8-byte aligned");
+    hipError_t e = ftdx::launch_promote_qc_fp32(static_cast<const uint8_t*>(qc), static_cast<float2*>(out),
+                                                n, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_promote_qc_to_fp32: launch");
+}
+
+int ftdx_convert_qc_to_ftd(const void* qc, void* out, int64_t n, void* stream) {
+    if (n < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_convert_qc_to_ftd: negative size");
+    if (n == 0) return FTDX_OK;
+    if (!qc || !out) return fail(FTDX_ERR_INVALID_ARG, "ftdx_convert_qc_to_ftd: NULL pointer");
+    if ((reinterpret_cast<uintptr_t>(qc) | reinterpret_cast<uintptr_t>(out)) & 15u)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_convert_qc_to_ftd: buffers must be 16-byte aligned");
+    hipError_t e = ftdx::launch_convert_qc_to_ftd(static_cast<const uint8_t*>(qc),
+                                                  static_cast<uint8_t*>(out), n,
+                                                  static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_convert_qc_to_ftd: launch");
+}
+
+int ftdx_correlate_ftd(ftdx_context* ctx, const void* raw, void* tri_fp32, int n_ant, int n_chan,
+                       int n_time, int n_time_inner, int n_pol, int twos, void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_correlate_ftd: NULL context");
+    if (n_ant < 0 || n_chan < 0 || n_time < 0 || n_time_inner <= 0 || n_pol < 0)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_correlate_ftd: negative size or n_time_inner <= 0");
+    const int64_t batch = (int64_t)n_chan * n_pol * n_time_inner;
+    if (batch > 0x7FFFFFFF || (int64_t)n_time * n_ant * n_chan * n_pol > ((int64_t)1 << 40))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_correlate_ftd: sizes too large");
+    const int K = n_time / n_time_inner;
+    if (n_ant == 0 || batch == 0) return FTDX_OK;
+    if (!tri_fp32 || (!raw && K > 0)) return fail(FTDX_ERR_INVALID_ARG, "ftdx_correlate_ftd: NULL pointer");
+    int rc = ftdx::grow_precast(ctx, ftdx::rows_bytes(n_ant, K, (int)batch) + 16, "ftdx_correlate_ftd: rows");
+    if (rc != FTDX_OK) return rc;
+    const int cfg = ftdx::xengine_cfg(ctx->forced_cfg >= 0 ? ctx->forced_cfg : ftdx::direct_tile_to_cfg(ctx->direct_tile),
+                                      n_ant, (int)batch);
+    const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : 1;
+    hipError_t e = ftdx::run_ftd_payload(static_cast<const uint8_t*>(raw), static_cast<float*>(tri_fp32),
+                                         ctx->precast, n_ant, n_chan, n_time, n_time_inner, n_pol,
+                                         twos != 0, cfg, remap, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_correlate_ftd: kernels");
+}
+
+int ftdx_interleave_tri(const void* re, const void* im, void* out, int64_t n, int prec, void* stream) {
+    if (prec != FTDX_PREC_FP32 && prec != FTDX_PREC_FP64)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_interleave_tri: prec must be FTDX_PREC_FP32 or _FP64");
+    if (n < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_interleave_tri: negative size");
+    if (n == 0) return FTDX_OK;
+    if (!re || !im || !out) return fail(FTDX_ERR_INVALID_ARG, "ftdx_interleave_tri: NULL pointer");
+    hipError_t e = ftdx::launch_interleave_tri(static_cast<const uint16_t*>(re),
+                                               static_cast<const uint16_t*>(im), out, n,
+                                               prec == FTDX_PREC_FP64, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_interleave_tri: launch");
 }
 
 int ftdx_herk_each(ftdx_context* ctx, const ftdx_herk_problem* problems, int count, int output,

@@ -20,13 +20,22 @@ int tile_config_rows(int cfg);   // TM
 int tile_config_cols(int cfg);   // TN
 int tile_config_wg_per_cu(int cfg);
 
+// Output layout of a launch (element (i, j <= i) below is sum_k a_ik conj(a_jk)):
+//   OUT_LOWER_PACKED  (i, j) at i(i+1)/2 + j                   (the HERK layout)
+//   OUT_UPPER_PACKED  GGP: row q = j, column p = i at q*N + p - q(q+1)/2
+//   OUT_HERM_FULL     GGP: full M^H M row-major, (i, j) at j*N + i, conj at i*N + j
+enum OutFormat { OUT_LOWER_PACKED = 0, OUT_UPPER_PACKED = 1, OUT_HERM_FULL = 2 };
+
 struct HerkPlainArgs {
-    const uint8_t* A;   // [batch][N][K] QC bytes
+    const uint8_t* A;   // [batch][N][K] QC bytes, or pre-cast FP8 rows when precast != 0
     void* C;            // [batch][N(N+1)] scalars (float or half), packed lower triangle
+                        // (GGP launches: the layout and type launch_herk_ggp names)
     int N, K, batch;
     float alpha, beta;
     int64_t a_bstride;  // bytes of A per batch (N*K)
     int64_t c_bstride;  // scalars of C per batch (N*(N+1))
+    int precast = 0;    // 1: A is pre-cast FP8 (dword-planar), row pitch lda bytes
+    int64_t lda = 0;
 };
 
 // Lower-triangle tiles per problem for a configuration.
@@ -36,6 +45,12 @@ int herk_tiles(int cfg, int N);
 // xcd_remap != 0.
 hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, int cfg, bool out_fp16, int xcd_remap,
                                   hipStream_t stream);
+
+// GGP-convention outputs of a pre-cast (precast = 1) batch: fmt 1 = upper triangle
+// of M^H M packed row-major, fmt 2 = full N x N Hermitian row-major; element type
+// float or (out_fp64) double.  C batch stride c_bstride counts scalars.
+hipError_t launch_herk_ggp(const HerkPlainArgs& a, int cfg, int fmt, bool out_fp64, int xcd_remap,
+                           hipStream_t stream);
 
 // ---- grouped persistent launch ----
 struct GroupProblemDev {

@@ -32,7 +32,7 @@ __device__ __forceinline__ int xcd_logical(int w, int total) {
     return x * q + (x < r ? x : r) + local;
 }
 
-template <class Cfg, typename OutT, int MODE>
+template <class Cfg, typename OutT, int MODE, int FMT = OUT_LOWER_PACKED>
 __global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
 herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
@@ -51,7 +51,8 @@ herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
     j.col0 = J * Cfg::TN;
     j.alpha = p.alpha;
     j.beta = p.beta;
-    TileRunner<Cfg>::template run<OutT, MODE>(j, lds);
+    j.lda = p.lda;
+    TileRunner<Cfg>::template run<OutT, MODE, FMT>(j, lds);
 }
 
 template <class Cfg, typename OutT>
@@ -75,6 +76,7 @@ herk_int4_fp8_grouped_kernel(const GroupProblemDev* __restrict__ probs,
         j.col0 = J * Cfg::TN;
         j.alpha = pr.alpha;
         j.beta = pr.beta;
+        j.lda = pr.K;
         if (pr.vec)
             TileRunner<Cfg>::template run<OutT, LOAD_VEC16>(j, lds);
         else
@@ -90,6 +92,15 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
     const bool vec = (a.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15u) == 0);
     dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    if (a.precast) {
+        if (out_fp16)
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_FP8>), grid, block, 0,
+                               stream, a, tiles, xcd_remap);
+        else
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_FP8>), grid, block, 0,
+                               stream, a, tiles, xcd_remap);
+        return hipGetLastError();
+    }
     if (out_fp16) {
         if (vec)
             hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_VEC16>), grid, block,
@@ -104,6 +115,34 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
         else
             hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_BYTES>), grid, block, 0,
                                stream, a, tiles, xcd_remap);
+    }
+    return hipGetLastError();
+}
+
+// GGP layouts: pre-cast FP8 operand rows only.
+template <class Cfg>
+hipError_t launch_ggp_cfg(const HerkPlainArgs& a, int fmt, bool out_fp64, int xcd_remap,
+                          hipStream_t stream) {
+    const int tiles = tri_tiles(a.N, Cfg::TM, Cfg::TN);
+    const int64_t blocks = (int64_t)tiles * a.batch;
+    if (blocks > 0x7FFFFFFF || !a.precast) return hipErrorInvalidValue;
+    dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    if (fmt == OUT_UPPER_PACKED) {
+        if (out_fp64)
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, double, LOAD_FP8, OUT_UPPER_PACKED>),
+                               grid, block, 0, stream, a, tiles, xcd_remap);
+        else
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_FP8, OUT_UPPER_PACKED>),
+                               grid, block, 0, stream, a, tiles, xcd_remap);
+    } else if (fmt == OUT_HERM_FULL) {
+        if (out_fp64)
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, double, LOAD_FP8, OUT_HERM_FULL>),
+                               grid, block, 0, stream, a, tiles, xcd_remap);
+        else
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_FP8, OUT_HERM_FULL>),
+                               grid, block, 0, stream, a, tiles, xcd_remap);
+    } else {
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -173,6 +212,18 @@ hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, int cfg, bool out_fp16
         case TILE_128x128_K128: return launch_plain_cfg<Cfg1>(a, out_fp16, xcd_remap, stream);
         case TILE_64x64_K64: return launch_plain_cfg<Cfg2>(a, out_fp16, xcd_remap, stream);
         case TILE_256x128_K64: return launch_plain_cfg<Cfg3>(a, out_fp16, xcd_remap, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_herk_ggp(const HerkPlainArgs& a, int cfg, int fmt, bool out_fp64, int xcd_remap,
+                           hipStream_t stream) {
+    if (a.N <= 0 || a.batch <= 0) return hipSuccess;
+    switch (cfg) {
+        case TILE_128x128_K64: return launch_ggp_cfg<Cfg0>(a, fmt, out_fp64, xcd_remap, stream);
+        case TILE_128x128_K128: return launch_ggp_cfg<Cfg1>(a, fmt, out_fp64, xcd_remap, stream);
+        case TILE_64x64_K64: return launch_ggp_cfg<Cfg2>(a, fmt, out_fp64, xcd_remap, stream);
+        case TILE_256x128_K64: return launch_ggp_cfg<Cfg3>(a, fmt, out_fp64, xcd_remap, stream);
         default: return hipErrorInvalidValue;
     }
 }

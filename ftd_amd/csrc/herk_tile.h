@@ -16,6 +16,7 @@
 //   LDS (double-buffered, one barrier per stage) -> ds_read_b128 -> MFMA.
 #pragma once
 #include "herk_common.h"
+#include "herk_fp8.h"
 
 namespace ftdx {
 
@@ -40,6 +41,9 @@ struct TileCfg {
     static constexpr int SEGS_PER_ROW = KC / 16;      // 16-byte global segments per row
     static constexpr int SEGS = (TM + TN) * SEGS_PER_ROW;
     static constexpr int SEGS_PER_THREAD = SEGS / THREADS;
+    static constexpr int SEGS8_PER_ROW = KC / 8;       // 16-byte segments of pre-cast FP8 rows
+    static constexpr int SEGS8_PER_THREAD = (TM + TN) * SEGS8_PER_ROW / THREADS;
+    static_assert(((TM + TN) * SEGS8_PER_ROW) % THREADS == 0, "FP8 segments must divide over threads");
     static_assert(WM % 32 == 0 && WN % 32 == 0, "wave block must be a multiple of 32");
     static_assert(KC % 32 == 0, "stage must hold whole 32-sample MFMA steps");
     static_assert(SEGS % THREADS == 0, "segments must divide evenly over threads");
@@ -47,14 +51,18 @@ struct TileCfg {
     static_assert(LDS_BYTES * WG_PER_CU <= 160 * 1024, "LDS of resident workgroups exceeds 160 KB");
 };
 
-enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1 };
+// LOAD_VEC16 / LOAD_BYTES: A holds QC bytes [N][K] (row pitch K), unpacked in
+// the load stage.  LOAD_FP8: A holds pre-cast operand rows (row pitch lda bytes,
+// 8 bytes per 4 samples in the dword-planar order), copied to LDS as they are.
+enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1, LOAD_FP8 = 2 };
 
 struct TileJob {
-    const uint8_t* A;  // this problem's [N][K] QC bytes
+    const uint8_t* A;  // this problem's operand rows
     void* C;           // this problem's packed triangle (scalars)
     int N, K;
     int row0, col0;    // first row / column of the tile
     float alpha, beta;
+    int64_t lda;       // row pitch of A in bytes (LOAD_FP8 only)
 };
 
 // 4 QC bytes (4 samples) -> u dwords: re = fp8(Re0..Re3), im = fp8(Im0..Im3).
@@ -77,6 +85,18 @@ __device__ __forceinline__ v8i conj_planar(v8i a) {
     }
     return r;
 }
+
+// Output layouts of one problem's Gram matrix (element (i, j) below is
+// C_ij = sum_k a_ik conj(a_jk), the value the tile computes for j <= i):
+//   OUT_LOWER_PACKED  C_ij at i(i+1)/2 + j                      (cutlass_gemm_api herk)
+//   OUT_UPPER_PACKED  row q = j, column p = i >= q at q*N + p - q(q+1)/2, value C_ij:
+//                     GGP's upper triangle of M^H M, row-major (tri_from_herm.cuh:28-73,
+//                     xengine_test.cpp:282-287)
+//   OUT_HERM_FULL     the full N x N matrix M^H M row-major: C_ij at j*N + i and
+//                     conj(C_ij) at i*N + j (xengine_test.cpp:288-292)
+// The GGP layouts keep their stores coalesced by swapping the MFMA operands: the
+// accumulator then holds the transposed tile, so consecutive lanes own consecutive
+// i, which is the contiguous index of both GGP layouts.  (enum OutFormat: herk_fp8.h)
 
 template <typename OutT>
 struct OutIO;
@@ -121,6 +141,33 @@ struct OutIO<_Float16> {
     }
 };
 
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+// FP64 output: the FP32 accumulators hold exact integers (|sum| < 2^24), so the
+// widening is exact; alpha and beta are applied in double.
+template <>
+struct OutIO<double> {
+    static __device__ __forceinline__ void load(const void* C, int64_t idx, double& re, double& im) {
+        v2d v = reinterpret_cast<const v2d*>(C)[idx];
+        re = v[0];
+        im = v[1];
+    }
+    static __device__ __forceinline__ void store(void* C, int64_t idx, double re, double im) {
+        v2d v;
+        v[0] = re;
+        v[1] = im;
+        __builtin_nontemporal_store(v, reinterpret_cast<v2d*>(C) + idx);
+    }
+};
+
+__device__ __forceinline__ float fma_e(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_e(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename OutT>
+struct EpiType { typedef float type; };
+template <>
+struct EpiType<double> { typedef double type; };
+
 template <class Cfg>
 struct TileRunner {
     // Staged-row source for segment slot q of thread tid: global row, LDS row.
@@ -128,6 +175,35 @@ struct TileRunner {
         const int s = tid + Cfg::THREADS * q;
         lrow = s / Cfg::SEGS_PER_ROW;  // 0 .. TM+TN-1 (rows first, then columns)
         seg = s % Cfg::SEGS_PER_ROW;
+    }
+
+    // Pre-cast FP8 rows: segment slot q of thread tid covers 8 samples of one row.
+    static __device__ __forceinline__ void load_stage_fp8(const TileJob& j, int k0, bool share_cols,
+                                                          int tid, uint4 (&g)[Cfg::SEGS8_PER_THREAD]) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS8_PER_THREAD; ++q) {
+            const int s = tid + Cfg::THREADS * q;
+            const int lrow = s / Cfg::SEGS8_PER_ROW, seg = s % Cfg::SEGS8_PER_ROW;
+            const bool is_col = lrow >= Cfg::TM;
+            const int grow = is_col ? j.col0 + (lrow - Cfg::TM) : j.row0 + lrow;
+            const int kk = k0 + seg * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            // pre-cast rows are zero-padded to a multiple of 16 samples
+            if (!(is_col && share_cols) && grow < j.N && kk < j.K)
+                v = *reinterpret_cast<const uint4*>(j.A + (int64_t)grow * j.lda + (int64_t)kk * 2);
+            g[q] = v;
+        }
+    }
+
+    static __device__ __forceinline__ void store_stage_fp8(uint8_t* st, bool share_cols, int tid,
+                                                           const uint4 (&g)[Cfg::SEGS8_PER_THREAD]) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS8_PER_THREAD; ++q) {
+            const int s = tid + Cfg::THREADS * q;
+            const int lrow = s / Cfg::SEGS8_PER_ROW, seg = s % Cfg::SEGS8_PER_ROW;
+            if (lrow >= Cfg::TM && share_cols) continue;
+            *reinterpret_cast<uint4*>(st + lrow * Cfg::PITCH + seg * 16) = g[q];
+        }
     }
 
     template <int MODE>
@@ -186,7 +262,21 @@ struct TileRunner {
 
     // Runs one tile; all threads of the workgroup must call it (it uses barriers).
     // Ends with a barrier, so the caller may reuse `lds` right away.
-    template <typename OutT, int MODE>
+    template <typename OutT>
+    static __device__ __forceinline__ void put(const TileJob& j, int64_t idx, float are, float aim) {
+        typedef typename EpiType<OutT>::type E;
+        const E alpha = (E)j.alpha, beta = (E)j.beta;
+        E re = (E)are * alpha, im = (E)aim * alpha;
+        if (beta != (E)0) {
+            E ore, oim;
+            OutIO<OutT>::load(j.C, idx, ore, oim);
+            re = fma_e(beta, ore, re);  // one rounding, as the oracle's fma(beta, old, .)
+            im = fma_e(beta, oim, im);
+        }
+        OutIO<OutT>::store(j.C, idx, re, im);
+    }
+
+    template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED>
     static __device__ __forceinline__ void run(const TileJob& j, uint8_t* __restrict__ lds) {
         const int tid = threadIdx.x;
         const int lane = tid & 63, wave = tid >> 6;
@@ -211,10 +301,19 @@ struct TileRunner {
                 }
 
         const int nst = (j.K + Cfg::KC - 1) / Cfg::KC;
-        uint4 g[Cfg::SEGS_PER_THREAD];
+        constexpr int NG = MODE == LOAD_FP8 ? Cfg::SEGS8_PER_THREAD : Cfg::SEGS_PER_THREAD;
+        uint4 g[NG];
+        auto load = [&](int k0) {
+            if constexpr (MODE == LOAD_FP8) load_stage_fp8(j, k0, share_cols, tid, g);
+            else load_stage<MODE>(j, k0, share_cols, tid, g);
+        };
+        auto store = [&](uint8_t* st) {
+            if constexpr (MODE == LOAD_FP8) store_stage_fp8(st, share_cols, tid, g);
+            else store_stage(st, share_cols, tid, g);
+        };
         if (nst > 0) {
-            load_stage<MODE>(j, 0, share_cols, tid, g);
-            store_stage(lds, share_cols, tid, g);
+            load(0);
+            store(lds);
         }
         __syncthreads();
 
@@ -225,7 +324,7 @@ struct TileRunner {
 
         for (int s = 0; s < nst; ++s) {
             const bool more = (s + 1 < nst);
-            if (more) load_stage<MODE>(j, (s + 1) * Cfg::KC, share_cols, tid, g);
+            if (more) load((s + 1) * Cfg::KC);
             const uint8_t* st = lds + (s & 1) * Cfg::STAGE;
             if (active) {
 #pragma unroll
@@ -242,44 +341,48 @@ struct TileRunner {
                         const v8i av = conj_planar(au[a]);
 #pragma unroll
                         for (int b = 0; b < Cfg::FN; ++b) {
-                            acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
-                            acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
+                            if constexpr (FMT == OUT_LOWER_PACKED) {
+                                acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
+                                acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
+                            } else {  // transposed accumulator, same values
+                                acc_re[a][b] = mfma_32x32x64_fp8(bu[b], au[a], acc_re[a][b]);
+                                acc_im[a][b] = mfma_32x32x64_fp8(bu[b], av, acc_im[a][b]);
+                            }
                         }
                     }
                 }
             }
-            if (more) store_stage(lds + ((s + 1) & 1) * Cfg::STAGE, share_cols, tid, g);
+            if (more) store(lds + ((s + 1) & 1) * Cfg::STAGE);
             __syncthreads();
         }
 
         if (!active) return;
 
         // Epilogue. 32x32 accumulator layout: lane holds column (lane & 31); register
-        // r holds row 8*(r/4) + 4*(lane/32) + (r%4).  Re and Im of one (i, j) sit in
-        // the same lane and register, so each lane writes interleaved [re, im].
-        const float alpha = j.alpha, beta = j.beta;
+        // r holds row 8*(r/4) + 4*(lane/32) + (r%4) -- of (i, j) for the lower-packed
+        // layout, of (j, i) for the transposed GGP layouts.  Re and Im of one element
+        // sit in the same lane and register, so each lane writes interleaved [re, im].
         const int n = lane & 31, g4 = (lane >> 5) * 4;
 #pragma unroll
         for (int a = 0; a < Cfg::FM; ++a)
 #pragma unroll
             for (int b = 0; b < Cfg::FN; ++b) {
-                const int gj = wcol0 + b * 32 + n;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const int gi = wrow0 + a * 32 + (r >> 2) * 8 + g4 + (r & 3);
+                    const int rr = (r >> 2) * 8 + g4 + (r & 3);
+                    const int gi = wrow0 + a * 32 + (FMT == OUT_LOWER_PACKED ? rr : n);
+                    const int gj = wcol0 + b * 32 + (FMT == OUT_LOWER_PACKED ? n : rr);
                     if (gi < j.N && gj <= gi) {
-                        const int64_t idx = tri_index(gi, gj);
-                        float re = acc_re[a][b][r];
-                        float im = (gi == gj) ? 0.0f : acc_im[a][b][r];
-                        re = re * alpha;
-                        im = im * alpha;
-                        if (beta != 0.0f) {
-                            float ore, oim;
-                            OutIO<OutT>::load(j.C, idx, ore, oim);
-                            re = __builtin_fmaf(beta, ore, re);
-                            im = __builtin_fmaf(beta, oim, im);
+                        const float re = acc_re[a][b][r];
+                        const float im = (gi == gj) ? 0.0f : acc_im[a][b][r];
+                        if constexpr (FMT == OUT_LOWER_PACKED) {
+                            put<OutT>(j, tri_index(gi, gj), re, im);
+                        } else if constexpr (FMT == OUT_UPPER_PACKED) {
+                            put<OutT>(j, (int64_t)gj * j.N + gi - ((int64_t)gj * (gj + 1) >> 1), re, im);
+                        } else {
+                            put<OutT>(j, (int64_t)gj * j.N + gi, re, im);
+                            if (gi != gj) put<OutT>(j, (int64_t)gi * j.N + gj, re, -im);
                         }
-                        OutIO<OutT>::store(j.C, idx, re, im);
                     }
                 }
             }

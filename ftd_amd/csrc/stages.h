@@ -1,0 +1,37 @@
+// stages.h -- host-side interface of the stand-alone stage kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ftdx {
+
+// Samples per pre-cast operand row (K rounded up to 16); row pitch = 2 * that bytes.
+int precast_row_samples(int K);
+
+hipError_t launch_promote_qc_fp8_interleaved(const uint8_t* in, uint8_t* out, int64_t n,
+                                             hipStream_t s);
+hipError_t launch_promote_qc_fp8_planar(const uint8_t* in, uint8_t* out, int64_t rows, int K,
+                                        hipStream_t s);
+hipError_t launch_promote_fp16_fp8_planar(const uint16_t* in, uint8_t* out, int64_t rows, int K,
+                                          hipStream_t s);
+hipError_t launch_promote_qc_fp16_planar(const uint8_t* in, uint16_t* re, uint16_t* im, int64_t n,
+                                         hipStream_t s);
+hipError_t launch_promote_data(const uint8_t* in, void* out, int64_t n, bool fp64, hipStream_t s);
+hipError_t launch_pack_triangle(const float2* full, float2* packed, const float2* old, int N,
+                                int batch, float beta, hipStream_t s);
+hipError_t launch_tri_from_herm(const void* full, void* tri, int N, int batch, bool fp64,
+                                hipStream_t s);
+hipError_t launch_promote_qc_fp32(const uint8_t* in, float2* out, int64_t n, hipStream_t s);
+hipError_t launch_convert_qc_to_ftd(const uint8_t* in, uint8_t* out, int64_t n, hipStream_t s);
+// GGP [batch][K][N] two's-complement bytes -> pre-cast rows [batch][N][Kp] (pitch 2*Kp B).
+hipError_t launch_ggp_to_fp8_rows(const uint8_t* in, uint8_t* out, int N, int K, int batch,
+                                  hipStream_t s);
+// FTD raw [time][ant][chan][pol] bytes (time = k*n_time_inner + ti) -> pre-cast rows
+// [batch][ant][Kp], batch = (chan*n_pol + pol)*n_time_inner + ti, K = n_time / n_time_inner.
+// twos: two's complement nibbles, low = Re; else QC sign-magnitude, high = Re.
+hipError_t launch_ftd_to_fp8_rows(const uint8_t* in, uint8_t* out, int n_ant, int n_chan, int n_time,
+                                  int n_time_inner, int n_pol, bool twos, hipStream_t s);
+hipError_t launch_interleave_tri(const uint16_t* re, const uint16_t* im, void* out, int64_t n,
+                                 bool fp64, hipStream_t s);
+
+}  // namespace ftdx

@@ -1,0 +1,557 @@
+// stages.hip -- stand-alone (unfused) stage kernels of the X-engine path on gfx950.
+//
+// The fused HERK kernel does all of these inside its load stage / epilogue; the
+// stand-alone forms serve callers that use the stages directly (the reference's
+// GGP element-wise kernels and CUTLASS-path casts), the unfused HerkMode path,
+// and FP16 input.  All are HBM-bound: 16-byte vector loads/stores where the
+// layout allows, one grid-stride pass.
+//
+//   promote_qc_fp8_interleaved   QC -> [fp8 Re, fp8 Im] bytes     (cutlass_gemm_api.cu:1566-1623)
+//   promote_qc_fp8_planar        QC -> pre-cast operand rows (dword-planar FP8)
+//   promote_fp16_fp8_planar      FP16 [re, im] -> pre-cast rows, clamp +-448 then
+//                                nearest-even E4M3 (cast_kernels_common.hpp:30-48)
+//   promote_qc_fp16_planar       QC -> FP16 Re plane + Im plane   (cutlass_gemm_api.cu:1639-1698)
+//   promote_data                 GGP (4,4)-bit two's complement, low nibble = Re ->
+//                                FP32/FP64 [re, im]               (include/kernels/promote_data.cuh:70-88)
+//   pack_triangle                N x N [re, im] FP32 -> packed lower triangle (+ beta old)
+//                                                                 (shared/pack_kernels_common.hpp:602-659)
+//   tri_from_herm                N x N complex -> packed upper row-major  (include/kernels/tri_from_herm.cuh:28-73)
+//   interleave_tri               FP16 Re/Im planes -> [re, im] FP32/FP64  (include/kernels/interleave_tri.cuh:27-42)
+//   ggp_to_fp8_rows              GGP XEngine input [batch][K][N] (two's complement, low
+//                                nibble = Re; lib/xengine.cpp:150-160, promote_data.cuh:80-81)
+//                                -> pre-cast operand rows [batch][N][Kp], transposed
+//   ftd_to_fp8_rows              FTD raw order [time][ant][chan][pol] (QC or two's
+//                                complement) -> pre-cast rows [batch][ant][Kp] with
+//                                batch = (chan*n_pol + pol)*n_ti + ti (lib/tcc_interface.cu:124-178,
+//                                visibility_pipeline_test.cpp:54-79)
+#include <hip/hip_runtime.h>
+
+#include "herk_common.h"
+#include "herk_tile.h"
+#include "stages.h"
+
+namespace ftdx {
+
+namespace {
+
+constexpr int ST_THREADS = 256;
+
+unsigned grid_for(int64_t work_items) {
+    int64_t g = (work_items + ST_THREADS - 1) / ST_THREADS;
+    if (g > 65536) g = 65536;  // grid-stride beyond this
+    return (unsigned)(g > 0 ? g : 1);
+}
+
+// 4 QC bytes -> [re0, im0, re1, im1 | re2, im2, re3, im3] FP8 bytes (reference layout)
+__device__ __forceinline__ void qc4_to_fp8_interleaved(uint32_t x, uint32_t& o0, uint32_t& o1) {
+    uint32_t re, im;
+    qc4_to_fp8_planar(x, re, im);
+    o0 = __builtin_amdgcn_perm(im, re, 0x05010400u);
+    o1 = __builtin_amdgcn_perm(im, re, 0x07030602u);
+}
+
+__global__ void __launch_bounds__(ST_THREADS)
+promote_qc_fp8_interleaved_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                  int64_t n) {
+    const int64_t n16 = n / 16;
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < n16; t += stride) {
+        const uint4 v = reinterpret_cast<const uint4*>(in)[t];
+        uint4 a, b;
+        qc4_to_fp8_interleaved(v.x, a.x, a.y);
+        qc4_to_fp8_interleaved(v.y, a.z, a.w);
+        qc4_to_fp8_interleaved(v.z, b.x, b.y);
+        qc4_to_fp8_interleaved(v.w, b.z, b.w);
+        reinterpret_cast<uint4*>(out)[2 * t] = a;
+        reinterpret_cast<uint4*>(out)[2 * t + 1] = b;
+    }
+    // tail (< 16 samples), one thread per sample
+    const int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x;
+    if (t < n - n16 * 16) {
+        const int64_t i = n16 * 16 + t;
+        uint32_t re, im;
+        qc4_to_fp8_planar(in[i], re, im);
+        out[2 * i] = (uint8_t)(re & 0xFF);
+        out[2 * i + 1] = (uint8_t)(im & 0xFF);
+    }
+}
+
+// QC rows [rows][K] -> pre-cast rows [rows][Kp] (Kp = K rounded up to 16 samples),
+// 16 samples (32 output bytes) per thread.
+template <bool VEC>
+__global__ void __launch_bounds__(ST_THREADS)
+promote_qc_fp8_planar_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                             int64_t rows, int K, int Kp) {
+    const int segs = Kp / 16;
+    const int64_t total = rows * segs;
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride) {
+        const int64_t r = t / segs;
+        const int s = (int)(t - r * segs);
+        const int k0 = s * 16;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        const uint8_t* src = in + r * K + k0;
+        if (VEC) {
+            const uint4 v = *reinterpret_cast<const uint4*>(src);
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+            for (int e = 0; e < 16; ++e)
+                if (k0 + e < K) w[e >> 2] |= (uint32_t)src[e] << (8 * (e & 3));
+        }
+        uint4 a, b;
+        qc4_to_fp8_planar(w[0], a.x, a.y);
+        qc4_to_fp8_planar(w[1], a.z, a.w);
+        qc4_to_fp8_planar(w[2], b.x, b.y);
+        qc4_to_fp8_planar(w[3], b.z, b.w);
+        uint4* dst = reinterpret_cast<uint4*>(out + r * (int64_t)Kp * 2 + (int64_t)k0 * 2);
+        dst[0] = a;
+        dst[1] = b;
+    }
+}
+
+// float -> OCP E4M3 byte, nearest-even, inputs already clamped to +-448
+__device__ __forceinline__ uint32_t f32_to_e4m3(float x) {
+    const uint32_t sign = __builtin_signbitf(x) ? 0x80u : 0u;
+    const float a = __builtin_fabsf(x);
+    uint32_t code;
+    int e;
+    __builtin_frexpf(a, &e);
+    e -= 1;
+    if (a == 0.0f || e < -6) {
+        code = (uint32_t)__builtin_rintf(__builtin_ldexpf(a, 9));  // subnormal step 2^-9
+    } else {
+        uint32_t q = (uint32_t)__builtin_rintf(__builtin_ldexpf(a, 3 - e));
+        if (q == 16u) {
+            q = 8u;
+            ++e;
+        }
+        code = ((uint32_t)(e + 7) << 3) | (q - 8u);
+    }
+    if (code > 0x7Eu) code = 0x7Eu;  // 448: largest finite E4M3
+    return sign | code;
+}
+
+__device__ __forceinline__ float half_bits_to_float(uint32_t h) {
+    _Float16 v;
+    const uint16_t b = (uint16_t)h;
+    __builtin_memcpy(&v, &b, 2);
+    return (float)v;
+}
+
+__device__ __forceinline__ uint32_t fp16_to_e4m3_clamped(uint32_t h) {
+    float f = half_bits_to_float(h);
+    f = __builtin_fminf(__builtin_fmaxf(f, -448.0f), 448.0f);
+    return f32_to_e4m3(f);
+}
+
+// FP16 [rows][K][2] -> pre-cast rows; 4 samples (16 input bytes, 8 output bytes) per thread
+__global__ void __launch_bounds__(ST_THREADS)
+promote_fp16_fp8_planar_kernel(const uint16_t* __restrict__ in, uint8_t* __restrict__ out,
+                               int64_t rows, int K, int Kp) {
+    const int groups = Kp / 4;
+    const int64_t total = rows * groups;
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride) {
+        const int64_t r = t / groups;
+        const int gi = (int)(t - r * groups);
+        uint32_t re = 0u, im = 0u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k = gi * 4 + e;
+            if (k < K) {
+                const uint16_t* p = in + (r * K + k) * 2;
+                re |= fp16_to_e4m3_clamped(p[0]) << (8 * e);
+                im |= fp16_to_e4m3_clamped(p[1]) << (8 * e);
+            }
+        }
+        uint2* dst = reinterpret_cast<uint2*>(out + r * (int64_t)Kp * 2 + (int64_t)gi * 8);
+        *dst = make_uint2(re, im);
+    }
+}
+
+__device__ __forceinline__ uint16_t qc_nibble_fp16(uint32_t n) {
+    // sign-magnitude nibble -> FP16 bits (the -0 nibble gives -0.0, as the reference LUT)
+    const uint32_t m = n & 7u;
+    uint32_t bits = 0u;
+    if (m) {
+        const int e = 31 - __builtin_clz(m);  // floor(log2 m)
+        bits = ((uint32_t)(e + 15) << 10) | ((m << (10 - e)) & 0x3FFu);
+    }
+    return (uint16_t)(bits | ((n & 8u) << 12));
+}
+
+__global__ void __launch_bounds__(ST_THREADS)
+promote_qc_fp16_planar_kernel(const uint8_t* __restrict__ in, uint16_t* __restrict__ re,
+                              uint16_t* __restrict__ im, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t i = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; i < n; i += stride) {
+        const uint32_t b = in[i];
+        re[i] = qc_nibble_fp16(b >> 4);
+        im[i] = qc_nibble_fp16(b & 15u);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ST_THREADS)
+promote_data_kernel(const uint8_t* __restrict__ in, T* __restrict__ out, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t i = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; i < n; i += stride) {
+        const int b = in[i];
+        const int lo = ((b & 15) ^ 8) - 8;  // two's complement nibble
+        const int hi = (((b >> 4) & 15) ^ 8) - 8;
+        out[2 * i] = (T)lo;
+        out[2 * i + 1] = (T)hi;
+    }
+}
+
+// one row per block (grid N x batch), as the reference's row-per-block pack
+__global__ void __launch_bounds__(ST_THREADS)
+pack_triangle_kernel(const float2* __restrict__ full, float2* __restrict__ packed,
+                     const float2* __restrict__ old, int N, float beta) {
+    const int row = (int)blockIdx.x, b = (int)blockIdx.y;
+    const float2* src = full + ((int64_t)b * N + row) * N;
+    const int64_t base = (int64_t)b * N * (N + 1) / 2 + tri_index(row, 0);
+    for (int col = threadIdx.x; col <= row; col += ST_THREADS) {
+        float2 v = src[col];
+        if (beta != 0.0f && old) {
+            const float2 o = old[base + col];
+            v.x = __builtin_fmaf(beta, o.x, v.x);
+            v.y = __builtin_fmaf(beta, o.y, v.y);
+        }
+        packed[base + col] = v;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ST_THREADS)
+tri_from_herm_kernel(const T* __restrict__ full, T* __restrict__ tri, int N) {
+    const int row = (int)blockIdx.x, b = (int)blockIdx.y;
+    const T* src = full + (((int64_t)b * N + row) * N) * 2;
+    // upper triangle, row-major: (row, col >= row) at row*N - row(row-1)/2 + col - row
+    T* dst = tri + ((int64_t)b * N * (N + 1) / 2 + (int64_t)row * N - (int64_t)row * (row - 1) / 2 - row) * 2;
+    for (int col = row + threadIdx.x; col < N; col += ST_THREADS) {
+        dst[2 * col] = src[2 * col];
+        dst[2 * col + 1] = src[2 * col + 1];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ST_THREADS)
+interleave_tri_kernel(const uint16_t* __restrict__ re, const uint16_t* __restrict__ im,
+                      T* __restrict__ out, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t i = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; i < n; i += stride) {
+        out[2 * i] = (T)half_bits_to_float(re[i]);
+        out[2 * i + 1] = (T)half_bits_to_float(im[i]);
+    }
+}
+
+// QC sign-magnitude (high nibble Re) -> interleaved FP32 [re, im]; the -0 nibble
+// gives +0 (integer product first, as PromoteQcSm, include/kernels/visibility_bf.cuh:38-56)
+__global__ void __launch_bounds__(ST_THREADS)
+promote_qc_fp32_kernel(const uint8_t* __restrict__ in, float2* __restrict__ out, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t i = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; i < n; i += stride) {
+        const int b = in[i];
+        const int re = ((b >> 4) & 8) ? -((b >> 4) & 7) : ((b >> 4) & 7);
+        const int im = (b & 8) ? -(b & 7) : (b & 7);
+        out[i] = make_float2((float)re, (float)im);
+    }
+}
+
+// QC sign-magnitude (high = Re) -> FTD two's complement nibbles (low = Re, high = Im),
+// 16 bytes per thread; in == out allowed.
+__device__ __forceinline__ uint32_t qc_to_ftd4(uint32_t x) {
+    // per byte: re = sm(hi nibble), im = sm(lo nibble); out = (im & 15) << 4 | (re & 15)
+    const uint32_t mag_hi = (x >> 4) & 0x07070707u, neg_hi = (x >> 7) & 0x01010101u;
+    const uint32_t mag_lo = x & 0x07070707u, neg_lo = (x >> 3) & 0x01010101u;
+    // two's complement of a 4-bit field: (m ^ 0xF) + 1 for negative fields.  m <= 7
+    // gives at most 0x10 per byte (the -0 case, m = 0), so no carry reaches the next
+    // byte and the final mask drops bit 4.
+    const uint32_t re = ((mag_hi ^ (neg_hi * 0x0Fu)) + neg_hi) & 0x0F0F0F0Fu;
+    const uint32_t im = ((mag_lo ^ (neg_lo * 0x0Fu)) + neg_lo) & 0x0F0F0F0Fu;
+    return re | (im << 4);
+}
+
+__global__ void __launch_bounds__(ST_THREADS)
+convert_qc_to_ftd_kernel(const uint8_t* in, uint8_t* out, int64_t n) {
+    const int64_t n16 = n / 16;
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < n16; t += stride) {
+        uint4 v = reinterpret_cast<const uint4*>(in)[t];
+        v.x = qc_to_ftd4(v.x);
+        v.y = qc_to_ftd4(v.y);
+        v.z = qc_to_ftd4(v.z);
+        v.w = qc_to_ftd4(v.w);
+        reinterpret_cast<uint4*>(out)[t] = v;
+    }
+    const int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x;
+    if (t < n - n16 * 16) {
+        const int64_t i = n16 * 16 + t;
+        out[i] = (uint8_t)(qc_to_ftd4(in[i]) & 0xFFu);
+    }
+}
+
+// ---- GGP XEngine input: [batch][K][N] two's-complement bytes -> pre-cast rows ----
+
+// 4 GGP bytes (4 samples; low nibble = Re, high = Im, two's complement in [-8, 7])
+// -> u dwords re = fp8(Re0..Re3), im = fp8(Im0..Im3).  A 16-entry byte table over
+// two v_perm lookups (nibbles 0-7 and 8-15) and a per-byte select on bit 3.
+__device__ __forceinline__ uint32_t ggp_nib4_to_fp8(uint32_t nib) {
+    const uint32_t P0 = 0x44403800u, P1 = 0x4E4C4A48u;  // fp8(0, 1, 2, 3), fp8(4 .. 7)
+    const uint32_t M0 = 0xCACCCED0u, M1 = 0xB8C0C4C8u;  // fp8(-8, -7, -6, -5), fp8(-4 .. -1)
+    const uint32_t sel = nib & 0x07070707u;
+    const uint32_t pos = __builtin_amdgcn_perm(P1, P0, sel);
+    const uint32_t neg = __builtin_amdgcn_perm(M1, M0, sel);
+    const uint32_t m = ((nib >> 3) & 0x01010101u) * 0xFFu;
+    return (pos & ~m) | (neg & m);
+}
+
+// d0..d3: one dword per sample holding 4 antennas' bytes -> one dword per antenna
+// holding its 4 samples' bytes (byte a of dword s goes to byte s of output a).
+__device__ __forceinline__ void transpose4x4_bytes(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3,
+                                                   uint32_t (&o)[4]) {
+    const uint32_t l01 = __builtin_amdgcn_perm(d1, d0, 0x05010400u);  // d0.b0 d1.b0 d0.b1 d1.b1
+    const uint32_t h01 = __builtin_amdgcn_perm(d1, d0, 0x07030602u);  // d0.b2 d1.b2 d0.b3 d1.b3
+    const uint32_t l23 = __builtin_amdgcn_perm(d3, d2, 0x05010400u);
+    const uint32_t h23 = __builtin_amdgcn_perm(d3, d2, 0x07030602u);
+    o[0] = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+    o[1] = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+    o[2] = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+    o[3] = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
+}
+
+// Transposing converters: the input holds, for every sample k, a run of R
+// contiguous bytes (index r) that belong to R different operand rows; the output
+// is pre-cast operand rows [row][Kp] (pitch 2*Kp bytes, samples K .. Kp-1 zero).
+//   GGP XEngine  (outer o = batch b):        byte at b*K*N + k*N + r,
+//                r = antenna, row = b*N + r
+//   FTD raw      (outer o = ti*n_ant + ant): byte at (k*n_ti + ti)*n_ant*C + ant*C + r,
+//                C = n_chan*n_pol, r = chan*n_pol + pol,
+//                row = (r*n_ti + ti)*n_ant + ant   (batch = (chan*n_pol + pol)*n_ti + ti)
+struct RowsLayout {
+    int R;            // contiguous run length (row-index extent)
+    int K, Kp;
+    int outers;       // number of outer indices
+    int64_t k_stride; // bytes between samples k and k+1
+    int ftd;          // 0: GGP layout, 1: FTD raw layout
+    int n_ti, n_ant;  // FTD only
+    __device__ int64_t in_base(int o) const {
+        if (!ftd) return (int64_t)o * K * R;
+        const int ti = o / n_ant, ant = o - ti * n_ant;
+        return (int64_t)ti * n_ant * R + (int64_t)ant * R;
+    }
+    __device__ int64_t out_row(int o, int r) const {
+        if (!ftd) return (int64_t)o * R + r;
+        const int ti = o / n_ant, ant = o - ti * n_ant;
+        return ((int64_t)r * n_ti + ti) * n_ant + ant;
+    }
+};
+
+// Thread = 4 row indices x 16 samples.  Within a wave 16 lanes run along r (64-byte
+// reads of each sample's run) and 4 along samples (128-byte runs of each output
+// row).  Block: 64 quads of r x 64 samples; grid.z strides over the outer index.
+// TWOS: two's complement nibbles with low = Re (GGP / FTD); else QC sign-magnitude
+// with high = Re.
+template <bool ALIGNED, bool TWOS>
+__global__ void __launch_bounds__(ST_THREADS)
+transpose_to_fp8_rows_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, RowsLayout L) {
+    const int tid = threadIdx.x;
+    const int rq = (tid & 15) + 16 * (tid >> 6);
+    const int kg = (tid >> 4) & 3;
+    const int r0 = ((int)blockIdx.y * 64 + rq) * 4;
+    const int k0 = ((int)blockIdx.x * 4 + kg) * 16;
+    if (r0 >= L.R || k0 >= L.Kp) return;
+    for (int o = blockIdx.z; o < L.outers; o += gridDim.z) {
+        const uint8_t* src = in + L.in_base(o) + r0;
+        uint32_t d[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int k = k0 + s;
+            uint32_t v = 0u;
+            if (k < L.K) {
+                const uint8_t* run = src + (int64_t)k * L.k_stride;
+                if (ALIGNED) {
+                    v = *reinterpret_cast<const uint32_t*>(run);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+                        if (r0 + a < L.R) v |= (uint32_t)run[a] << (8 * a);
+                }
+            }
+            d[s] = v;
+        }
+        uint32_t t[4][4];  // [sample group][row index]
+#pragma unroll
+        for (int g = 0; g < 4; ++g) transpose4x4_bytes(d[4 * g], d[4 * g + 1], d[4 * g + 2], d[4 * g + 3], t[g]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            if (r0 + a >= L.R) break;
+            uint32_t w[8];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                if (TWOS) {
+                    w[2 * g] = ggp_nib4_to_fp8(t[g][a] & 0x0F0F0F0Fu);             // Re: low nibbles
+                    w[2 * g + 1] = ggp_nib4_to_fp8((t[g][a] >> 4) & 0x0F0F0F0Fu);  // Im: high nibbles
+                } else {
+                    qc4_to_fp8_planar(t[g][a], w[2 * g], w[2 * g + 1]);
+                }
+            }
+            // samples >= K were read as 0x00 bytes, which both codes map to fp8 +0
+            uint4* dst = reinterpret_cast<uint4*>(out + L.out_row(o, r0 + a) * (int64_t)L.Kp * 2 +
+                                                  (int64_t)k0 * 2);
+            dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+            dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        }
+    }
+}
+
+template <bool TWOS>
+hipError_t launch_transpose_rows(const uint8_t* in, uint8_t* out, const RowsLayout& L, hipStream_t s) {
+    dim3 grid((unsigned)((L.Kp + 63) / 64), (unsigned)((L.R + 255) / 256),
+              (unsigned)(L.outers < 65535 ? L.outers : 65535));
+    const bool aligned = (L.R % 4 == 0) && (L.k_stride % 4 == 0) &&
+                         ((reinterpret_cast<uintptr_t>(in) & 3u) == 0);
+    if (aligned)
+        hipLaunchKernelGGL((transpose_to_fp8_rows_kernel<true, TWOS>), grid, dim3(ST_THREADS), 0, s, in,
+                           out, L);
+    else
+        hipLaunchKernelGGL((transpose_to_fp8_rows_kernel<false, TWOS>), grid, dim3(ST_THREADS), 0, s, in,
+                           out, L);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_ggp_to_fp8_rows(const uint8_t* in, uint8_t* out, int N, int K, int batch,
+                                  hipStream_t s) {
+    if (N <= 0 || K <= 0 || batch <= 0) return hipSuccess;
+    RowsLayout L;
+    L.R = N;
+    L.K = K;
+    L.Kp = precast_row_samples(K);
+    L.outers = batch;
+    L.k_stride = N;
+    L.ftd = 0;
+    L.n_ti = 1;
+    L.n_ant = 1;
+    return launch_transpose_rows<true>(in, out, L, s);
+}
+
+hipError_t launch_ftd_to_fp8_rows(const uint8_t* in, uint8_t* out, int n_ant, int n_chan, int n_time,
+                                  int n_time_inner, int n_pol, bool twos, hipStream_t s) {
+    const int K = n_time / n_time_inner;
+    if (n_ant <= 0 || n_chan <= 0 || n_pol <= 0 || K <= 0) return hipSuccess;
+    RowsLayout L;
+    L.R = n_chan * n_pol;
+    L.K = K;
+    L.Kp = precast_row_samples(K);
+    L.outers = n_time_inner * n_ant;
+    L.k_stride = (int64_t)n_time_inner * n_ant * n_chan * n_pol;
+    L.ftd = 1;
+    L.n_ti = n_time_inner;
+    L.n_ant = n_ant;
+    return twos ? launch_transpose_rows<true>(in, out, L, s) : launch_transpose_rows<false>(in, out, L, s);
+}
+
+hipError_t launch_promote_qc_fp32(const uint8_t* in, float2* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(promote_qc_fp32_kernel, dim3(grid_for(n)), dim3(ST_THREADS), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_qc_to_ftd(const uint8_t* in, uint8_t* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(convert_qc_to_ftd_kernel, dim3(grid_for(n / 16 + 16)), dim3(ST_THREADS), 0, s,
+                       in, out, n);
+    return hipGetLastError();
+}
+
+int precast_row_samples(int K) { return (K + 15) / 16 * 16; }
+
+hipError_t launch_promote_qc_fp8_interleaved(const uint8_t* in, uint8_t* out, int64_t n,
+                                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(promote_qc_fp8_interleaved_kernel, dim3(grid_for(n / 16 + 16)),
+                       dim3(ST_THREADS), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_promote_qc_fp8_planar(const uint8_t* in, uint8_t* out, int64_t rows, int K,
+                                        hipStream_t s) {
+    if (rows <= 0 || K <= 0) return hipSuccess;
+    const int Kp = precast_row_samples(K);
+    const int64_t work = rows * (Kp / 16);
+    const bool vec = (K % 16 == 0) && ((reinterpret_cast<uintptr_t>(in) & 15u) == 0);
+    if (vec)
+        hipLaunchKernelGGL((promote_qc_fp8_planar_kernel<true>), dim3(grid_for(work)), dim3(ST_THREADS),
+                           0, s, in, out, rows, K, Kp);
+    else
+        hipLaunchKernelGGL((promote_qc_fp8_planar_kernel<false>), dim3(grid_for(work)), dim3(ST_THREADS),
+                           0, s, in, out, rows, K, Kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_promote_fp16_fp8_planar(const uint16_t* in, uint8_t* out, int64_t rows, int K,
+                                          hipStream_t s) {
+    if (rows <= 0 || K <= 0) return hipSuccess;
+    const int Kp = precast_row_samples(K);
+    hipLaunchKernelGGL(promote_fp16_fp8_planar_kernel, dim3(grid_for(rows * (Kp / 4))),
+                       dim3(ST_THREADS), 0, s, in, out, rows, K, Kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_promote_qc_fp16_planar(const uint8_t* in, uint16_t* re, uint16_t* im, int64_t n,
+                                         hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(promote_qc_fp16_planar_kernel, dim3(grid_for(n)), dim3(ST_THREADS), 0, s, in,
+                       re, im, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_promote_data(const uint8_t* in, void* out, int64_t n, bool fp64, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (fp64)
+        hipLaunchKernelGGL((promote_data_kernel<double>), dim3(grid_for(n)), dim3(ST_THREADS), 0, s,
+                           in, static_cast<double*>(out), n);
+    else
+        hipLaunchKernelGGL((promote_data_kernel<float>), dim3(grid_for(n)), dim3(ST_THREADS), 0, s,
+                           in, static_cast<float*>(out), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_triangle(const float2* full, float2* packed, const float2* old, int N,
+                                int batch, float beta, hipStream_t s) {
+    if (N <= 0 || batch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_triangle_kernel, dim3((unsigned)N, (unsigned)batch), dim3(ST_THREADS), 0,
+                       s, full, packed, old, N, beta);
+    return hipGetLastError();
+}
+
+hipError_t launch_tri_from_herm(const void* full, void* tri, int N, int batch, bool fp64,
+                                hipStream_t s) {
+    if (N <= 0 || batch <= 0) return hipSuccess;
+    if (fp64)
+        hipLaunchKernelGGL((tri_from_herm_kernel<double>), dim3((unsigned)N, (unsigned)batch),
+                           dim3(ST_THREADS), 0, s, static_cast<const double*>(full),
+                           static_cast<double*>(tri), N);
+    else
+        hipLaunchKernelGGL((tri_from_herm_kernel<float>), dim3((unsigned)N, (unsigned)batch),
+                           dim3(ST_THREADS), 0, s, static_cast<const float*>(full),
+                           static_cast<float*>(tri), N);
+    return hipGetLastError();
+}
+
+hipError_t launch_interleave_tri(const uint16_t* re, const uint16_t* im, void* out, int64_t n,
+                                 bool fp64, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (fp64)
+        hipLaunchKernelGGL((interleave_tri_kernel<double>), dim3(grid_for(n)), dim3(ST_THREADS), 0, s,
+                           re, im, static_cast<double*>(out), n);
+    else
+        hipLaunchKernelGGL((interleave_tri_kernel<float>), dim3(grid_for(n)), dim3(ST_THREADS), 0, s,
+                           re, im, static_cast<float*>(out), n);
+    return hipGetLastError();
+}
+
+}  // namespace ftdx

@@ -113,7 +113,27 @@ _SIGNATURES = {
     "ftdx_mx_layout": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ftdx_gain_loss_grad": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
     "ftdx_herk_adjoint": (_I, [_VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP]),
+    "ftdx_promote_qc_to_fp8": (_I, [_VP, _VP, ctypes.c_int64, _VP]),
+    "ftdx_promote_qc_to_fp16_planar": (_I, [_VP, _VP, _VP, ctypes.c_int64, _VP]),
+    "ftdx_promote_data": (_I, [_VP, _VP, ctypes.c_int64, _I, _VP]),
+    "ftdx_pack_triangle": (_I, [_VP, _VP, _VP, _I, _I, _F, _VP]),
+    "ftdx_tri_from_herm": (_I, [_VP, _VP, _I, _I, _I, _VP]),
+    "ftdx_interleave_tri": (_I, [_VP, _VP, _VP, ctypes.c_int64, _I, _VP]),
+    "ftdx_promote_qc_to_fp32": (_I, [_VP, _VP, ctypes.c_int64, _VP]),
+    "ftdx_convert_qc_to_ftd": (_I, [_VP, _VP, ctypes.c_int64, _VP]),
+    "ftdx_correlate_ftd": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP]),
+    "ftdx_xengine_param_init": (_I, [_VP]),
+    "ftdx_xengine_create": (_I, [_VP, ctypes.POINTER(_VP)]),
+    "ftdx_xengine_compute": (_I, [_VP, _VP, _VP, _VP]),
+    "ftdx_xengine_query": (_I, [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                                ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ftdx_xengine_set_launch_config": (_I, [_VP, _I, _I]),
+    "ftdx_xengine_destroy": (_I, [_VP]),
+    "ftdx_xengine_chpc": (_I, [_VP, _VP, _VP]),
 }
+
+PREC_FP32 = 1
+PREC_FP64 = 2
 
 TILE_AUTO = -1
 TILE_128x128_K64 = 0
@@ -347,6 +367,81 @@ def herk_adjoint(ctx: "CutlassComplexGemm", Gamma, X, N: int, K: int, batch: int
     return dX
 
 
+# ---- stand-alone stage kernels (return new device tensors) ----
+def promote_qc_to_fp8(qc, stream=None):
+    """QC bytes -> [n, 2] uint8 FP8 E4M3 bytes (Re, Im)."""
+    import torch
+    out = torch.empty((qc.numel(), 2), dtype=torch.uint8, device=qc.device)
+    _check(lib().ftdx_promote_qc_to_fp8(_ptr(qc), _ptr(out), qc.numel(), _stream_ptr(stream)),
+           "promote_qc_to_fp8")
+    return out
+
+
+def promote_qc_to_fp16_planar(qc, stream=None):
+    import torch
+    re = torch.empty(qc.numel(), dtype=torch.float16, device=qc.device)
+    im = torch.empty_like(re)
+    _check(lib().ftdx_promote_qc_to_fp16_planar(_ptr(qc), _ptr(re), _ptr(im), qc.numel(),
+                                                _stream_ptr(stream)), "promote_qc_to_fp16_planar")
+    return re, im
+
+
+def promote_data(data, fp64=False, stream=None):
+    """GGP two's-complement (low nibble Re) bytes -> [n, 2] float32/float64."""
+    import torch
+    out = torch.empty((data.numel(), 2), dtype=torch.float64 if fp64 else torch.float32,
+                      device=data.device)
+    _check(lib().ftdx_promote_data(_ptr(data), _ptr(out), data.numel(),
+                                   PREC_FP64 if fp64 else PREC_FP32, _stream_ptr(stream)),
+           "promote_data")
+    return out
+
+
+def pack_triangle(full, N: int, batch: int, old=None, beta: float = 0.0, stream=None):
+    """[batch, N, N, 2] float32 -> packed lower [batch, N(N+1)/2, 2] (+ beta old)."""
+    import torch
+    out = torch.empty((batch, packed_len(N), 2), dtype=torch.float32, device=full.device)
+    _check(lib().ftdx_pack_triangle(_ptr(full), _ptr(out), _ptr(old) if old is not None else None,
+                                    N, batch, beta, _stream_ptr(stream)), "pack_triangle")
+    return out
+
+
+def tri_from_herm(full, N: int, batch: int, stream=None):
+    """[batch, N, N, 2] float32/float64 -> packed upper row-major [batch, N(N+1)/2, 2]."""
+    import torch
+    out = torch.empty((batch, packed_len(N), 2), dtype=full.dtype, device=full.device)
+    prec = PREC_FP64 if full.dtype == torch.float64 else PREC_FP32
+    _check(lib().ftdx_tri_from_herm(_ptr(full), _ptr(out), N, batch, prec, _stream_ptr(stream)),
+           "tri_from_herm")
+    return out
+
+
+def interleave_tri(re, im, fp64=False, stream=None):
+    import torch
+    out = torch.empty((re.numel(), 2), dtype=torch.float64 if fp64 else torch.float32,
+                      device=re.device)
+    _check(lib().ftdx_interleave_tri(_ptr(re), _ptr(im), _ptr(out), re.numel(),
+                                     PREC_FP64 if fp64 else PREC_FP32, _stream_ptr(stream)),
+           "interleave_tri")
+    return out
+
+
+def promote_qc_to_fp32(qc, stream=None):
+    import torch
+    out = torch.empty((qc.numel(), 2), dtype=torch.float32, device=qc.device)
+    _check(lib().ftdx_promote_qc_to_fp32(_ptr(qc), _ptr(out), qc.numel(), _stream_ptr(stream)),
+           "promote_qc_to_fp32")
+    return out
+
+
+def convert_qc_to_ftd(qc, out=None, stream=None):
+    import torch
+    out = torch.empty_like(qc) if out is None else out
+    _check(lib().ftdx_convert_qc_to_ftd(_ptr(qc), _ptr(out), qc.numel(), _stream_ptr(stream)),
+           "convert_qc_to_ftd")
+    return out
+
+
 def problem_array(problems):
     """ctypes array of ftdx_herk_problem from (A, C, N, K[, alpha, beta]) tuples."""
     arr = (HerkProblem * max(1, len(problems)))()
@@ -465,3 +560,116 @@ def herkBatchedCutlassQC_FP16(qc_data, tri_output, N: int, K: int, batch: int,
     return _ggp().herk(qc_data, tri_output, N, K, batch, InputPrecision.INT4,
                        ComputePrecision.FP8, OutputPrecision.FP16, OutputFormat.PackedTriangle,
                        1.0, 0.0, stream, _ggp_tune)
+
+
+# ---- FTD raw-order correlation (ftdx_correlate_ftd) ----
+def correlate_ftd(ctx: "CutlassComplexGemm", raw, n_ant: int, n_chan: int, n_time: int,
+                  n_time_inner: int, n_pol: int, twos: bool = True, out=None, stream=None):
+    """FTD [time][ant][chan][pol] bytes (device uint8) -> packed lower triangle FP32
+    [n_chan*n_pol*n_time_inner, n_ant(n_ant+1)/2, 2] (correlateTCC's contract,
+    lib/tcc_interface.cu:251-314)."""
+    import torch
+    batch = n_chan * n_pol * n_time_inner
+    if out is None:
+        out = torch.empty((batch, packed_len(n_ant), 2), dtype=torch.float32, device=raw.device)
+    _check(lib().ftdx_correlate_ftd(ctx.handle, _ptr(raw), _ptr(out), n_ant, n_chan, n_time,
+                                    n_time_inner, n_pol, int(bool(twos)), _stream_ptr(stream)),
+           "correlate_ftd")
+    return out
+
+
+# ---- GGP XEngine (lib/xengine.cpp:22-306) ----
+LOCATION_HOST = 1
+LOCATION_DEVICE = 2
+PACKET_DSA110 = 4608
+PACKET_CASM = 46080
+PACKET_DSA2K = 8192
+QPREC_QUARTER, QPREC_HALF, QPREC_SINGLE, QPREC_DOUBLE = 1, 2, 4, 8
+BLAS_ENGINE_CUBLAS, BLAS_ENGINE_CUTLASS, BLAS_ENGINE_TCC = 0, 1, 2
+XENGINE_MAT_HERM, XENGINE_MAT_TRI = 0, 1
+
+
+class XEngineParam(ctypes.Structure):
+    """ftdx_xengine_param (mirrors XEngineParam, include/ggp.h:101-123)."""
+    _fields_ = [("struct_size", ctypes.c_size_t), ("in_location", ctypes.c_int),
+                ("out_location", ctypes.c_int), ("packet_format", ctypes.c_int),
+                ("n_payload", ctypes.c_size_t), ("n_antennae_per_payload", ctypes.c_size_t),
+                ("n_channels_per_payload", ctypes.c_size_t), ("n_time_per_payload", ctypes.c_size_t),
+                ("n_time_inner", ctypes.c_size_t), ("n_polarizations", ctypes.c_size_t),
+                ("compute_prec", ctypes.c_int), ("output_prec", ctypes.c_int),
+                ("engine", ctypes.c_int), ("format", ctypes.c_int)]
+
+
+def new_xengine_param(**fields) -> XEngineParam:
+    """newXEngineParam defaults (lib/check_params.h:104-145), then `fields`; sets
+    struct_size."""
+    p = XEngineParam()
+    _check(lib().ftdx_xengine_param_init(ctypes.byref(p)), "xengine_param_init")
+    for k, v in fields.items():
+        setattr(p, k, v)
+    p.struct_size = ctypes.sizeof(XEngineParam)
+    return p
+
+
+def _host_or_device_ptr(x, location: int, what: str):
+    if location == LOCATION_HOST:
+        import numpy as np
+        if not isinstance(x, np.ndarray) or not x.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"XEngine: host {what} must be a C-contiguous numpy array")
+        return x.ctypes.data
+    return _ptr(x)
+
+
+class XEngine:
+    """ggp::XEngine over ftdx_xengine_* (lib/xengine.cpp:22-224).  Host locations
+    take numpy arrays, device locations device tensors."""
+
+    def __init__(self, param: XEngineParam):
+        self.param = param
+        h = _VP()
+        _check(lib().ftdx_xengine_create(ctypes.byref(param), ctypes.byref(h)), "xengine_create")
+        self.handle = h.value
+        ip, op = ctypes.c_size_t(), ctypes.c_size_t()
+        n, k, b = _I(), _I(), _I()
+        _check(lib().ftdx_xengine_query(self.handle, ctypes.byref(ip), ctypes.byref(op),
+                                        ctypes.byref(n), ctypes.byref(k), ctypes.byref(b)),
+               "xengine_query")
+        self.in_payload_bytes, self.out_payload_bytes = ip.value, op.value
+        self.N, self.K, self.batch = n.value, k.value, b.value
+
+    def set_launch_config(self, tile_config: int = -1, xcd_remap: int = -1) -> None:
+        _check(lib().ftdx_xengine_set_launch_config(self.handle, tile_config, xcd_remap),
+               "xengine_set_launch_config")
+
+    def compute(self, output, input_data, stream=None) -> None:
+        p = self.param
+        nin = self.in_payload_bytes * p.n_payload
+        nout = self.out_payload_bytes * p.n_payload
+        for x, n, what in ((input_data, nin, "input"), (output, nout, "output")):
+            nbytes = x.nbytes if hasattr(x, "nbytes") else x.numel() * x.element_size()
+            if nbytes < n:
+                raise ValueError(f"XEngine: {what} holds {nbytes} bytes, needs {n}")
+        sp = None
+        if p.in_location == LOCATION_DEVICE or p.out_location == LOCATION_DEVICE:
+            sp = _stream_ptr(stream)
+        _check(lib().ftdx_xengine_compute(
+            self.handle, _host_or_device_ptr(output, p.out_location, "output"),
+            _host_or_device_ptr(input_data, p.in_location, "input"), sp), "xengine_compute")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().ftdx_xengine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def xengine_chpc(output, input_data, param: XEngineParam) -> None:
+    """XEngineCHPC (lib/interface_ggp.cpp:536-555): one-shot create/compute/destroy."""
+    _check(lib().ftdx_xengine_chpc(_host_or_device_ptr(output, param.out_location, "output"),
+                                   _host_or_device_ptr(input_data, param.in_location, "input"),
+                                   ctypes.byref(param)), "xengine_chpc")

@@ -202,6 +202,126 @@ int ftdx_gain_loss_grad(ftdx_context* ctx, const void* V, const void* M, const v
 int ftdx_herk_adjoint(ftdx_context* ctx, const void* Gamma, const void* X, void* dX, int N, int K,
                       int batch_count, int input, void* stream);
 
+/* ---- stand-alone stage kernels (the fused HERK does these internally) ---- */
+#define FTDX_PREC_FP32 1 /* element type selectors for the GGP-convention stages */
+#define FTDX_PREC_FP64 2
+
+/* QC bytes -> [fp8 Re, fp8 Im] byte pairs, n complex samples (replaces
+ * cast_int4_to_fp8_interleaved, cutlass_interface/cutlass_gemm_api.cu:1553-1623;
+ * same bytes as its LUT, -0 nibble -> 0x80). */
+int ftdx_promote_qc_to_fp8(const void* qc, void* fp8_pairs, int64_t n, void* stream);
+/* QC bytes -> FP16 Re plane and Im plane (replaces cast_int4_to_fp16_planar,
+ * cutlass_gemm_api.cu:1627-1698; -0 nibble -> -0.0). */
+int ftdx_promote_qc_to_fp16_planar(const void* qc, void* re, void* im, int64_t n, void* stream);
+/* GGP convention: (4,4)-bit two's complement, low nibble = Re, high = Im ->
+ * interleaved FP32 or FP64 pairs (replaces promoteData, lib/promote_data.cu:54-97). */
+int ftdx_promote_data(const void* in, void* out, int64_t n, int prec, void* stream);
+/* Full N x N interleaved FP32 matrices [batch][N][N][2] -> packed lower
+ * triangle, out = full (+ beta * old when beta != 0 and old != NULL)
+ * (replaces pack_scratch_to_triangle_fp32, shared/pack_kernels_common.hpp:602-659). */
+int ftdx_pack_triangle(const void* full, void* packed, const void* old, int N, int batch_count,
+                       float beta, void* stream);
+/* GGP convention: full N x N complex [batch][N][N][2] -> packed UPPER triangle,
+ * row-major, (i, j >= i) at i*N - i(i-1)/2 + (j - i) (replaces
+ * triangulateFromHerm, lib/tri_from_herm.cu:10-54). */
+int ftdx_tri_from_herm(const void* full, void* tri, int N, int batch_count, int prec,
+                       void* stream);
+/* FP16 Re/Im planes -> interleaved FP32/FP64 pairs (replaces promInterTri,
+ * lib/interleave_tri.cu:9-54). */
+int ftdx_interleave_tri(const void* re, const void* im, void* out, int64_t n, int prec,
+                        void* stream);
+
+/* QC bytes -> interleaved FP32 [re, im] (replaces promoteQcSmToFp32,
+ * include/kernels/visibility_bf.cuh:38-56; -0 nibble -> +0). */
+int ftdx_promote_qc_to_fp32(const void* qc, void* out, int64_t n, void* stream);
+/* QC sign-magnitude (high nibble Re) -> FTD two's-complement nibbles (low nibble
+ * Re, high nibble Im), n bytes; qc == out allowed (replaces convertQcSmToFtd,
+ * lib/visibility_bf_kernels.cu:422-440). */
+int ftdx_convert_qc_to_ftd(const void* qc, void* out, int64_t n, void* stream);
+
+/* ---- FTD raw-order correlation (the TCC engine's contract) ----
+ * Input: one byte per complex sample in FTD order [time][ant][chan][pol], time =
+ * k * n_time_inner + ti, K = n_time / n_time_inner samples per product.  Output:
+ * packed lower triangle FP32 [batch][n_ant(n_ant+1)/2][2], batch = (chan * n_pol +
+ * pol) * n_time_inner + ti, element (r2, r1 <= r2) = sum_k x_r2 conj(x_r1), i.e. the
+ * HERK convention above.  `twos` = 1: FTD two's-complement nibbles (low nibble Re,
+ * values in [-8, 7]) as correlateTCC takes them (lib/tcc_interface.cu:251-314);
+ * `twos` = 0: QC sign-magnitude bytes as XEngine's TCC branch receives them before
+ * convertQcSmToFtd (lib/xengine.cpp:130-150; CPU form cpu_herk_ftd,
+ * tests/visibility_pipeline_test.cpp:54-96).  The input is not modified.  Any
+ * n_pol >= 1 (libtcc needs 2); no time-block multiple is required. */
+int ftdx_correlate_ftd(ftdx_context* ctx, const void* raw, void* tri_fp32, int n_ant, int n_chan,
+                       int n_time, int n_time_inner, int n_pol, int twos, void* stream);
+
+/* ---- GGP XEngine (lib/xengine.cpp:22-224, include/ggp.h:101-123) ----
+ * Enum values equal the reference's (include/enum_ggp.h). */
+#define FTDX_LOCATION_HOST 1     /* QUDA_CPU_FIELD_LOCATION */
+#define FTDX_LOCATION_DEVICE 2   /* QUDA_CUDA_FIELD_LOCATION */
+#define FTDX_PACKET_DSA110 4608  /* QUDA_PACKET_FORMAT_DSA110 */
+#define FTDX_PACKET_CASM 46080   /* QUDA_PACKET_FORMAT_CASM */
+#define FTDX_PACKET_DSA2K 8192   /* QUDA_PACKET_FORMAT_DSA2K */
+#define FTDX_QPREC_QUARTER 1     /* QudaPrecision */
+#define FTDX_QPREC_HALF 2
+#define FTDX_QPREC_SINGLE 4
+#define FTDX_QPREC_DOUBLE 8
+#define FTDX_BLAS_ENGINE_CUBLAS 0  /* QudaBLASEngine */
+#define FTDX_BLAS_ENGINE_CUTLASS 1
+#define FTDX_BLAS_ENGINE_TCC 2
+#define FTDX_XENGINE_MAT_HERM 0  /* QudaXEngineMatFormat */
+#define FTDX_XENGINE_MAT_TRI 1
+#define FTDX_INVALID_ENUM (-0x7fffffff - 1) /* QUDA_INVALID_ENUM */
+
+/* Mirrors XEngineParam (include/ggp.h:101-123) minus data_type / data_order /
+ * verbosity, which the reference's XEngine does not read. */
+typedef struct ftdx_xengine_param {
+    size_t struct_size;    /* sizeof(ftdx_xengine_param) or (size_t)FTDX_INVALID_ENUM */
+    int in_location;       /* FTDX_LOCATION_* of the input payloads */
+    int out_location;      /* FTDX_LOCATION_* of the outputs */
+    int packet_format;     /* FTDX_PACKET_DSA2K */
+    size_t n_payload;
+    size_t n_antennae_per_payload;
+    size_t n_channels_per_payload;
+    size_t n_time_per_payload;
+    size_t n_time_inner;
+    size_t n_polarizations;
+    int compute_prec;      /* FTDX_QPREC_SINGLE or _DOUBLE */
+    int output_prec;       /* must equal compute_prec */
+    int engine;            /* FTDX_BLAS_ENGINE_*: CUBLAS / CUTLASS -> GGP layout; TCC -> FTD order */
+    int format;            /* FTDX_XENGINE_MAT_HERM or _TRI */
+} ftdx_xengine_param;
+
+typedef struct ftdx_xengine ftdx_xengine;
+
+/* newXEngineParam (lib/check_params.h:104-145): DSA2K, compute HALF (which
+ * ftdx_xengine_create rejects, as the reference constructor does), output
+ * SINGLE, engine CUBLAS, format HERM, sizes invalid; locations host. */
+int ftdx_xengine_param_init(ftdx_xengine_param* param);
+
+/* ggp::XEngine::XEngine (lib/xengine.cpp:22-108): validates, sizes and allocates.
+ * Layouts (engine CUBLAS or CUTLASS, packet DSA2K):
+ *   input per payload: n_time * n_ant * n_chan * n_pol bytes, GGP two's-complement
+ *     nibbles (low = Re), batches [chan][pol][time_inner] of [K][n_ant] (ant fastest),
+ *     K = n_time / n_time_inner;
+ *   output per payload: batch = n_chan * n_pol * n_time_inner matrices of M^H M
+ *     (M = K x n_ant), TRI: upper triangle row-major (row q, col p >= q at
+ *     q*N + p - q(q+1)/2), HERM: full N x N row-major; FP32 or FP64 [re, im].
+ * Engine TCC (packet DSA2K, format TRI, output SINGLE): the ftdx_correlate_ftd
+ * layout with QC sign-magnitude input.  Packet DSA110 is FTDX_ERR_UNSUPPORTED. */
+int ftdx_xengine_create(const ftdx_xengine_param* param, ftdx_xengine** out);
+/* XEngine::compute (lib/xengine.cpp:110-178): for each payload, copy in (host
+ * input), convert, HERK, copy out (host output); returns after the stream is
+ * idle, as the reference ends with a device synchronize. */
+int ftdx_xengine_compute(ftdx_xengine* xe, void* output, const void* input, void* stream);
+/* Bytes of one input / output payload and the HERK shape of an engine. */
+int ftdx_xengine_query(const ftdx_xengine* xe, size_t* in_payload_bytes, size_t* out_payload_bytes,
+                       int* N, int* K, int* batch_count);
+/* Launch configuration for the HERK (FTDX_TILE_*, xcd_remap 0/1; -1 = library choice). */
+int ftdx_xengine_set_launch_config(ftdx_xengine* xe, int tile_config, int xcd_remap);
+int ftdx_xengine_destroy(ftdx_xengine* xe);
+/* XEngineCHPC (lib/interface_ggp.cpp:536-555): create, compute on the default
+ * stream, destroy. */
+int ftdx_xengine_chpc(void* output, const void* input, const ftdx_xengine_param* param);
+
 /* Replaces CutlassComplexGemm::print_build_info (cutlass_gemm_api.h:773-775). */
 int ftdx_print_build_info(void);
 

@@ -11,6 +11,14 @@
  *                         of complex<double>: exact, same values)
  *   cpu_herk_ab           cutlass_interface/test_gemm_api.cu:726-751
  *   FP32/FP16 epilogue    cutlass_interface/shared/herk_kernel_common.hpp:405-420
+ *   MX scale / codes      gemm_complex_sm100/mxfp_kernels.hpp:122-176 (scale rule) and
+ *                         the OCP MX v1.0 element formats (E3M2, E2M1)
+ *   gain loss / adjoint   FP64 closed forms of the calibration gradients
+ *   promote_to_float      tests/xengine_test.cpp:162-167 (GGP two's complement)
+ *   cpu_convert_qc_sm_to_ftd  tests/visibility_pipeline_test.cpp:101-112
+ *   triangulateFromHerm   include/kernels/tri_from_herm.cuh:28-73
+ *   verifyXEngine GEMM    tests/xengine_test.cpp:213-321 (M^H M, TRI / HERM)
+ *   cpu_herk_ftd          tests/visibility_pipeline_test.cpp:54-96 (int64)
  * Parallelism follows the reference's OpenMP "parallel for schedule(dynamic)"
  * over (batch, row) (test_herk_int4.cu:56, :323).
  *
@@ -484,4 +492,172 @@ void ftdo_herk_adjoint_f64(const double* Gamma, const double* X, double* dX, int
         }
     }
     (void)threads;
+}
+
+/* ------------------------------------------------------------------ */
+/* stage converters */
+static int twos_nibble(unsigned n) { return (int)((n & 0xF) ^ 0x8) - 8; }
+
+void ftdo_promote_data(const uint8_t* in, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        out[2 * i] = (double)twos_nibble(in[i] & 0xF);
+        out[2 * i + 1] = (double)twos_nibble(in[i] >> 4);
+    }
+}
+
+uint8_t ftdo_convert_qc_to_ftd(uint8_t b) {
+    int re, im;
+    ftdo_decode_qc(b, &re, &im);
+    return (uint8_t)((((unsigned)im & 0xF) << 4) | ((unsigned)re & 0xF));
+}
+
+uint16_t ftdo_qc_nibble_fp16(unsigned nibble) {
+    const unsigned m = nibble & 7u;
+    const uint16_t sign = (nibble & 8u) ? 0x8000u : 0u;
+    return (uint16_t)(sign | ftdo_f32_to_f16_rne((float)m));
+}
+
+uint8_t ftdo_f32_to_e4m3_clamped(float x) {
+    if (x > 448.0f) x = 448.0f;
+    if (x < -448.0f) x = -448.0f;
+    const unsigned sign = signbit(x) ? 0x80u : 0u;
+    const float a = fabsf(x);
+    unsigned code;
+    int e;
+    frexpf(a, &e);
+    e -= 1;
+    if (a == 0.0f || e < -6) {
+        code = (unsigned)rintf(ldexpf(a, 9)); /* subnormal step 2^-9 */
+    } else {
+        unsigned q = (unsigned)rintf(ldexpf(a, 3 - e));
+        if (q == 16u) {
+            q = 8u;
+            ++e;
+        }
+        code = ((unsigned)(e + 7) << 3) | (q - 8u);
+    }
+    if (code > 0x7Eu) code = 0x7Eu;
+    return (uint8_t)(sign | code);
+}
+
+void ftdo_tri_from_herm(const double* full, double* tri, int N, int batch) {
+    const int64_t P = (int64_t)N * (N + 1) / 2;
+    for (int b = 0; b < batch; ++b)
+        for (int i = 0; i < N; ++i)
+            for (int j = i; j < N; ++j) {
+                const int64_t src = (((int64_t)b * N + i) * N + j) * 2;
+                const int64_t dst = ((int64_t)b * P + (int64_t)i * N + j - (int64_t)i * (i + 1) / 2) * 2;
+                tri[dst] = full[src];
+                tri[dst + 1] = full[src + 1];
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* GGP XEngine: per batch b, M_b = K x N row-major (antenna fastest) of
+ * promote_to_float values (tests/xengine_test.cpp:162-167: two's-complement
+ * nibbles, low = Re), C_b = M_b^H M_b (xengine_test.cpp:254-279).  TRI: upper
+ * triangle row-major, (q, p >= q) at q*N + p - q(q+1)/2 (xengine_test.cpp:282-287,
+ * tri_from_herm.cuh:28-73); HERM: the full N x N matrix row-major.  int64 exact. */
+void ftdo_xengine_ggp_i64(const uint8_t* in, int N, int K, int batch, int herm, int64_t* out,
+                          int threads) {
+    if (N <= 0 || batch <= 0) return;
+    const int64_t nk = (int64_t)N * (K > 0 ? K : 0);
+    int16_t* re = (int16_t*)malloc(sizeof(int16_t) * (size_t)(nk * batch > 0 ? nk * batch : 1));
+    int16_t* im = (int16_t*)malloc(sizeof(int16_t) * (size_t)(nk * batch > 0 ? nk * batch : 1));
+    /* transpose to [b][antenna][time] planes */
+    for (int b = 0; b < batch; ++b)
+        for (int t = 0; t < K; ++t)
+            for (int a = 0; a < N; ++a) {
+                const uint8_t v = in[(int64_t)b * nk + (int64_t)t * N + a];
+                re[(int64_t)b * nk + (int64_t)a * K + t] = (int16_t)twos_nibble(v & 0xF);
+                im[(int64_t)b * nk + (int64_t)a * K + t] = (int16_t)twos_nibble(v >> 4);
+            }
+    const int64_t per = herm ? (int64_t)N * N : (int64_t)N * (N + 1) / 2;
+    const int64_t total = (int64_t)batch * N;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+#endif
+    for (int64_t t = 0; t < total; ++t) {
+        const int b = (int)(t / N), q = (int)(t % N);
+        const int16_t* qr = re + b * nk + (int64_t)q * K;
+        const int16_t* qi = im + b * nk + (int64_t)q * K;
+        int64_t* o = out + b * per * 2;
+        for (int p = q; p < N; ++p) {
+            const int16_t* pr = re + b * nk + (int64_t)p * K;
+            const int16_t* pi = im + b * nk + (int64_t)p * K;
+            int64_t sr = 0, si = 0;
+            for (int k0 = 0; k0 < K; k0 += (1 << 20)) {
+                const int k1 = (K - k0 > (1 << 20)) ? k0 + (1 << 20) : K;
+                int32_t ar = 0, ai = 0;
+                for (int k = k0; k < k1; ++k) {
+                    /* conj(x_q) x_p = (qr pr + qi pi) + i (qr pi - qi pr) */
+                    ar += (int32_t)qr[k] * pr[k] + (int32_t)qi[k] * pi[k];
+                    ai += (int32_t)qr[k] * pi[k] - (int32_t)qi[k] * pr[k];
+                }
+                sr += ar;
+                si += ai;
+            }
+            if (herm) {
+                o[((int64_t)q * N + p) * 2] = sr;
+                o[((int64_t)q * N + p) * 2 + 1] = si;
+                o[((int64_t)p * N + q) * 2] = sr;
+                o[((int64_t)p * N + q) * 2 + 1] = -si;
+            } else {
+                const int64_t idx = (int64_t)q * N + p - (int64_t)q * (q + 1) / 2;
+                o[idx * 2] = sr;
+                o[idx * 2 + 1] = si;
+            }
+        }
+    }
+    (void)threads;
+    free(re);
+    free(im);
+}
+
+/* FTD raw-order HERK (cpu_herk_ftd, tests/visibility_pipeline_test.cpp:54-96), int64:
+ * batch b = (chan * n_pol + pol) * n_ti + ti; x_a[k] is the byte at
+ * ((k * n_ti + ti) * n_ant + a) * n_chan * n_pol + chan * n_pol + pol; output the
+ * packed lower triangle of sum_k x_i conj(x_j) per batch.  twos != 0: two's-complement
+ * nibbles, low = Re (FTD, lib/tcc_interface.cu:124-133); else QC sign-magnitude. */
+void ftdo_herk_ftd_i64(const uint8_t* raw, int n_ant, int n_chan, int n_time, int n_ti, int n_pol,
+                       int twos, int64_t* out, int threads) {
+    if (n_ant <= 0 || n_chan <= 0 || n_pol <= 0 || n_ti <= 0) return;
+    const int K = n_time / n_ti;
+    const int batch = n_chan * n_pol * n_ti;
+    const int64_t nk = (int64_t)n_ant * K;
+    int16_t* re = (int16_t*)malloc(sizeof(int16_t) * (size_t)(nk * batch > 0 ? nk * batch : 1));
+    int16_t* im = (int16_t*)malloc(sizeof(int16_t) * (size_t)(nk * batch > 0 ? nk * batch : 1));
+    const int64_t C = (int64_t)n_chan * n_pol;
+    for (int b = 0; b < batch; ++b) {
+        const int ti = b % n_ti, pol = (b / n_ti) % n_pol, chan = b / (n_pol * n_ti);
+        for (int a = 0; a < n_ant; ++a)
+            for (int k = 0; k < K; ++k) {
+                const int64_t t = (int64_t)k * n_ti + ti;
+                const uint8_t v = raw[(t * n_ant + a) * C + (int64_t)chan * n_pol + pol];
+                int r, m;
+                if (twos) {
+                    r = twos_nibble(v & 0xF);
+                    m = twos_nibble(v >> 4);
+                } else {
+                    ftdo_decode_qc(v, &r, &m);
+                }
+                re[(int64_t)b * nk + (int64_t)a * K + k] = (int16_t)r;
+                im[(int64_t)b * nk + (int64_t)a * K + k] = (int16_t)m;
+            }
+    }
+    const int64_t packed2 = (int64_t)n_ant * (n_ant + 1);
+    const int64_t total = (int64_t)batch * n_ant;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+#endif
+    for (int64_t t = 0; t < total; ++t) {
+        const int b = (int)(t / n_ant);
+        const int i = n_ant - 1 - (int)(t % n_ant);
+        herk_row(re + b * nk, im + b * nk, K, i, out + b * packed2 + (int64_t)i * (i + 1));
+    }
+    (void)threads;
+    free(re);
+    free(im);
 }

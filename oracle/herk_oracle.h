@@ -89,6 +89,29 @@ void ftdo_gain_loss_grad_f64(const double* V, const double* M, const double* g, 
 void ftdo_herk_adjoint_f64(const double* Gamma, const double* X, double* dX, int N, int K,
                            int batch, int threads);
 
+/* ---- stage converters ---- */
+/* GGP (4,4)-bit two's complement, low nibble = Re -> doubles [re, im]
+ * (include/kernels/promote_data.cuh:80-81). */
+void ftdo_promote_data(const uint8_t* in, int64_t n, double* out);
+/* QC sign-magnitude -> FTD two's-complement byte (low = Re, high = Im)
+ * (tests/visibility_pipeline_test.cpp:105-117). */
+uint8_t ftdo_convert_qc_to_ftd(uint8_t b);
+/* QC nibble -> FP16 bits (cutlass_gemm_api.cu:1548-1551 table: -0 -> 0x8000). */
+uint16_t ftdo_qc_nibble_fp16(unsigned nibble);
+/* float -> OCP E4M3 after clamping to +-448, nearest-even
+ * (cast_kernels_common.hpp:30-48: clamp, then __nv_fp8_e4m3). */
+uint8_t ftdo_f32_to_e4m3_clamped(float x);
+/* full [batch][N][N][2] -> packed upper row-major (include/kernels/tri_from_herm.cuh:28-73). */
+void ftdo_tri_from_herm(const double* full, double* tri, int N, int batch);
+
+/* GGP XEngine M^H M per batch ([batch][K][N] two's-complement input); herm = 0:
+ * upper triangle row-major [batch][N(N+1)/2][2], herm = 1: full [batch][N][N][2]. */
+void ftdo_xengine_ggp_i64(const uint8_t* in, int N, int K, int batch, int herm, int64_t* out,
+                          int threads);
+/* FTD raw-order HERK, packed lower triangle [n_chan*n_pol*n_ti][n_ant(n_ant+1)/2][2]. */
+void ftdo_herk_ftd_i64(const uint8_t* raw, int n_ant, int n_chan, int n_time, int n_ti, int n_pol,
+                       int twos, int64_t* out, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,23 @@ def lib():
         L.ftdo_herk_adjoint_f64.restype = None
         L.ftdo_herk_adjoint_f64.argtypes = [dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int]
+        L.ftdo_promote_data.restype = None
+        L.ftdo_promote_data.argtypes = [u8p, ctypes.c_int64, dp]
+        L.ftdo_convert_qc_to_ftd.restype = ctypes.c_uint8
+        L.ftdo_convert_qc_to_ftd.argtypes = [ctypes.c_uint8]
+        L.ftdo_qc_nibble_fp16.restype = ctypes.c_uint16
+        L.ftdo_qc_nibble_fp16.argtypes = [ctypes.c_uint]
+        L.ftdo_f32_to_e4m3_clamped.restype = ctypes.c_uint8
+        L.ftdo_f32_to_e4m3_clamped.argtypes = [ctypes.c_float]
+        L.ftdo_tri_from_herm.restype = None
+        L.ftdo_tri_from_herm.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.ftdo_xengine_ggp_i64.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, i64p, ctypes.c_int]
+        L.ftdo_xengine_ggp_i64.restype = None
+        L.ftdo_herk_ftd_i64.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, i64p, ctypes.c_int]
+        L.ftdo_herk_ftd_i64.restype = None
         L.ftdo_herk_c64.restype = None
         L.ftdo_herk_c64.argtypes = [ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
@@ -265,3 +282,57 @@ def decode_qc_array(A: np.ndarray) -> np.ndarray:
     nib = np.array([0, 1, 2, 3, 4, 5, 6, 7, 0, -1, -2, -3, -4, -5, -6, -7], dtype=np.float64)
     A = np.asarray(A, dtype=np.uint8)
     return nib[A >> 4] + 1j * nib[A & 15]
+
+
+# ---- stage converters ----
+def promote_data(data: np.ndarray) -> np.ndarray:
+    d = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+    out = np.zeros((d.size, 2))
+    lib().ftdo_promote_data(_ptr(d, ctypes.c_uint8), d.size, _ptr(out, ctypes.c_double))
+    return out
+
+
+def convert_qc_to_ftd(data: np.ndarray) -> np.ndarray:
+    table = np.array([lib().ftdo_convert_qc_to_ftd(b) for b in range(256)], dtype=np.uint8)
+    return table[np.asarray(data, dtype=np.uint8)]
+
+
+def qc_nibble_fp16(n: int) -> int:
+    return int(lib().ftdo_qc_nibble_fp16(n))
+
+
+def f32_to_e4m3_clamped(x: float) -> int:
+    return int(lib().ftdo_f32_to_e4m3_clamped(x))
+
+
+def tri_from_herm(full: np.ndarray, N: int, batch: int) -> np.ndarray:
+    f = np.ascontiguousarray(full, dtype=np.float64).reshape(batch, N, N, 2)
+    out = np.zeros((batch, N * (N + 1) // 2, 2))
+    lib().ftdo_tri_from_herm(_ptr(f, ctypes.c_double), _ptr(out, ctypes.c_double), N, batch)
+    return out
+
+
+# ---- GGP XEngine and FTD raw-order correlation ----
+def xengine_ggp_i64(data: np.ndarray, N: int, K: int, batch: int, herm: bool,
+                    threads: int = 0) -> np.ndarray:
+    """M^H M per batch of GGP bytes [batch][K][N]: TRI -> [batch, N(N+1)/2, 2] (upper,
+    row-major), HERM -> [batch, N, N, 2]; int64."""
+    d = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+    assert d.size >= N * K * batch
+    shape = (batch, N, N, 2) if herm else (batch, N * (N + 1) // 2, 2)
+    out = np.zeros(shape, dtype=np.int64)
+    lib().ftdo_xengine_ggp_i64(_ptr(d, ctypes.c_uint8), N, K, batch, int(herm),
+                               _ptr(out, ctypes.c_int64), threads)
+    return out
+
+
+def herk_ftd_i64(raw: np.ndarray, n_ant: int, n_chan: int, n_time: int, n_ti: int, n_pol: int,
+                 twos: bool, threads: int = 0) -> np.ndarray:
+    """FTD [time][ant][chan][pol] bytes -> packed lower triangle
+    [n_chan*n_pol*n_ti, n_ant(n_ant+1)/2, 2] int64."""
+    d = np.ascontiguousarray(raw, dtype=np.uint8).reshape(-1)
+    assert d.size >= n_time * n_ant * n_chan * n_pol
+    out = np.zeros((n_chan * n_pol * n_ti, n_ant * (n_ant + 1) // 2, 2), dtype=np.int64)
+    lib().ftdo_herk_ftd_i64(_ptr(d, ctypes.c_uint8), n_ant, n_chan, n_time, n_ti, n_pol, int(twos),
+                            _ptr(out, ctypes.c_int64), threads)
+    return out

@@ -31,8 +31,9 @@ def _setup(oracle, dev, N, K, B, seed):
     V = gt[:, il] * gt[:, jl].conj() * Mc + 20.0 * (rng.standard_normal(Mc.shape)
                                                     + 1j * rng.standard_normal(Mc.shape))
     V[:, il == jl] = V[:, il == jl].real
-    V32 = np.stack([V.real, V.imag], -1).astype(np.float32)
-    g32 = np.stack([g.real, g.imag], -1).astype(np.float32)
+    # numpy keeps the operands' memory order ('K'): force C order for the device copies
+    V32 = np.ascontiguousarray(np.stack([V.real, V.imag], -1), dtype=np.float32)
+    g32 = np.ascontiguousarray(np.stack([g.real, g.imag], -1), dtype=np.float32)
     # the FP64 references use exactly the FP32 values the GPU sees
     V = V32[..., 0].astype(np.float64) + 1j * V32[..., 1]
     g = g32[..., 0].astype(np.float64) + 1j * g32[..., 1]

@@ -48,7 +48,7 @@ def test_headers_compile():
     inc = os.path.join(ROOT, "include")
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-x", "c",
                     os.path.join(inc, "ftdx.h")], check=True)
-    for h in ("cutlass_gemm_api.h", "ggp_cutlass_interface.h"):
+    for h in ("cutlass_gemm_api.h", "ggp_cutlass_interface.h", "ggp_xengine.h"):
         subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-x", "c++",
                         "-I", inc, "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-"],
                        input=f'#include "{h}"\n', text=True, check=True)
@@ -103,3 +103,49 @@ def test_tune_cache_file_parsing(tmp_path):
     assert ctx2.tune_load(str(out)) == (2, 0)
     with pytest.raises(ftdx.FtdxError):
         ctx2.tune_load(str(tmp_path / "missing.txt"))
+
+
+def test_xengine_param_validation_without_device():
+    """ftdx_xengine_create rejects what the reference XEngine rejects
+    (lib/xengine.cpp:34-88, lib/check_params.h:118-119), before touching a device."""
+    import ctypes
+    from ftd_amd import ftdx
+    L = ftdx.lib()
+    base = dict(n_payload=1, n_antennae_per_payload=8, n_channels_per_payload=1,
+                n_time_per_payload=32, n_time_inner=2, n_polarizations=2,
+                compute_prec=ftdx.QPREC_SINGLE, output_prec=ftdx.QPREC_SINGLE,
+                format=ftdx.XENGINE_MAT_TRI)
+
+    def create(struct_size=None, **kw):
+        p = ftdx.new_xengine_param(**{**base, **kw})
+        if struct_size is not None:
+            p.struct_size = struct_size
+        h = ctypes.c_void_p()
+        rc = L.ftdx_xengine_create(ctypes.byref(p), ctypes.byref(h))
+        assert h.value is None
+        return rc, ftdx.last_error()
+
+    d = ftdx.new_xengine_param()
+    assert (d.compute_prec, d.output_prec, d.packet_format, d.format, d.engine) == (
+        ftdx.QPREC_HALF, ftdx.QPREC_SINGLE, ftdx.PACKET_DSA2K, ftdx.XENGINE_MAT_HERM,
+        ftdx.BLAS_ENGINE_CUBLAS)
+    rc, msg = create(compute_prec=ftdx.QPREC_HALF)
+    assert rc == ftdx.FTDX_ERR_UNSUPPORTED and "compute precision" in msg
+    rc, msg = create(output_prec=ftdx.QPREC_DOUBLE)
+    assert rc == ftdx.FTDX_ERR_UNSUPPORTED and "must match" in msg
+    rc, msg = create(packet_format=ftdx.PACKET_DSA110)
+    assert rc == ftdx.FTDX_ERR_UNSUPPORTED and "DSA110" in msg
+    assert create(packet_format=ftdx.PACKET_CASM)[0] == ftdx.FTDX_ERR_INVALID_ARG
+    assert create(format=7)[0] == ftdx.FTDX_ERR_INVALID_ARG
+    rc, msg = create(engine=ftdx.BLAS_ENGINE_TCC, format=ftdx.XENGINE_MAT_HERM)
+    assert rc == ftdx.FTDX_ERR_INVALID_ARG and "triangular" in msg
+    assert create(engine=ftdx.BLAS_ENGINE_TCC, compute_prec=ftdx.QPREC_DOUBLE,
+                  output_prec=ftdx.QPREC_DOUBLE)[0] == ftdx.FTDX_ERR_UNSUPPORTED
+    assert create(engine=9)[0] == ftdx.FTDX_ERR_INVALID_ARG
+    assert create(n_payload=0)[0] == ftdx.FTDX_ERR_INVALID_ARG
+    assert create(in_location=3)[0] == ftdx.FTDX_ERR_INVALID_ARG
+    rc, msg = create(struct_size=3)
+    assert rc == ftdx.FTDX_ERR_INVALID_ARG and "struct size" in msg
+    assert L.ftdx_xengine_create(None, ctypes.byref(ctypes.c_void_p())) == ftdx.FTDX_ERR_INVALID_ARG
+    assert L.ftdx_xengine_destroy(None) == ftdx.FTDX_OK
+    assert L.ftdx_correlate_ftd(None, None, None, 4, 1, 4, 1, 2, 1, None) == ftdx.FTDX_ERR_INVALID_ARG
