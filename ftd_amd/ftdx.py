@@ -130,6 +130,11 @@ _SIGNATURES = {
     "ftdx_xengine_set_launch_config": (_I, [_VP, _I, _I]),
     "ftdx_xengine_destroy": (_I, [_VP]),
     "ftdx_xengine_chpc": (_I, [_VP, _VP, _VP]),
+    "ftdx_pipeline_create": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(_VP)]),
+    "ftdx_pipeline_query": (_I, [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "ftdx_pipeline_correlate": (_I, [_VP, _VP, _VP]),
+    "ftdx_pipeline_correlate_many": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I]),
+    "ftdx_pipeline_destroy": (_I, [_VP]),
 }
 
 PREC_FP32 = 1
@@ -673,3 +678,60 @@ def xengine_chpc(output, input_data, param: XEngineParam) -> None:
     _check(lib().ftdx_xengine_chpc(_host_or_device_ptr(output, param.out_location, "output"),
                                    _host_or_device_ptr(input_data, param.in_location, "input"),
                                    ctypes.byref(param)), "xengine_chpc")
+
+
+# ---- streaming correlation of host payloads (ftdx_pipeline_*) ----
+class Pipeline:
+    """VisibilityPipeline::correlate / correlate_pipelined
+    (lib/visibility_pipeline.cu:253-300, 484-545) over host numpy payloads."""
+
+    def __init__(self, ctx: "CutlassComplexGemm", N: int, K: int, batch: int,
+                 output: str = "fp32"):
+        self.ctx = ctx  # keeps the context alive
+        self.N, self.K, self.batch = N, K, batch
+        self.output = output
+        h = _VP()
+        out = int(OutputPrecision.FP32 if output == "fp32" else OutputPrecision.FP16)
+        _check(lib().ftdx_pipeline_create(ctx.handle, N, K, batch, out, ctypes.byref(h)),
+               "pipeline_create")
+        self.handle = h.value
+
+    def out_shape(self):
+        return (self.batch, packed_len(self.N), 2)
+
+    def _out_dtype(self):
+        import numpy as np
+        return np.float32 if self.output == "fp32" else np.float16
+
+    def correlate_many(self, raw_inputs, outputs=None):
+        import numpy as np
+        n = len(raw_inputs)
+        ins = [np.ascontiguousarray(x, dtype=np.uint8).reshape(-1) for x in raw_inputs]
+        for x in ins:
+            if x.size != self.N * self.K * self.batch:
+                raise ValueError("Pipeline: payload has the wrong number of bytes")
+        if outputs is None:
+            outputs = [np.empty(self.out_shape(), dtype=self._out_dtype()) for _ in range(n)]
+        for o in outputs:
+            if not o.flags["C_CONTIGUOUS"] or o.nbytes < self.batch * packed_len(self.N) * 2 * \
+                    np.dtype(self._out_dtype()).itemsize:
+                raise ValueError("Pipeline: output too small or not contiguous")
+        arr_in = (_VP * max(1, n))(*[x.ctypes.data for x in ins])
+        arr_out = (_VP * max(1, n))(*[o.ctypes.data for o in outputs])
+        _check(lib().ftdx_pipeline_correlate_many(self.handle, arr_out, arr_in, n),
+               "pipeline_correlate_many")
+        return outputs
+
+    def correlate(self, raw_input, output=None):
+        return self.correlate_many([raw_input], None if output is None else [output])[0]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().ftdx_pipeline_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -322,6 +322,24 @@ int ftdx_xengine_destroy(ftdx_xengine* xe);
  * stream, destroy. */
 int ftdx_xengine_chpc(void* output, const void* input, const ftdx_xengine_param* param);
 
+/* ---- streaming correlation of host payloads ----
+ * VisibilityPipeline::correlate / correlate_pipelined (lib/visibility_pipeline.cu:253-300,
+ * 484-545): each payload is QC bytes [batch][N][K] in host memory (K = n_time /
+ * n_time_inner, batch = n_chan * n_pol * n_time_inner); the output is the packed
+ * lower triangle (FP32 or FP16, output = FTDX_OUTPUT_*) in host memory.  The
+ * pipeline owns two device buffer pairs, two pinned staging pairs and three
+ * default-priority streams; payload p's H2D, HERK and D2H overlap its neighbours'.
+ * The HERK uses ctx's settings (mode, launch configuration, graph replay).  Host
+ * buffers may be pageable.  Returns when every output has been written. */
+typedef struct ftdx_pipeline ftdx_pipeline;
+int ftdx_pipeline_create(ftdx_context* ctx, int N, int K, int batch_count, int output,
+                         ftdx_pipeline** out);
+int ftdx_pipeline_query(const ftdx_pipeline* pl, size_t* in_payload_bytes, size_t* out_payload_bytes);
+int ftdx_pipeline_correlate(ftdx_pipeline* pl, void* vis_output, const void* raw_input);
+int ftdx_pipeline_correlate_many(ftdx_pipeline* pl, void* const* vis_outputs,
+                                 const void* const* raw_inputs, int n_payloads);
+int ftdx_pipeline_destroy(ftdx_pipeline* pl);
+
 /* Replaces CutlassComplexGemm::print_build_info (cutlass_gemm_api.h:773-775). */
 int ftdx_print_build_info(void);
 

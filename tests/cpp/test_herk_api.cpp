@@ -15,6 +15,7 @@
 
 #include "cutlass_gemm_api.h"
 #include "ggp_cutlass_interface.h"
+#include "ggp_xengine.h"
 #include "herk_oracle.h"
 
 #define HIP_OK(x)                                                                       \
@@ -126,6 +127,48 @@ int main(int argc, char** argv) {
     }
     std::printf("  %-40s %s\n", "FP4 + FP32 output throws", threw ? "PASS" : "FAIL");
     failures += threw ? 0 : 1;
+
+    // Test 5: ggp::XEngine (TRI, FP32, host buffers) against the oracle's M^H M
+    {
+        ftdx_xengine_param xp;
+        ftdx_xengine_param_init(&xp);
+        xp.struct_size = sizeof(xp);
+        xp.n_payload = 1;
+        xp.n_antennae_per_payload = 48;
+        xp.n_channels_per_payload = 2;
+        xp.n_time_per_payload = 64;
+        xp.n_time_inner = 2;
+        xp.n_polarizations = 2;
+        xp.compute_prec = xp.output_prec = FTDX_QPREC_SINGLE;
+        xp.format = FTDX_XENGINE_MAT_TRI;
+        const int xN = 48, xK = 32, xB = 8;
+        std::vector<uint8_t> raw((size_t)xN * 64 * 2 * 2);
+        ftdo_gen_bytes(seed + 1, (int64_t)raw.size(), raw.data());
+        const size_t P = (size_t)xN * (xN + 1) / 2;
+        std::vector<float> out(P * xB * 2, -1.0f);
+        std::vector<int64_t> ref(P * xB * 2);
+        ftdo_xengine_ggp_i64(raw.data(), xN, xK, xB, 0, ref.data(), 0);
+        bool ok = true;
+        try {
+            ggp::XEngine xe(xp);
+            xe.compute(out.data(), raw.data());
+            for (size_t i = 0; i < out.size(); ++i) ok = ok && out[i] == (float)ref[i];
+        } catch (const std::runtime_error& e) {
+            std::printf("  XEngine threw: %s\n", e.what());
+            ok = false;
+        }
+        std::printf("  %-40s %s\n", "ggp::XEngine TRI FP32 (host payload)", ok ? "PASS" : "FAIL");
+        failures += ok ? 0 : 1;
+        xp.compute_prec = FTDX_QPREC_HALF;  // rejected, as the reference constructor does
+        bool threw2 = false;
+        try {
+            ggp::XEngine bad(xp);
+        } catch (const std::runtime_error&) {
+            threw2 = true;
+        }
+        std::printf("  %-40s %s\n", "XEngine HALF compute throws", threw2 ? "PASS" : "FAIL");
+        failures += threw2 ? 0 : 1;
+    }
 
     HIP_OK(hipFree(dA));
     HIP_OK(hipFree(dC));

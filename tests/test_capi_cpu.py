@@ -149,3 +149,14 @@ def test_xengine_param_validation_without_device():
     assert L.ftdx_xengine_create(None, ctypes.byref(ctypes.c_void_p())) == ftdx.FTDX_ERR_INVALID_ARG
     assert L.ftdx_xengine_destroy(None) == ftdx.FTDX_OK
     assert L.ftdx_correlate_ftd(None, None, None, 4, 1, 4, 1, 2, 1, None) == ftdx.FTDX_ERR_INVALID_ARG
+
+
+def test_pipeline_validation_without_device():
+    import ctypes
+    from ftd_amd import ftdx
+    L = ftdx.lib()
+    h = ctypes.c_void_p()
+    assert L.ftdx_pipeline_create(None, 8, 8, 1, 1, ctypes.byref(h)) == ftdx.FTDX_ERR_INVALID_ARG
+    assert L.ftdx_pipeline_create(None, 8, 8, 1, 1, None) == ftdx.FTDX_ERR_INVALID_ARG
+    assert L.ftdx_pipeline_correlate_many(None, None, None, 1) == ftdx.FTDX_ERR_INVALID_ARG
+    assert L.ftdx_pipeline_destroy(None) == ftdx.FTDX_OK
