@@ -12,7 +12,9 @@ enum TileConfigId {
     TILE_128x128_K128 = 1,  // 4 waves, 128-sample stages, 1 workgroup / CU
     TILE_64x64_K64 = 2,     // 1 wave, 4 workgroups / CU: small N, fine-grained balance
     TILE_256x128_K64 = 3,   // 8 waves (4x2, 64x64 each), 1 workgroup / CU
-    NUM_TILE_CONFIGS = 4
+    TILE_128x128_K64_P2 = 4,  // as 0, global loads issued two stages ahead
+    TILE_256x128_K64_P2 = 5,  // as 3, global loads issued two stages ahead
+    NUM_TILE_CONFIGS = 6
 };
 
 const char* tile_config_name(int cfg);
@@ -24,7 +26,15 @@ int tile_config_wg_per_cu(int cfg);
 //   OUT_LOWER_PACKED  (i, j) at i(i+1)/2 + j                   (the HERK layout)
 //   OUT_UPPER_PACKED  GGP: row q = j, column p = i at q*N + p - q(q+1)/2
 //   OUT_HERM_FULL     GGP: full M^H M row-major, (i, j) at j*N + i, conj at i*N + j
-enum OutFormat { OUT_LOWER_PACKED = 0, OUT_UPPER_PACKED = 1, OUT_HERM_FULL = 2 };
+//   OUT_GEMM_PLANAR   C = alpha A B^T + beta C, Re and Im planes [M][ldc] (B pre-cast as conj)
+//   OUT_GEMM_POWER    P = alpha |A B^T|^2 + beta P, FP32 [M][ldc]
+enum OutFormat {
+    OUT_LOWER_PACKED = 0,
+    OUT_UPPER_PACKED = 1,
+    OUT_HERM_FULL = 2,
+    OUT_GEMM_PLANAR = 3,
+    OUT_GEMM_POWER = 4
+};
 
 struct HerkPlainArgs {
     const uint8_t* A;   // [batch][N][K] QC bytes, or pre-cast FP8 rows when precast != 0
@@ -50,6 +60,22 @@ hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, int cfg, bool out_fp16
 // of M^H M packed row-major, fmt 2 = full N x N Hermitian row-major; element type
 // float or (out_fp64) double.  C batch stride c_bstride counts scalars.
 hipError_t launch_herk_ggp(const HerkPlainArgs& a, int cfg, int fmt, bool out_fp64, int xcd_remap,
+                           hipStream_t stream);
+
+// ---- complex GEMM over pre-cast rows (the prepared-B beamformer path) ----
+struct GemmArgs {
+    const uint8_t* A;   // pre-cast rows of A [batch][M] (pitch lda bytes)
+    const uint8_t* B;   // pre-cast rows of conj(B) [batch][N] (pitch ldb bytes)
+    void* C;            // OUT_GEMM_PLANAR: Re plane; OUT_GEMM_POWER: power (float)
+    void* C2;           // OUT_GEMM_PLANAR: Im plane
+    int M, N, K, batch;
+    float alpha, beta;
+    int64_t lda, ldb, a_bstride, b_bstride;  // bytes
+    int64_t ldc, c_bstride;                  // elements
+};
+
+// fmt = OUT_GEMM_PLANAR (float or, out_fp16, half planes) or OUT_GEMM_POWER (float).
+hipError_t launch_gemm_fp8(const GemmArgs& a, int cfg, int fmt, bool out_fp16, int xcd_remap,
                            hipStream_t stream);
 
 // ---- grouped persistent launch ----

@@ -23,6 +23,32 @@ using Cfg0 = TileCfg<128, 128, 2, 2, 64, 2, 2>;
 using Cfg1 = TileCfg<128, 128, 2, 2, 128, 1, 1>;
 using Cfg2 = TileCfg<64, 64, 1, 1, 64, 1, 4>;
 using Cfg3 = TileCfg<256, 128, 4, 2, 64, 2, 1>;
+using Cfg4 = TileCfg<128, 128, 2, 2, 64, 2, 2, 2>;  // Cfg0 with loads two stages ahead
+using Cfg5 = TileCfg<256, 128, 4, 2, 64, 2, 1, 2>;  // Cfg3 with loads two stages ahead
+
+// Calls f(CfgN{}) for tile configuration id `cfg`; hipErrorInvalidValue for unknown ids.
+template <typename F>
+hipError_t with_cfg(int cfg, F&& f) {
+    switch (cfg) {
+        case TILE_128x128_K64: return f(Cfg0{});
+        case TILE_128x128_K128: return f(Cfg1{});
+        case TILE_64x64_K64: return f(Cfg2{});
+        case TILE_256x128_K64: return f(Cfg3{});
+        case TILE_128x128_K64_P2: return f(Cfg4{});
+        case TILE_256x128_K64_P2: return f(Cfg5{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename F>
+int cfg_value(int cfg, F&& f, int fallback) {
+    int v = fallback;
+    (void)with_cfg(cfg, [&](auto c) {
+        v = f(c);
+        return hipSuccess;
+    });
+    return v;
+}
 
 // logical workgroup id for hardware id w when consecutive ids go round-robin over
 // 8 XCDs: XCD x gets one contiguous range of logical ids.
@@ -42,7 +68,7 @@ herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
     const int t = w - b * tiles;
     int I, J;
     tri_decode_r(t, Cfg::TM / Cfg::TN, I, J);
-    TileJob j;
+    TileJob j = {};
     j.A = p.A + (int64_t)b * p.a_bstride;
     j.C = static_cast<OutT*>(p.C) + (int64_t)b * p.c_bstride;
     j.N = p.N;
@@ -67,7 +93,7 @@ herk_int4_fp8_grouped_kernel(const GroupProblemDev* __restrict__ probs,
         const GroupProblemDev pr = probs[item.prob];
         int I, J;
         tri_decode_r(item.tile, Cfg::TM / Cfg::TN, I, J);
-        TileJob j;
+        TileJob j = {};
         j.A = pr.A;
         j.C = pr.C;
         j.N = pr.N;
@@ -78,9 +104,9 @@ herk_int4_fp8_grouped_kernel(const GroupProblemDev* __restrict__ probs,
         j.beta = pr.beta;
         j.lda = pr.K;
         if (pr.vec)
-            TileRunner<Cfg>::template run<OutT, LOAD_VEC16>(j, lds);
+            TileRunner<Cfg>::template run<OutT, LOAD_VEC16, OUT_LOWER_PACKED, true>(j, lds);
         else
-            TileRunner<Cfg>::template run<OutT, LOAD_BYTES>(j, lds);
+            TileRunner<Cfg>::template run<OutT, LOAD_BYTES, OUT_LOWER_PACKED, true>(j, lds);
     }
 }
 
@@ -116,6 +142,56 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
             hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_BYTES>), grid, block, 0,
                                stream, a, tiles, xcd_remap);
     }
+    return hipGetLastError();
+}
+
+// Complex GEMM C = alpha A B^T + beta C over pre-cast rows of A and conj(B): full
+// tile grid, row-major tile order within a batch, XCD-aware order when asked.
+template <class Cfg, typename OutT, int FMT>
+__global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
+gemm_fp8_kernel(GemmArgs p, int tiles_n, int tiles, int xcd_remap) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
+    const int total = tiles * p.batch;
+    const int w = xcd_remap ? xcd_logical((int)blockIdx.x, total) : (int)blockIdx.x;
+    const int b = w / tiles;
+    const int t = w - b * tiles;
+    const int I = t / tiles_n, J = t - I * tiles_n;
+    TileJob j = {};
+    j.A = p.A + (int64_t)b * p.a_bstride;
+    j.B = p.B + (int64_t)b * p.b_bstride;
+    j.C = static_cast<OutT*>(p.C) + (int64_t)b * p.c_bstride;
+    j.C2 = p.C2 ? static_cast<void*>(static_cast<OutT*>(p.C2) + (int64_t)b * p.c_bstride) : nullptr;
+    j.N = p.M;
+    j.Ncol = p.N;
+    j.K = p.K;
+    j.row0 = I * Cfg::TM;
+    j.col0 = J * Cfg::TN;
+    j.alpha = p.alpha;
+    j.beta = p.beta;
+    j.lda = p.lda;
+    j.ldb = p.ldb;
+    j.ldc = p.ldc;
+    TileRunner<Cfg>::template run<OutT, LOAD_FP8, FMT>(j, lds);
+}
+
+template <class Cfg>
+hipError_t launch_gemm_cfg(const GemmArgs& a, int fmt, bool out_fp16, int xcd_remap, hipStream_t stream) {
+    const int64_t tiles_m = (a.M + Cfg::TM - 1) / Cfg::TM, tiles_n = (a.N + Cfg::TN - 1) / Cfg::TN;
+    const int64_t blocks = tiles_m * tiles_n * a.batch;
+    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+    dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    const int tn = (int)tiles_n, tiles = (int)(tiles_m * tiles_n);
+    if (fmt == OUT_GEMM_POWER)
+        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_POWER>), grid, block, 0, stream, a, tn,
+                           tiles, xcd_remap);
+    else if (fmt == OUT_GEMM_PLANAR && out_fp16)
+        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, _Float16, OUT_GEMM_PLANAR>), grid, block, 0, stream, a,
+                           tn, tiles, xcd_remap);
+    else if (fmt == OUT_GEMM_PLANAR)
+        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_PLANAR>), grid, block, 0, stream, a, tn,
+                           tiles, xcd_remap);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -168,38 +244,22 @@ const char* tile_config_name(int cfg) {
         case TILE_128x128_K128: return "t128x128_k128";
         case TILE_64x64_K64: return "t64x64_k64";
         case TILE_256x128_K64: return "t256x128_k64";
+        case TILE_128x128_K64_P2: return "t128x128_k64_p2";
+        case TILE_256x128_K64_P2: return "t256x128_k64_p2";
         default: return "unknown";
     }
 }
 
 int tile_config_rows(int cfg) {
-    switch (cfg) {
-        case TILE_128x128_K64: return Cfg0::TM;
-        case TILE_128x128_K128: return Cfg1::TM;
-        case TILE_64x64_K64: return Cfg2::TM;
-        case TILE_256x128_K64: return Cfg3::TM;
-        default: return 0;
-    }
+    return cfg_value(cfg, [](auto c) { return decltype(c)::TM; }, 0);
 }
 
 int tile_config_cols(int cfg) {
-    switch (cfg) {
-        case TILE_128x128_K64: return Cfg0::TN;
-        case TILE_128x128_K128: return Cfg1::TN;
-        case TILE_64x64_K64: return Cfg2::TN;
-        case TILE_256x128_K64: return Cfg3::TN;
-        default: return 0;
-    }
+    return cfg_value(cfg, [](auto c) { return decltype(c)::TN; }, 0);
 }
 
 int tile_config_wg_per_cu(int cfg) {
-    switch (cfg) {
-        case TILE_128x128_K64: return Cfg0::WG_PER_CU;
-        case TILE_128x128_K128: return Cfg1::WG_PER_CU;
-        case TILE_64x64_K64: return Cfg2::WG_PER_CU;
-        case TILE_256x128_K64: return Cfg3::WG_PER_CU;
-        default: return 1;
-    }
+    return cfg_value(cfg, [](auto c) { return decltype(c)::WG_PER_CU; }, 1);
 }
 
 int herk_tiles(int cfg, int N) { return tri_tiles(N, tile_config_rows(cfg), tile_config_cols(cfg)); }
@@ -207,42 +267,34 @@ int herk_tiles(int cfg, int N) { return tri_tiles(N, tile_config_rows(cfg), tile
 hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, int cfg, bool out_fp16, int xcd_remap,
                                   hipStream_t stream) {
     if (a.N <= 0 || a.batch <= 0) return hipSuccess;
-    switch (cfg) {
-        case TILE_128x128_K64: return launch_plain_cfg<Cfg0>(a, out_fp16, xcd_remap, stream);
-        case TILE_128x128_K128: return launch_plain_cfg<Cfg1>(a, out_fp16, xcd_remap, stream);
-        case TILE_64x64_K64: return launch_plain_cfg<Cfg2>(a, out_fp16, xcd_remap, stream);
-        case TILE_256x128_K64: return launch_plain_cfg<Cfg3>(a, out_fp16, xcd_remap, stream);
-        default: return hipErrorInvalidValue;
-    }
+    return with_cfg(cfg, [&](auto c) {
+        return launch_plain_cfg<decltype(c)>(a, out_fp16, xcd_remap, stream);
+    });
 }
 
 hipError_t launch_herk_ggp(const HerkPlainArgs& a, int cfg, int fmt, bool out_fp64, int xcd_remap,
                            hipStream_t stream) {
     if (a.N <= 0 || a.batch <= 0) return hipSuccess;
-    switch (cfg) {
-        case TILE_128x128_K64: return launch_ggp_cfg<Cfg0>(a, fmt, out_fp64, xcd_remap, stream);
-        case TILE_128x128_K128: return launch_ggp_cfg<Cfg1>(a, fmt, out_fp64, xcd_remap, stream);
-        case TILE_64x64_K64: return launch_ggp_cfg<Cfg2>(a, fmt, out_fp64, xcd_remap, stream);
-        case TILE_256x128_K64: return launch_ggp_cfg<Cfg3>(a, fmt, out_fp64, xcd_remap, stream);
-        default: return hipErrorInvalidValue;
-    }
+    return with_cfg(cfg, [&](auto c) {
+        return launch_ggp_cfg<decltype(c)>(a, fmt, out_fp64, xcd_remap, stream);
+    });
+}
+
+hipError_t launch_gemm_fp8(const GemmArgs& a, int cfg, int fmt, bool out_fp16, int xcd_remap,
+                           hipStream_t stream) {
+    if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return hipSuccess;
+    return with_cfg(cfg, [&](auto c) {
+        return launch_gemm_cfg<decltype(c)>(a, fmt, out_fp16, xcd_remap, stream);
+    });
 }
 
 hipError_t launch_herk_int4_grouped(const GroupProblemDev* probs, const GroupItemDev* items,
                                     const int* wg_begin, int num_wg, int cfg, bool out_fp16,
                                     hipStream_t stream) {
     if (num_wg <= 0) return hipSuccess;
-    switch (cfg) {
-        case TILE_128x128_K64:
-            return launch_grouped_cfg<Cfg0>(probs, items, wg_begin, num_wg, out_fp16, stream);
-        case TILE_128x128_K128:
-            return launch_grouped_cfg<Cfg1>(probs, items, wg_begin, num_wg, out_fp16, stream);
-        case TILE_64x64_K64:
-            return launch_grouped_cfg<Cfg2>(probs, items, wg_begin, num_wg, out_fp16, stream);
-        case TILE_256x128_K64:
-            return launch_grouped_cfg<Cfg3>(probs, items, wg_begin, num_wg, out_fp16, stream);
-        default: return hipErrorInvalidValue;
-    }
+    return with_cfg(cfg, [&](auto c) {
+        return launch_grouped_cfg<decltype(c)>(probs, items, wg_begin, num_wg, out_fp16, stream);
+    });
 }
 
 }  // namespace ftdx

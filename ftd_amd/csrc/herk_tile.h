@@ -25,9 +25,14 @@ namespace ftdx {
 // WAVES_PER_EU is the occupancy target given to the compiler (caps VGPRs at
 // 512 / WAVES_PER_EU); WG_PER_CU is how many workgroups fit one CU (LDS- and
 // register-limited), used to size persistent grids.
-template <int TM_, int TN_, int WAVES_M_, int WAVES_N_, int KC_, int WAVES_PER_EU_, int WG_PER_CU_>
+template <int TM_, int TN_, int WAVES_M_, int WAVES_N_, int KC_, int WAVES_PER_EU_, int WG_PER_CU_,
+          int DEPTH_ = 1>
 struct TileCfg {
     static constexpr int TM = TM_, TN = TN_;
+    // Global-load prefetch distance in stages: 1 = the next stage's loads are in
+    // flight while this stage computes; 2 = two register sets, loads issued two
+    // stages ahead (covers twice the latency for SEGS_PER_THREAD more VGPRs).
+    static constexpr int DEPTH = DEPTH_;
     static constexpr int WAVES_M = WAVES_M_, WAVES_N = WAVES_N_;
     static constexpr int KC = KC_;
     static constexpr int WAVES_PER_EU = WAVES_PER_EU_;
@@ -38,6 +43,10 @@ struct TileCfg {
     static constexpr int PITCH = 2 * KC + 16;         // LDS bytes per staged row (+16 B pad)
     static constexpr int STAGE = (TM + TN) * PITCH;   // row block then column block
     static constexpr int LDS_BYTES = 2 * STAGE;
+    // Staged epilogue: each wave gathers 16 or 32 rows x WN columns of FP32 [re, im]
+    // in its own LDS region.
+    static constexpr bool EPI_STAGED =
+        WN % 64 == 0 && WAVES_M * WAVES_N * 16 * (WN * 8 + 16) <= LDS_BYTES;
     static constexpr int SEGS_PER_ROW = KC / 16;      // 16-byte global segments per row
     static constexpr int SEGS = (TM + TN) * SEGS_PER_ROW;
     static constexpr int SEGS_PER_THREAD = SEGS / THREADS;
@@ -59,10 +68,18 @@ enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1, LOAD_FP8 = 2 };
 struct TileJob {
     const uint8_t* A;  // this problem's operand rows
     void* C;           // this problem's packed triangle (scalars)
-    int N, K;
+    int N, K;          // N: rows (HERK: also columns)
     int row0, col0;    // first row / column of the tile
     float alpha, beta;
     int64_t lda;       // row pitch of A in bytes (LOAD_FP8 only)
+    // GEMM layouts only (C = alpha A B^T + beta C, full M x Ncol grid): the column
+    // operand's pre-cast rows and pitch, the column count, the imaginary output
+    // plane (planar layout) and the output row pitch in elements.
+    const uint8_t* B;
+    int64_t ldb;
+    int Ncol;
+    void* C2;
+    int64_t ldc;
 };
 
 // 4 QC bytes (4 samples) -> u dwords: re = fp8(Re0..Re3), im = fp8(Im0..Im3).
@@ -178,6 +195,8 @@ struct TileRunner {
     }
 
     // Pre-cast FP8 rows: segment slot q of thread tid covers 8 samples of one row.
+    // GEMM: column rows come from j.B (pitch ldb, Ncol rows) instead of j.A.
+    template <bool GEMM = false>
     static __device__ __forceinline__ void load_stage_fp8(const TileJob& j, int k0, bool share_cols,
                                                           int tid, uint4 (&g)[Cfg::SEGS8_PER_THREAD]) {
 #pragma unroll
@@ -187,10 +206,14 @@ struct TileRunner {
             const bool is_col = lrow >= Cfg::TM;
             const int grow = is_col ? j.col0 + (lrow - Cfg::TM) : j.row0 + lrow;
             const int kk = k0 + seg * 8;
+            const bool from_b = GEMM && is_col;
+            const uint8_t* base = from_b ? j.B : j.A;
+            const int64_t pitch = from_b ? j.ldb : j.lda;
+            const int lim = from_b ? j.Ncol : j.N;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             // pre-cast rows are zero-padded to a multiple of 16 samples
-            if (!(is_col && share_cols) && grow < j.N && kk < j.K)
-                v = *reinterpret_cast<const uint4*>(j.A + (int64_t)grow * j.lda + (int64_t)kk * 2);
+            if (!(is_col && share_cols) && grow < lim && kk < j.K)
+                v = *reinterpret_cast<const uint4*>(base + (int64_t)grow * pitch + (int64_t)kk * 2);
             g[q] = v;
         }
     }
@@ -276,8 +299,94 @@ struct TileRunner {
         OutIO<OutT>::store(j.C, idx, re, im);
     }
 
-    template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED>
+    // Lower-packed epilogue through LDS.  Packed-triangle rows start at arbitrary
+    // 8-byte offsets, so the accumulator layout's stores (two 256-byte runs of two
+    // rows per instruction) leave partial cache lines at every run end.  Here each
+    // wave gathers RP rows of its own block in a private LDS region (the stage
+    // buffers are free after the main loop) and writes each row's WN-column run as
+    // one 512-byte instruction.  No cross-wave barrier until the final one, which
+    // lets the caller reuse `lds`.
+    template <typename OutT>
+    static __device__ __forceinline__ void epilogue_staged(
+        const TileJob& j, uint8_t* __restrict__ lds, const v16f (&acc_re)[Cfg::FM][Cfg::FN],
+        const v16f (&acc_im)[Cfg::FM][Cfg::FN], bool active, int wrow0, int wcol0, int lane,
+        int wave) {
+        constexpr int NW = Cfg::WAVES_M * Cfg::WAVES_N;
+        constexpr int EPW = Cfg::WN * 8 + 16;
+        constexpr int HS = (NW * 32 * EPW <= Cfg::LDS_BYTES) ? 1 : 2;  // passes per 32-row block
+        constexpr int RP = 32 / HS;
+        static_assert(NW * RP * EPW <= Cfg::LDS_BYTES, "wave epilogue regions exceed LDS");
+        static_assert(Cfg::WN % 64 == 0, "row runs are written 64 columns at a time");
+        uint8_t* region = lds + wave * (RP * EPW);
+        const int n = lane & 31, g4 = (lane >> 5) * 4;
+        if (active) {
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+                for (int h = 0; h < HS; ++h) {
+#pragma unroll
+                    for (int b = 0; b < Cfg::FN; ++b)
+#pragma unroll
+                        for (int r = h * (16 / HS); r < (h + 1) * (16 / HS); ++r) {
+                            const int lrow = (r >> 2) * 8 + g4 + (r & 3) - h * RP;
+                            v2f v;
+                            v[0] = acc_re[a][b][r];
+                            v[1] = acc_im[a][b][r];
+                            *reinterpret_cast<v2f*>(region + lrow * EPW + (b * 32 + n) * 8) = v;
+                        }
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+                    for (int lr = 0; lr < RP; ++lr) {
+                        const int gi = wrow0 + a * 32 + h * RP + lr;
+                        if (gi >= j.N) break;
+                        const int64_t rbase = tri_index(gi, 0);
+#pragma unroll
+                        for (int c0 = 0; c0 < Cfg::WN; c0 += 64) {
+                            const int c = c0 + lane;
+                            const int gj = wcol0 + c;
+                            if (gj <= gi) {
+                                const v2f v = *reinterpret_cast<const v2f*>(region + lr * EPW + c * 8);
+                                put<OutT>(j, rbase + gj, v[0], gi == gj ? 0.0f : v[1]);
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+        }
+        __syncthreads();
+    }
+
+    // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
+    // live and would spill the second register set).
+    // GEMM-layout epilogue: element (i, j) of C = alpha A B^T + beta C at i*ldc + j.
+    template <typename OutT, int FMT>
+    static __device__ __forceinline__ void put_gemm(const TileJob& j, int64_t idx, float re, float im) {
+        if constexpr (FMT == OUT_GEMM_POWER) {
+            // power = alpha (re^2 + im^2) [+ beta old]: the reference's expression
+            // (cutlass_gemm_api.cu:425-437), evaluated as fma(re, re, im * im)
+            float p = j.alpha * __builtin_fmaf(re, re, im * im);
+            float* P = static_cast<float*>(j.C);
+            if (j.beta != 0.0f) p = __builtin_fmaf(j.beta, P[idx], p);
+            __builtin_nontemporal_store(p, P + idx);
+        } else {
+            OutT* Cr = static_cast<OutT*>(j.C);
+            OutT* Ci = static_cast<OutT*>(j.C2);
+            float r = re * j.alpha, m = im * j.alpha;
+            if (j.beta != 0.0f) {
+                r = __builtin_fmaf(j.beta, (float)Cr[idx], r);
+                m = __builtin_fmaf(j.beta, (float)Ci[idx], m);
+            }
+            __builtin_nontemporal_store((OutT)r, Cr + idx);
+            __builtin_nontemporal_store((OutT)m, Ci + idx);
+        }
+    }
+
+    // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
+    // live and would spill the second register set).
+    template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false>
     static __device__ __forceinline__ void run(const TileJob& j, uint8_t* __restrict__ lds) {
+        constexpr bool GEMM = FMT == OUT_GEMM_PLANAR || FMT == OUT_GEMM_POWER;
+        static_assert(!GEMM || MODE == LOAD_FP8, "GEMM layouts read pre-cast rows");
         const int tid = threadIdx.x;
         const int lane = tid & 63, wave = tid >> 6;
         const int wm = wave / Cfg::WAVES_N, wn = wave % Cfg::WAVES_N;
@@ -285,9 +394,12 @@ struct TileRunner {
         const int wcol0 = j.col0 + wn * Cfg::WN;
         // The column block lies inside the row block (diagonal tiles): read it from the
         // row region instead of loading it twice.
-        const bool share_cols = (j.col0 >= j.row0) && (j.col0 + Cfg::TN <= j.row0 + Cfg::TM);
-        // Wave has output iff some (i >= j) with i < N in its block.
-        const bool active = (wrow0 < j.N) && (wrow0 + Cfg::WM - 1 >= wcol0);
+        const bool share_cols =
+            !GEMM && (j.col0 >= j.row0) && (j.col0 + Cfg::TN <= j.row0 + Cfg::TM);
+        // HERK: wave has output iff some (i >= j) with i < N in its block; GEMM: any
+        // in-range element.
+        const bool active = GEMM ? (wrow0 < j.N && wcol0 < j.Ncol)
+                                 : (wrow0 < j.N) && (wrow0 + Cfg::WM - 1 >= wcol0);
 
         v16f acc_re[Cfg::FM][Cfg::FN], acc_im[Cfg::FM][Cfg::FN];
 #pragma unroll
@@ -302,30 +414,21 @@ struct TileRunner {
 
         const int nst = (j.K + Cfg::KC - 1) / Cfg::KC;
         constexpr int NG = MODE == LOAD_FP8 ? Cfg::SEGS8_PER_THREAD : Cfg::SEGS_PER_THREAD;
-        uint4 g[NG];
-        auto load = [&](int k0) {
-            if constexpr (MODE == LOAD_FP8) load_stage_fp8(j, k0, share_cols, tid, g);
+        auto load = [&](int k0, uint4 (&g)[NG]) {
+            if constexpr (MODE == LOAD_FP8) load_stage_fp8<GEMM>(j, k0, share_cols, tid, g);
             else load_stage<MODE>(j, k0, share_cols, tid, g);
         };
-        auto store = [&](uint8_t* st) {
+        auto store = [&](uint8_t* st, const uint4 (&g)[NG]) {
             if constexpr (MODE == LOAD_FP8) store_stage_fp8(st, share_cols, tid, g);
             else store_stage(st, share_cols, tid, g);
         };
-        if (nst > 0) {
-            load(0);
-            store(lds);
-        }
-        __syncthreads();
 
         const int lr = lane & 31, lh = (lane >> 5) * 32;
         const int a_off = (wm * Cfg::WM + lr) * Cfg::PITCH + lh;
         const int b_off = (share_cols ? (j.col0 - j.row0) : Cfg::TM) * Cfg::PITCH +
                           (wn * Cfg::WN + lr) * Cfg::PITCH + lh;
 
-        for (int s = 0; s < nst; ++s) {
-            const bool more = (s + 1 < nst);
-            if (more) load((s + 1) * Cfg::KC);
-            const uint8_t* st = lds + (s & 1) * Cfg::STAGE;
+        auto compute = [&](const uint8_t* st) {
             if (active) {
 #pragma unroll
                 for (int ks = 0; ks < Cfg::KC / 32; ++ks) {
@@ -341,7 +444,7 @@ struct TileRunner {
                         const v8i av = conj_planar(au[a]);
 #pragma unroll
                         for (int b = 0; b < Cfg::FN; ++b) {
-                            if constexpr (FMT == OUT_LOWER_PACKED) {
+                            if constexpr (FMT == OUT_LOWER_PACKED || GEMM) {
                                 acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
                                 acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
                             } else {  // transposed accumulator, same values
@@ -352,10 +455,72 @@ struct TileRunner {
                     }
                 }
             }
-            if (more) store(lds + ((s + 1) & 1) * Cfg::STAGE);
+        };
+
+        // Depth 2 only where its second register set fits without spilling: the
+        // fused 16-byte load mode (pre-cast stages are twice as wide).
+        constexpr int DEPTH = (MODE == LOAD_VEC16 && !SHALLOW) ? Cfg::DEPTH : 1;
+        if constexpr (DEPTH == 1) {
+            uint4 g[NG];
+            if (nst > 0) {
+                load(0, g);
+                store(lds, g);
+            }
             __syncthreads();
+            for (int s = 0; s < nst; ++s) {
+                const bool more = (s + 1 < nst);
+                if (more) load((s + 1) * Cfg::KC, g);
+                compute(lds + (s & 1) * Cfg::STAGE);
+                if (more) store(lds + ((s + 1) & 1) * Cfg::STAGE, g);
+                __syncthreads();
+            }
+        } else {
+            // Two register sets: ga holds the stage after the one being computed
+            // while gb is loading the one after that; roles swap every stage, so the
+            // loop runs in pairs (stage s in LDS buffer 0, s + 1 in buffer 1).
+            uint4 ga[NG], gb[NG];
+            if (nst > 0) {
+                load(0, ga);
+                store(lds, ga);
+            }
+            if (nst > 1) load(Cfg::KC, ga);
+            __syncthreads();
+            for (int s = 0; s < nst; s += 2) {
+                if (s + 2 < nst) load((s + 2) * Cfg::KC, gb);
+                compute(lds);
+                if (s + 1 < nst) store(lds + Cfg::STAGE, ga);
+                __syncthreads();
+                if (s + 1 >= nst) break;
+                if (s + 3 < nst) load((s + 3) * Cfg::KC, ga);
+                compute(lds + Cfg::STAGE);
+                if (s + 2 < nst) store(lds, gb);
+                __syncthreads();
+            }
         }
 
+        if constexpr (GEMM) {
+            // B rows hold conj(B), so the HERK product A conj(B)^H = A B^T; lane =
+            // column in the accumulator layout: consecutive lanes store consecutive j.
+            if (!active) return;
+            const int n = lane & 31, g4 = (lane >> 5) * 4;
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+                for (int b = 0; b < Cfg::FN; ++b) {
+                    const int gj = wcol0 + b * 32 + n;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int gi = wrow0 + a * 32 + (r >> 2) * 8 + g4 + (r & 3);
+                        if (gi < j.N && gj < j.Ncol)
+                            put_gemm<OutT, FMT>(j, (int64_t)gi * j.ldc + gj, acc_re[a][b][r],
+                                                acc_im[a][b][r]);
+                    }
+                }
+            return;
+        } else if constexpr (FMT == OUT_LOWER_PACKED && Cfg::EPI_STAGED) {
+            epilogue_staged<OutT>(j, lds, acc_re, acc_im, active, wrow0, wcol0, lane, wave);
+            return;
+        }
         if (!active) return;
 
         // Epilogue. 32x32 accumulator layout: lane holds column (lane & 31); register

@@ -145,7 +145,10 @@ TILE_128x128_K64 = 0
 TILE_128x128_K128 = 1
 TILE_64x64_K64 = 2
 TILE_256x128_K64 = 3
-TILE_CONFIGS = (TILE_128x128_K64, TILE_128x128_K128, TILE_64x64_K64, TILE_256x128_K64)
+TILE_128x128_K64_P2 = 4
+TILE_256x128_K64_P2 = 5
+TILE_CONFIGS = (TILE_128x128_K64, TILE_128x128_K128, TILE_64x64_K64, TILE_256x128_K64,
+                TILE_128x128_K64_P2, TILE_256x128_K64_P2)
 
 
 class HerkProblem(ctypes.Structure):

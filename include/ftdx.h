@@ -74,6 +74,8 @@ extern "C" {
 #define FTDX_TILE_128x128_K128 1 /* 4 waves, 1 workgroup per CU */
 #define FTDX_TILE_64x64_K64 2    /* 1 wave, 4 workgroups per CU */
 #define FTDX_TILE_256x128_K64 3  /* 8 waves, 1 workgroup per CU */
+#define FTDX_TILE_128x128_K64_P2 4 /* as 0, global loads two stages ahead */
+#define FTDX_TILE_256x128_K64_P2 5 /* as 3, global loads two stages ahead */
 
 typedef struct ftdx_context ftdx_context;
 typedef struct ftdx_grouped_plan ftdx_grouped_plan;

@@ -14,10 +14,11 @@ def _bits(a):
     return a.view(np.uint32 if a.dtype == np.float32 else np.uint16)
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("remap", [0, 1])
 @pytest.mark.parametrize("N,K,B", [(1, 1, 2), (63, 70, 3), (129, 64, 2), (257, 100, 1),
-                                   (300, 256, 2), (520, 128, 1)])
+                                   (300, 256, 2), (520, 128, 1),
+                                   (97, 150, 2), (64, 320, 1)])  # 3 and 5 stages of 64
 def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B):
     import torch
     from ftd_amd import ftdx
@@ -33,7 +34,7 @@ def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B):
     assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref))
 
 
-@pytest.mark.parametrize("cfg", [1, 2, 3])
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
 def test_tile_configs_fp16_beta(oracle, dev, cfg):
     import torch
     from ftd_amd import ftdx
@@ -65,7 +66,7 @@ def _grouped_case(oracle, dev, shapes, seed):
     return probs, refs
 
 
-@pytest.mark.parametrize("cfg", [-1, 0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", [-1, 0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("graph", [False, True])
 def test_grouped_uneven_bit_exact(oracle, dev, cfg, graph):
     import torch

@@ -50,7 +50,7 @@ def test_xengine_host_buffers_match_oracle(oracle, dev, shape, prec, fmt):
     xe.close()
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("fmt", ["tri", "herm"])
 def test_xengine_every_tile_config(oracle, dev, cfg, fmt):
     import torch
