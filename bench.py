@@ -234,6 +234,35 @@ def calibration_study(torch, ftdx, dev, N: int, K: int, B: int, seed: int, reps:
     }
 
 
+def beamformer_study(torch, ftdx, dev, M: int, N: int, K: int, B: int, seed: int, reps: int) -> dict:
+    """Voltage-beamformer product with prepared weights: INT4 QC voltages A [B][M][K]
+    times conj-prepared FP16 weights B [B][N][K], fused power detection
+    (gemm_prepared_power_int4); 4M-equivalent FLOP 8 M N K per batch."""
+    g = torch.Generator(device=dev).manual_seed(seed)
+    A = synth_qc(torch, M * K * B, seed, dev)
+    Wre = torch.randint(-7, 8, (B, N, K), generator=g, device=dev, dtype=torch.int16).to(torch.float16)
+    Wim = torch.randint(-7, 8, (B, N, K), generator=g, device=dev, dtype=torch.int16).to(torch.float16)
+    P = torch.empty((B, M, N), dtype=torch.float32, device=dev)
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.prepare_b(Wre, Wim, N, K, B)
+    del Wre, Wim
+    for _ in range(2):
+        ctx.gemm_prepared_power_int4(A, P, M, N, K, B)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ctx.gemm_prepared_power_int4(A, P, M, N, K, B)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    tf = 8.0 * M * N * K * B / (ms * 1e-3) / 1e12
+    return {"M": M, "N_beams": N, "K_antennas": K, "batch": B, "seed": seed,
+            "entry": "gemm_prepared_power_int4", "ms": round(ms, 4),
+            "tflops_4m_equiv": round(tf, 2),
+            "pct_fp8_peak": round(100.0 * tf / FP8_DENSE_PEAK_TFLOPS, 2)}
+
+
 def streaming_study(torch, ftdx, dev, N: int, K: int, B: int, payloads: int, seed: int) -> dict:
     """Host payloads through ftdx_pipeline_* (H2D, HERK, D2H double-buffered): the
     PCIe-inclusive rate next to the HERK alone on resident data."""
@@ -357,6 +386,7 @@ def main() -> int:
     ap.add_argument("--no-mx", action="store_true")
     ap.add_argument("--no-calib", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-beamformer", action="store_true")
     ap.add_argument("--batched-problems", type=int, default=512)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed steps, one plain device copy of this many MiB "
@@ -367,7 +397,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "channels shard across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
+              "channels shard across GPUs with no exchange, one process per GPU", file=sys.stderr)
         return 2
 
     import torch
@@ -451,6 +481,13 @@ def main() -> int:
             f"adjoint {calib['herk_adjoint_ms']:.3f} ms; dX err {calib['dX_max_rel_err_vs_fp64']:.2e}")
         torch.cuda.empty_cache()
 
+    bf = None
+    if not args.no_beamformer:
+        log("bench: voltage beamformer GEMM + power, M=256 N=4000 K=1664 batch=128 ...")
+        bf = beamformer_study(torch, ftdx, dev, 256, 4000, 1664, 128, 7, 10)
+        log(f"bench: beamformer {bf['ms']:.3f} ms, {bf['tflops_4m_equiv']:.0f} TFLOP/s 4M-equiv")
+        torch.cuda.empty_cache()
+
     stream_res = None
     if not args.no_stream:
         log("bench: streaming host payloads 1024 x 16 x K=2048, 8 payloads ...")
@@ -499,6 +536,7 @@ def main() -> int:
         "mx_block_scaled": mx,
         "gain_calibration": calib,
         "streaming_host_payloads": stream_res,
+        "beamformer_gemm_power": bf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -75,6 +75,72 @@ int CutlassComplexGemm::herk(const void* A, void* C, int N, int K, int batch_cou
                  "herk");
 }
 
+int CutlassComplexGemm::gemm(const void* A_re, const void* A_im, const void* B_re, const void* B_im,
+                             void* C_re, void* C_im, int M, int N, int K, int batch_count,
+                             ComputePrecision compute, OutputPrecision output, float alpha,
+                             float beta, hipStream_t stream, bool tune, InputPrecision input) {
+    return check(ftdx_gemm(ctx_, A_re, A_im, B_re, B_im, C_re, C_im, M, N, K, batch_count,
+                           static_cast<int>(compute), static_cast<int>(output), alpha, beta, stream,
+                           tune ? 1 : 0, static_cast<int>(input)),
+                 "gemm");
+}
+
+void CutlassComplexGemm::prepare_b(const void* B_re, const void* B_im, int N, int K, int batch_count,
+                                   ComputePrecision compute, hipStream_t stream) {
+    const int rc = ftdx_prepare_b(ctx_, B_re, B_im, N, K, batch_count, static_cast<int>(compute), stream);
+    if (rc != FTDX_OK) throw std::runtime_error(std::string("prepare_b: ") + ftdx_last_error());
+}
+
+int CutlassComplexGemm::gemm_prepared(const void* A_re, const void* A_im, void* C_re, void* C_im, int M,
+                                      int N, int K, int batch_count, OutputPrecision output,
+                                      float alpha, float beta, hipStream_t stream, bool tune,
+                                      InputPrecision input) {
+    return check(ftdx_gemm_prepared(ctx_, A_re, A_im, C_re, C_im, M, N, K, batch_count,
+                                    static_cast<int>(output), alpha, beta, stream, tune ? 1 : 0,
+                                    static_cast<int>(input)),
+                 "gemm_prepared");
+}
+
+int CutlassComplexGemm::gemm_prepared_direct(const void* A_re, const void* A_im, float* C_re, float* C_im,
+                                             int M, int N, int K, int batch_count, float alpha,
+                                             float beta, hipStream_t stream) {
+    return check(ftdx_gemm_prepared_direct(ctx_, A_re, A_im, C_re, C_im, M, N, K, batch_count, alpha,
+                                           beta, stream),
+                 "gemm_prepared_direct");
+}
+
+int CutlassComplexGemm::gemm_prepared_power(const void* A_re, const void* A_im, float* C_power, int M,
+                                            int N, int K, int batch_count, float alpha, float beta,
+                                            hipStream_t stream) {
+    return check(ftdx_gemm_prepared_power(ctx_, A_re, A_im, C_power, M, N, K, batch_count, alpha, beta,
+                                          stream),
+                 "gemm_prepared_power");
+}
+
+int CutlassComplexGemm::gemm_prepared_power_fp8(const void* A_fp8_interleaved, float* C_power, int M,
+                                                int N, int K, int batch_count, float alpha, float beta,
+                                                hipStream_t stream) {
+    return check(ftdx_gemm_prepared_power_fp8(ctx_, A_fp8_interleaved, C_power, M, N, K, batch_count,
+                                              alpha, beta, stream),
+                 "gemm_prepared_power_fp8");
+}
+
+int CutlassComplexGemm::gemm_prepared_power_int4(const void* A_int4, float* C_power, int M, int N,
+                                                 int K, int batch_count, float alpha, float beta,
+                                                 hipStream_t stream) {
+    return check(ftdx_gemm_prepared_power_int4(ctx_, A_int4, C_power, M, N, K, batch_count, alpha, beta,
+                                               stream),
+                 "gemm_prepared_power_int4");
+}
+
+int CutlassComplexGemm::gemm_prepared_direct_int4(const void* A_int4, float* C_re, float* C_im, int M,
+                                                  int N, int K, int batch_count, float alpha, float beta,
+                                                  hipStream_t stream) {
+    return check(ftdx_gemm_prepared_direct_int4(ctx_, A_int4, C_re, C_im, M, N, K, batch_count, alpha,
+                                                beta, stream),
+                 "gemm_prepared_direct_int4");
+}
+
 void CutlassComplexGemm::print_build_info() { ftdx_print_build_info(); }
 
 }  // namespace cutlass_gemm_api

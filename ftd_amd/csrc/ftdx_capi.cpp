@@ -20,6 +20,7 @@
 #include "herk_mx.h"
 #include "calib.h"
 #include "stages.h"
+#include "imaging.h"
 
 struct ftdx_context {
     int device = 0;
@@ -61,6 +62,13 @@ struct ftdx_context {
     GraphEntry graphs[4];
     unsigned long long graph_clock = 0;
     hipStream_t cap_stream = nullptr;
+    // complex GEMM: prepared conj(B) rows (prepare_b) and a separate buffer for the
+    // one-shot gemm(), so it never clobbers a prepared B
+    uint8_t* prep_b = nullptr;
+    size_t prep_b_cap = 0;
+    int prep_N = -1, prep_K = -1, prep_batch = -1;
+    uint8_t* gemm_b = nullptr;
+    size_t gemm_b_cap = 0;
 };
 
 namespace ftdx {
@@ -230,6 +238,65 @@ int grow_precast(ftdx_context* ctx, size_t need, const char* what) {
     return rc;
 }
 
+// A operand formats of the complex GEMM
+enum GemmInput { GEMM_A_FP16_PLANAR, GEMM_A_INT4, GEMM_A_FP8_INTERLEAVED };
+
+// Converts A to pre-cast rows (context scratch) and runs the GEMM against the rows
+// of conj(B) at `brows`.  fmt: OUT_GEMM_PLANAR (c0 = Re, c1 = Im) or OUT_GEMM_POWER.
+int gemm_run(ftdx_context* ctx, const uint8_t* brows, GemmInput kind, const void* a0, const void* a1,
+             void* c0, void* c1, int M, int N, int K, int batch, int fmt, bool out_fp16, float alpha,
+             float beta, void* stream, const char* what) {
+    const std::string w(what);
+    if (M < 0) return fail(FTDX_ERR_INVALID_ARG, w + ": M must be >= 0");
+    if (M == 0 || N == 0 || batch == 0) return FTDX_OK;
+    if (!c0 || (fmt == OUT_GEMM_PLANAR && !c1))
+        return fail(FTDX_ERR_INVALID_ARG, w + ": NULL output pointer");
+    if (K > 0 && (!a0 || (kind == GEMM_A_FP16_PLANAR && !a1)))
+        return fail(FTDX_ERR_INVALID_ARG, w + ": NULL A pointer");
+    if ((int64_t)M * N > ((int64_t)1 << 40) || (int64_t)M * K > ((int64_t)1 << 40))
+        return fail(FTDX_ERR_INVALID_ARG, w + ": sizes too large");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int Kp = precast_row_samples(K);
+    const int64_t rows = (int64_t)M * batch;
+    int rc = grow_precast(ctx, (size_t)rows * Kp * 2 + 16, "gemm: A rows");
+    if (rc != FTDX_OK) return rc;
+    hipError_t e = hipSuccess;
+    if (K > 0) {
+        if (kind == GEMM_A_FP16_PLANAR)
+            e = launch_promote_fp16_planar_rows(static_cast<const uint16_t*>(a0),
+                                                static_cast<const uint16_t*>(a1), ctx->precast, rows, K,
+                                                false, s);
+        else if (kind == GEMM_A_INT4)
+            e = launch_promote_qc_fp8_planar(static_cast<const uint8_t*>(a0), ctx->precast, rows, K, s);
+        else
+            e = launch_promote_fp8_interleaved_rows(static_cast<const uint8_t*>(a0), ctx->precast, rows,
+                                                    K, s);
+        if (e != hipSuccess) return hip_fail(e, what);
+    }
+    GemmArgs g;
+    g.A = ctx->precast;
+    g.B = brows;
+    g.C = c0;
+    g.C2 = fmt == OUT_GEMM_PLANAR ? c1 : nullptr;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.batch = batch;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.lda = g.ldb = (int64_t)Kp * 2;
+    g.a_bstride = (int64_t)M * Kp * 2;
+    g.b_bstride = (int64_t)N * Kp * 2;
+    g.ldc = N;
+    g.c_bstride = (int64_t)M * N;
+    // 256-row tiles once M fills them: each column block of B is then read once
+    const int cfg = ctx->forced_cfg >= 0 ? ctx->forced_cfg
+                                         : (M >= 192 ? TILE_256x128_K64 : TILE_128x128_K64);
+    const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : kDefaultRemap;
+    e = launch_gemm_fp8(g, cfg, fmt, out_fp16, remap, s);
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
+}
+
 int mx_format(int compute) {
     return compute == FTDX_COMPUTE_FP4 ? MX_FP4_E2M1 : MX_FP6_E3M2;
 }
@@ -264,6 +331,11 @@ int ftdx_create(ftdx_context** out) {
 int ftdx_destroy(ftdx_context* ctx) {
     if (!ctx) return FTDX_OK;
     ftdx_end_herk(ctx);
+    if (ctx->prep_b || ctx->gemm_b) {
+        (void)hipDeviceSynchronize();
+        if (ctx->prep_b) (void)hipFree(ctx->prep_b);
+        if (ctx->gemm_b) (void)hipFree(ctx->gemm_b);
+    }
     delete ctx;
     return FTDX_OK;
 }
@@ -789,6 +861,197 @@ int ftdx_correlate_ftd(ftdx_context* ctx, const void* raw, void* tri_fp32, int n
                                          ctx->precast, n_ant, n_chan, n_time, n_time_inner, n_pol,
                                          twos != 0, cfg, remap, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_correlate_ftd: kernels");
+}
+
+// ---- triangle consumers: polarization sum and pillbox gridding ----
+int ftdx_pol_reduce_triangle(const void* tri_in, void* tri_out, int N, int n_ch, int n_pol, int n_time_inner,
+                             int output, void* stream) {
+    if (N < 0 || n_ch < 0 || n_pol <= 0 || n_time_inner <= 0)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_pol_reduce_triangle: bad sizes");
+    if (output != FTDX_OUTPUT_FP32 && output != FTDX_OUTPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_pol_reduce_triangle: output must be FP32 or FP16");
+    if ((int64_t)N * (N + 1) * n_ch * n_time_inner == 0) return FTDX_OK;
+    if (!tri_in || !tri_out) return fail(FTDX_ERR_INVALID_ARG, "ftdx_pol_reduce_triangle: NULL pointer");
+    hipError_t e = ftdx::launch_pol_reduce(tri_in, tri_out, N, n_ch, n_pol, n_time_inner,
+                                           output == FTDX_OUTPUT_FP16, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_pol_reduce_triangle");
+}
+
+int ftdx_herk_pol_reduce(ftdx_context* ctx, const void* A, void* C, int N, int K, int n_ch, int n_pol,
+                         int n_time_inner, int output, float alpha, float beta, void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_reduce: NULL context");
+    if (N < 0 || K < 0 || n_ch < 0 || n_pol <= 0 || n_time_inner <= 0)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_reduce: bad sizes");
+    if (output != FTDX_OUTPUT_FP32 && output != FTDX_OUTPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_reduce: output must be FP32 or FP16");
+    const int64_t out_batch = (int64_t)n_ch * n_time_inner;
+    if ((int64_t)K * n_pol > 0x7FFFFFFF || out_batch * n_pol > 0x7FFFFFFF)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_reduce: sizes too large");
+    if (N == 0 || out_batch == 0) return FTDX_OK;
+    if (!C || (!A && K > 0)) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_reduce: NULL pointer");
+    ftdx::HerkPlainArgs a{};
+    a.A = static_cast<const uint8_t*>(A);
+    a.C = C;
+    a.N = N;
+    a.K = K;
+    a.batch = (int)out_batch;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.a_bstride = (int64_t)N * K;
+    a.c_bstride = (int64_t)N * (N + 1);
+    a.nseg = n_pol;
+    a.seg_group = n_time_inner;
+    const int cfg = ctx->forced_cfg >= 0 ? ctx->forced_cfg
+                                         : (ftdx::direct_tile_to_cfg(ctx->direct_tile) >= 0
+                                                ? ftdx::direct_tile_to_cfg(ctx->direct_tile)
+                                                : ftdx::kDefaultCfg);
+    const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
+    hipError_t e = ftdx::launch_herk_int4_plain(a, cfg, output == FTDX_OUTPUT_FP16, remap,
+                                                static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_herk_pol_reduce");
+}
+
+int ftdx_pillbox_grid(const void* vis_tri, int vis_output, const float* baseline_uv_m,
+                      const double* freq_hz, float* uv_grid, int n_baselines, int Ng, int Nf,
+                      int freq_offset, float cell_size_rad, void* stream) {
+    if (n_baselines < 0 || Ng < 0 || Nf < 0 || freq_offset < 0)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_pillbox_grid: bad sizes");
+    if (vis_output != FTDX_OUTPUT_FP32 && vis_output != FTDX_OUTPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_pillbox_grid: visibilities must be FP32 or FP16");
+    if ((int64_t)n_baselines * Nf == 0 || Ng == 0) return FTDX_OK;
+    if (!vis_tri || !baseline_uv_m || !freq_hz || !uv_grid)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_pillbox_grid: NULL pointer");
+    hipError_t e = ftdx::launch_pillbox_grid(vis_tri, vis_output == FTDX_OUTPUT_FP16, baseline_uv_m, freq_hz,
+                                             uv_grid, n_baselines, Ng, Nf, freq_offset, cell_size_rad,
+                                             static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_pillbox_grid");
+}
+
+// ---- complex GEMM with prepared B ----
+namespace {
+
+int prepare_rows(ftdx_context* ctx, uint8_t*& buf, size_t& cap, const void* B_re, const void* B_im,
+                 int N, int K, int batch, int compute, void* stream, const char* what) {
+    const std::string w(what);
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, w + ": NULL context");
+    if (N < 0 || K < 0 || batch < 0) return fail(FTDX_ERR_INVALID_ARG, w + ": negative size");
+    if (compute != FTDX_COMPUTE_FP8)
+        return fail(FTDX_ERR_UNSUPPORTED,
+                    w + ": only ComputePrecision::FP8 is supported for the complex GEMM");
+    if (K > 0 && (int64_t)N * batch > 0 && (!B_re || !B_im))
+        return fail(FTDX_ERR_INVALID_ARG, w + ": NULL B pointer");
+    const int Kp = ftdx::precast_row_samples(K);
+    const size_t need = (size_t)N * batch * Kp * 2 + 16;
+    int rc = ftdx::ensure_scratch(buf, cap, need, what);
+    if (rc != FTDX_OK) return rc;
+    hipError_t e = ftdx::launch_promote_fp16_planar_rows(static_cast<const uint16_t*>(B_re),
+                                                        static_cast<const uint16_t*>(B_im), buf,
+                                                        (int64_t)N * batch, K, true,
+                                                        static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
+}
+
+int check_prepared(ftdx_context* ctx, int N, int K, int batch, const char* what) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, std::string(what) + ": NULL context");
+    if (!ctx->prep_b || ctx->prep_N != N || ctx->prep_K != K || ctx->prep_batch != batch)
+        return fail(FTDX_ERR_INVALID_ARG,
+                    std::string(what) + ": prepare_b() must have been called with matching N, K, batch");
+    return FTDX_OK;
+}
+
+}  // namespace
+
+int ftdx_prepare_b(ftdx_context* ctx, const void* B_re, const void* B_im, int N, int K, int batch_count,
+                   int compute, void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_prepare_b: NULL context");
+    int rc = prepare_rows(ctx, ctx->prep_b, ctx->prep_b_cap, B_re, B_im, N, K, batch_count, compute, stream,
+                          "ftdx_prepare_b");
+    if (rc != FTDX_OK) {
+        ctx->prep_N = ctx->prep_K = ctx->prep_batch = -1;
+        return rc;
+    }
+    ctx->prep_N = N;
+    ctx->prep_K = K;
+    ctx->prep_batch = batch_count;
+    return FTDX_OK;
+}
+
+int ftdx_gemm_prepared(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_re, void* C_im,
+                       int M, int N, int K, int batch_count, int output, float alpha, float beta,
+                       void* stream, int tune, int input) {
+    (void)tune;
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared");
+    if (rc != FTDX_OK) return rc;
+    if (output != FTDX_OUTPUT_FP32 && output != FTDX_OUTPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm_prepared: output must be FP32 or FP16");
+    if (input != FTDX_INPUT_FP16 && input != FTDX_INPUT_INT4)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm_prepared: input must be FP16 or INT4");
+    return ftdx::gemm_run(ctx, ctx->prep_b, input == FTDX_INPUT_INT4 ? ftdx::GEMM_A_INT4 : ftdx::GEMM_A_FP16_PLANAR,
+                          A_re, A_im, C_re, C_im, M, N, K, batch_count, ftdx::OUT_GEMM_PLANAR,
+                          output == FTDX_OUTPUT_FP16, alpha, beta, stream, "ftdx_gemm_prepared");
+}
+
+int ftdx_gemm_prepared_direct(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_re,
+                              void* C_im, int M, int N, int K, int batch_count, float alpha, float beta,
+                              void* stream) {
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_direct");
+    if (rc != FTDX_OK) return rc;
+    return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_FP16_PLANAR, A_re, A_im, C_re, C_im, M, N, K,
+                          batch_count, ftdx::OUT_GEMM_PLANAR, false, alpha, beta, stream,
+                          "ftdx_gemm_prepared_direct");
+}
+
+int ftdx_gemm_prepared_direct_int4(ftdx_context* ctx, const void* A_int4, void* C_re, void* C_im, int M,
+                                   int N, int K, int batch_count, float alpha, float beta, void* stream) {
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_direct_int4");
+    if (rc != FTDX_OK) return rc;
+    return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_INT4, A_int4, nullptr, C_re, C_im, M, N, K,
+                          batch_count, ftdx::OUT_GEMM_PLANAR, false, alpha, beta, stream,
+                          "ftdx_gemm_prepared_direct_int4");
+}
+
+int ftdx_gemm_prepared_power(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_power, int M,
+                             int N, int K, int batch_count, float alpha, float beta, void* stream) {
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power");
+    if (rc != FTDX_OK) return rc;
+    return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_FP16_PLANAR, A_re, A_im, C_power, nullptr, M, N,
+                          K, batch_count, ftdx::OUT_GEMM_POWER, false, alpha, beta, stream,
+                          "ftdx_gemm_prepared_power");
+}
+
+int ftdx_gemm_prepared_power_fp8(ftdx_context* ctx, const void* A_fp8_interleaved, void* C_power, int M,
+                                 int N, int K, int batch_count, float alpha, float beta, void* stream) {
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power_fp8");
+    if (rc != FTDX_OK) return rc;
+    return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_FP8_INTERLEAVED, A_fp8_interleaved, nullptr,
+                          C_power, nullptr, M, N, K, batch_count, ftdx::OUT_GEMM_POWER, false, alpha, beta,
+                          stream, "ftdx_gemm_prepared_power_fp8");
+}
+
+int ftdx_gemm_prepared_power_int4(ftdx_context* ctx, const void* A_int4, void* C_power, int M, int N,
+                                  int K, int batch_count, float alpha, float beta, void* stream) {
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power_int4");
+    if (rc != FTDX_OK) return rc;
+    return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_INT4, A_int4, nullptr, C_power, nullptr, M, N, K,
+                          batch_count, ftdx::OUT_GEMM_POWER, false, alpha, beta, stream,
+                          "ftdx_gemm_prepared_power_int4");
+}
+
+int ftdx_gemm(ftdx_context* ctx, const void* A_re, const void* A_im, const void* B_re, const void* B_im,
+              void* C_re, void* C_im, int M, int N, int K, int batch_count, int compute, int output,
+              float alpha, float beta, void* stream, int tune, int input) {
+    (void)tune;
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm: NULL context");
+    if (output != FTDX_OUTPUT_FP32 && output != FTDX_OUTPUT_FP16)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm: output must be FP32 or FP16");
+    if (input != FTDX_INPUT_FP16 && input != FTDX_INPUT_INT4)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm: input must be FP16 or INT4");
+    int rc = prepare_rows(ctx, ctx->gemm_b, ctx->gemm_b_cap, B_re, B_im, N, K, batch_count, compute, stream,
+                          "ftdx_gemm");
+    if (rc != FTDX_OK) return rc;
+    return ftdx::gemm_run(ctx, ctx->gemm_b, input == FTDX_INPUT_INT4 ? ftdx::GEMM_A_INT4 : ftdx::GEMM_A_FP16_PLANAR,
+                          A_re, A_im, C_re, C_im, M, N, K, batch_count, ftdx::OUT_GEMM_PLANAR,
+                          output == FTDX_OUTPUT_FP16, alpha, beta, stream, "ftdx_gemm");
 }
 
 int ftdx_interleave_tri(const void* re, const void* im, void* out, int64_t n, int prec, void* stream) {

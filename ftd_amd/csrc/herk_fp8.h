@@ -46,6 +46,12 @@ struct HerkPlainArgs {
     int64_t c_bstride;  // scalars of C per batch (N*(N+1))
     int precast = 0;    // 1: A is pre-cast FP8 (dword-planar), row pitch lda bytes
     int64_t lda = 0;
+    // Sum over K segments (QC input only): output batch b = ob * seg_group + ib sums
+    // the HERKs of input batches (ob * nseg + s) * seg_group + ib, s < nseg, as one
+    // product over nseg * K samples (polarization sum: nseg = n_pol, seg_group =
+    // n_time_inner).  Defaults: one input batch per output batch.
+    int nseg = 1;
+    int seg_group = 1;
 };
 
 // Lower-triangle tiles per problem for a configuration.

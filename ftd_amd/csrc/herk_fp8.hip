@@ -69,7 +69,12 @@ herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
     int I, J;
     tri_decode_r(t, Cfg::TM / Cfg::TN, I, J);
     TileJob j = {};
-    j.A = p.A + (int64_t)b * p.a_bstride;
+    // output batch b = (outer, inner) of seg_group inners; its K segments are the
+    // input batches (outer * nseg + s) * seg_group + inner, s = 0 .. nseg-1
+    const int grp = p.seg_group, ob = b / grp, ib = b - ob * grp;
+    j.A = p.A + ((int64_t)ob * p.nseg * grp + ib) * p.a_bstride;
+    j.nseg = p.nseg;
+    j.seg_stride = (int64_t)grp * p.a_bstride;
     j.C = static_cast<OutT*>(p.C) + (int64_t)b * p.c_bstride;
     j.N = p.N;
     j.K = p.K;
@@ -103,6 +108,7 @@ herk_int4_fp8_grouped_kernel(const GroupProblemDev* __restrict__ probs,
         j.alpha = pr.alpha;
         j.beta = pr.beta;
         j.lda = pr.K;
+        j.nseg = 1;
         if (pr.vec)
             TileRunner<Cfg>::template run<OutT, LOAD_VEC16, OUT_LOWER_PACKED, true>(j, lds);
         else
@@ -146,16 +152,18 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
 }
 
 // Complex GEMM C = alpha A B^T + beta C over pre-cast rows of A and conj(B): full
-// tile grid, row-major tile order within a batch, XCD-aware order when asked.
+// tile grid in column-major tile order within a batch (consecutive workgroups share
+// the column block of B, the large operand in the beamformer shapes, while the few
+// row blocks of A stay in L2), XCD-aware order when asked.
 template <class Cfg, typename OutT, int FMT>
 __global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
-gemm_fp8_kernel(GemmArgs p, int tiles_n, int tiles, int xcd_remap) {
+gemm_fp8_kernel(GemmArgs p, int tiles_m, int tiles, int xcd_remap) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
     const int total = tiles * p.batch;
     const int w = xcd_remap ? xcd_logical((int)blockIdx.x, total) : (int)blockIdx.x;
     const int b = w / tiles;
     const int t = w - b * tiles;
-    const int I = t / tiles_n, J = t - I * tiles_n;
+    const int J = t / tiles_m, I = t - J * tiles_m;
     TileJob j = {};
     j.A = p.A + (int64_t)b * p.a_bstride;
     j.B = p.B + (int64_t)b * p.b_bstride;
@@ -171,6 +179,7 @@ gemm_fp8_kernel(GemmArgs p, int tiles_n, int tiles, int xcd_remap) {
     j.lda = p.lda;
     j.ldb = p.ldb;
     j.ldc = p.ldc;
+    j.nseg = 1;
     TileRunner<Cfg>::template run<OutT, LOAD_FP8, FMT>(j, lds);
 }
 
@@ -180,15 +189,15 @@ hipError_t launch_gemm_cfg(const GemmArgs& a, int fmt, bool out_fp16, int xcd_re
     const int64_t blocks = tiles_m * tiles_n * a.batch;
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
     dim3 grid((unsigned)blocks), block(Cfg::THREADS);
-    const int tn = (int)tiles_n, tiles = (int)(tiles_m * tiles_n);
+    const int tm = (int)tiles_m, tiles = (int)(tiles_m * tiles_n);
     if (fmt == OUT_GEMM_POWER)
-        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_POWER>), grid, block, 0, stream, a, tn,
+        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_POWER>), grid, block, 0, stream, a, tm,
                            tiles, xcd_remap);
     else if (fmt == OUT_GEMM_PLANAR && out_fp16)
         hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, _Float16, OUT_GEMM_PLANAR>), grid, block, 0, stream, a,
-                           tn, tiles, xcd_remap);
+                           tm, tiles, xcd_remap);
     else if (fmt == OUT_GEMM_PLANAR)
-        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_PLANAR>), grid, block, 0, stream, a, tn,
+        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_PLANAR>), grid, block, 0, stream, a, tm,
                            tiles, xcd_remap);
     else
         return hipErrorInvalidValue;

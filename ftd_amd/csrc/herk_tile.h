@@ -72,6 +72,10 @@ struct TileJob {
     int row0, col0;    // first row / column of the tile
     float alpha, beta;
     int64_t lda;       // row pitch of A in bytes (LOAD_FP8 only)
+    // K segments (QC load modes): the product runs over nseg * K samples, segment s
+    // being the rows at A + s * seg_stride (row pitch K).  nseg = 1: one matrix.
+    int nseg;
+    int64_t seg_stride;
     // GEMM layouts only (C = alpha A B^T + beta C, full M x Ncol grid): the column
     // operand's pre-cast rows and pitch, the column count, the imaginary output
     // plane (planar layout) and the output row pitch in elements.
@@ -241,14 +245,32 @@ struct TileRunner {
             const int kk = k0 + seg * 16;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (!(is_col && share_cols) && grow < j.N) {
-                const uint8_t* src = j.A + (int64_t)grow * j.K + kk;
-                if (MODE == LOAD_VEC16) {
-                    if (kk < j.K) v = *reinterpret_cast<const uint4*>(src);
+                if (j.nseg <= 1) {
+                    const uint8_t* src = j.A + (int64_t)grow * j.K + kk;
+                    if (MODE == LOAD_VEC16) {
+                        if (kk < j.K) v = *reinterpret_cast<const uint4*>(src);
+                    } else {
+                        uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                        for (int e = 0; e < 16; ++e)
+                            if (kk + e < j.K) w[e >> 2] |= (uint32_t)src[e] << (8 * (e & 3));
+                        v = make_uint4(w[0], w[1], w[2], w[3]);
+                    }
+                } else if (MODE == LOAD_VEC16) {
+                    // K % 16 == 0 here, so a 16-byte segment lies inside one K segment
+                    const int ps = kk / j.K, kin = kk - ps * j.K;
+                    if (ps < j.nseg)
+                        v = *reinterpret_cast<const uint4*>(j.A + ps * j.seg_stride +
+                                                            (int64_t)grow * j.K + kin);
                 } else {
                     uint32_t w[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-                    for (int e = 0; e < 16; ++e)
-                        if (kk + e < j.K) w[e >> 2] |= (uint32_t)src[e] << (8 * (e & 3));
+                    for (int e = 0; e < 16; ++e) {
+                        const int kv = kk + e, ps = kv / j.K, kin = kv - ps * j.K;
+                        if (ps < j.nseg)
+                            w[e >> 2] |= (uint32_t)j.A[ps * j.seg_stride + (int64_t)grow * j.K + kin]
+                                         << (8 * (e & 3));
+                    }
                     v = make_uint4(w[0], w[1], w[2], w[3]);
                 }
             }
@@ -412,7 +434,8 @@ struct TileRunner {
                     acc_im[a][b][r] = 0.0f;
                 }
 
-        const int nst = (j.K + Cfg::KC - 1) / Cfg::KC;
+        const int Kv = j.nseg > 1 ? j.K * j.nseg : j.K;  // samples the product runs over
+        const int nst = (Kv + Cfg::KC - 1) / Cfg::KC;
         constexpr int NG = MODE == LOAD_FP8 ? Cfg::SEGS8_PER_THREAD : Cfg::SEGS_PER_THREAD;
         auto load = [&](int k0, uint4 (&g)[NG]) {
             if constexpr (MODE == LOAD_FP8) load_stage_fp8<GEMM>(j, k0, share_cols, tid, g);

@@ -31,6 +31,12 @@ hipError_t launch_ggp_to_fp8_rows(const uint8_t* in, uint8_t* out, int N, int K,
 // twos: two's complement nibbles, low = Re; else QC sign-magnitude, high = Re.
 hipError_t launch_ftd_to_fp8_rows(const uint8_t* in, uint8_t* out, int n_ant, int n_chan, int n_time,
                                   int n_time_inner, int n_pol, bool twos, hipStream_t s);
+// Planar FP16 [rows][K] Re and Im -> pre-cast rows; conj: negate Im (FP8 sign bit).
+hipError_t launch_promote_fp16_planar_rows(const uint16_t* re, const uint16_t* im, uint8_t* out,
+                                           int64_t rows, int K, bool conj, hipStream_t s);
+// FP8 E4M3 [rows][K][2] interleaved pairs -> pre-cast rows.
+hipError_t launch_promote_fp8_interleaved_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K,
+                                               hipStream_t s);
 hipError_t launch_interleave_tri(const uint16_t* re, const uint16_t* im, void* out, int64_t n,
                                  bool fp64, hipStream_t s);
 

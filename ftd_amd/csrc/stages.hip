@@ -17,6 +17,9 @@
 //                                                                 (shared/pack_kernels_common.hpp:602-659)
 //   tri_from_herm                N x N complex -> packed upper row-major  (include/kernels/tri_from_herm.cuh:28-73)
 //   interleave_tri               FP16 Re/Im planes -> [re, im] FP32/FP64  (include/kernels/interleave_tri.cuh:27-42)
+//   promote_fp16_planar_rows     FP16 Re / Im planes -> pre-cast rows, optionally conjugated
+//                                (prepare_b's conj(B), cutlass_gemm_api.h:236-255)
+//   promote_fp8_interleaved_rows FP8 [re, im] pairs -> pre-cast rows (gemm_prepared_power_fp8's A)
 //   ggp_to_fp8_rows              GGP XEngine input [batch][K][N] (two's complement, low
 //                                nibble = Re; lib/xengine.cpp:150-160, promote_data.cuh:80-81)
 //                                -> pre-cast operand rows [batch][N][Kp], transposed
@@ -164,6 +167,60 @@ promote_fp16_fp8_planar_kernel(const uint16_t* __restrict__ in, uint8_t* __restr
                 im |= fp16_to_e4m3_clamped(p[1]) << (8 * e);
             }
         }
+        uint2* dst = reinterpret_cast<uint2*>(out + r * (int64_t)Kp * 2 + (int64_t)gi * 8);
+        *dst = make_uint2(re, im);
+    }
+}
+
+// Planar FP16 Re / Im [rows][K] -> pre-cast rows; conj negates Im (the prepared B
+// operand of the GEMM holds conj(B)).  4 samples per thread.
+__global__ void __launch_bounds__(ST_THREADS)
+promote_fp16_planar_rows_kernel(const uint16_t* __restrict__ re_in, const uint16_t* __restrict__ im_in,
+                                uint8_t* __restrict__ out, int64_t rows, int K, int Kp, uint32_t im_sign) {
+    const int groups = Kp / 4;
+    const int64_t total = rows * groups;
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride) {
+        const int64_t r = t / groups;
+        const int gi = (int)(t - r * groups);
+        uint32_t re = 0u, im = 0u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k = gi * 4 + e;
+            if (k < K) {
+                re |= fp16_to_e4m3_clamped(re_in[r * K + k]) << (8 * e);
+                im |= (fp16_to_e4m3_clamped(im_in[r * K + k]) ^ im_sign) << (8 * e);
+            }
+        }
+        uint2* dst = reinterpret_cast<uint2*>(out + r * (int64_t)Kp * 2 + (int64_t)gi * 8);
+        *dst = make_uint2(re, im);
+    }
+}
+
+// FP8 E4M3 interleaved [re, im] byte pairs [rows][K][2] -> pre-cast rows: per 4
+// samples, bytes re0 im0 re1 im1 | re2 im2 re3 im3 -> re0..re3 | im0..im3.
+__global__ void __launch_bounds__(ST_THREADS)
+promote_fp8_interleaved_rows_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                    int64_t rows, int K, int Kp) {
+    const int groups = Kp / 4;
+    const int64_t total = rows * groups;
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride) {
+        const int64_t r = t / groups;
+        const int gi = (int)(t - r * groups);
+        const int k0 = gi * 4;
+        const uint8_t* src = in + (r * K + k0) * 2;
+        uint32_t lo = 0u, hi = 0u;
+        if (k0 + 4 <= K && ((reinterpret_cast<uintptr_t>(src) & 7u) == 0)) {
+            const uint2 v = *reinterpret_cast<const uint2*>(src);
+            lo = v.x;
+            hi = v.y;
+        } else {
+            for (int e = 0; e < 8; ++e)
+                if (k0 + e / 2 < K) (e < 4 ? lo : hi) |= (uint32_t)src[e] << (8 * (e & 3));
+        }
+        const uint32_t re = __builtin_amdgcn_perm(hi, lo, 0x06040200u);  // bytes 0, 2, 4, 6
+        const uint32_t im = __builtin_amdgcn_perm(hi, lo, 0x07050301u);  // bytes 1, 3, 5, 7
         uint2* dst = reinterpret_cast<uint2*>(out + r * (int64_t)Kp * 2 + (int64_t)gi * 8);
         *dst = make_uint2(re, im);
     }
@@ -497,6 +554,24 @@ hipError_t launch_promote_fp16_fp8_planar(const uint16_t* in, uint8_t* out, int6
     if (rows <= 0 || K <= 0) return hipSuccess;
     const int Kp = precast_row_samples(K);
     hipLaunchKernelGGL(promote_fp16_fp8_planar_kernel, dim3(grid_for(rows * (Kp / 4))),
+                       dim3(ST_THREADS), 0, s, in, out, rows, K, Kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_promote_fp16_planar_rows(const uint16_t* re, const uint16_t* im, uint8_t* out,
+                                           int64_t rows, int K, bool conj, hipStream_t s) {
+    if (rows <= 0 || K <= 0) return hipSuccess;
+    const int Kp = precast_row_samples(K);
+    hipLaunchKernelGGL(promote_fp16_planar_rows_kernel, dim3(grid_for(rows * (Kp / 4))),
+                       dim3(ST_THREADS), 0, s, re, im, out, rows, K, Kp, conj ? 0x80u : 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_promote_fp8_interleaved_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K,
+                                               hipStream_t s) {
+    if (rows <= 0 || K <= 0) return hipSuccess;
+    const int Kp = precast_row_samples(K);
+    hipLaunchKernelGGL(promote_fp8_interleaved_rows_kernel, dim3(grid_for(rows * (Kp / 4))),
                        dim3(ST_THREADS), 0, s, in, out, rows, K, Kp);
     return hipGetLastError();
 }

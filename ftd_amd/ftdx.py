@@ -130,6 +130,17 @@ _SIGNATURES = {
     "ftdx_xengine_set_launch_config": (_I, [_VP, _I, _I]),
     "ftdx_xengine_destroy": (_I, [_VP]),
     "ftdx_xengine_chpc": (_I, [_VP, _VP, _VP]),
+    "ftdx_prepare_b": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _VP]),
+    "ftdx_gemm_prepared": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _F, _F, _VP, _I, _I]),
+    "ftdx_gemm_prepared_direct": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_gemm_prepared_direct_int4": (_I, [_VP, _VP, _VP, _VP, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_gemm_prepared_power": (_I, [_VP, _VP, _VP, _VP, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_gemm_prepared_power_fp8": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_gemm_prepared_power_int4": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_gemm": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _F, _F, _VP, _I, _I]),
+    "ftdx_pol_reduce_triangle": (_I, [_VP, _VP, _I, _I, _I, _I, _I, _VP]),
+    "ftdx_herk_pol_reduce": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_pillbox_grid": (_I, [_VP, _I, _VP, _VP, _VP, _I, _I, _I, _I, _F, _VP]),
     "ftdx_pipeline_create": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(_VP)]),
     "ftdx_pipeline_query": (_I, [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "ftdx_pipeline_correlate": (_I, [_VP, _VP, _VP]),
@@ -293,6 +304,65 @@ class CutlassComplexGemm:
     @staticmethod
     def print_build_info() -> None:
         lib().ftdx_print_build_info()
+
+    # ---- complex GEMM with prepared B (cutlass_gemm_api.h:215-470) ----
+    def _gemm_rc(self, rc: int, what: str) -> int:
+        if rc in (FTDX_ERR_UNSUPPORTED, FTDX_ERR_HIP, FTDX_ERR_NO_MEMORY):
+            _check(rc, what)
+        return rc
+
+    def prepare_b(self, B_re, B_im, N: int, K: int, batch_count: int,
+                  compute: ComputePrecision = ComputePrecision.FP8, stream=None) -> None:
+        _check(lib().ftdx_prepare_b(self._h, _ptr(B_re), _ptr(B_im), N, K, batch_count,
+                                    int(compute), _stream_ptr(stream)), "prepare_b")
+
+    def gemm_prepared(self, A_re, A_im, C_re, C_im, M: int, N: int, K: int, batch_count: int,
+                      output: OutputPrecision = OutputPrecision.FP32, alpha: float = 1.0,
+                      beta: float = 0.0, stream=None, tune: bool = True,
+                      input: InputPrecision = InputPrecision.FP16) -> int:
+        return self._gemm_rc(lib().ftdx_gemm_prepared(
+            self._h, _ptr(A_re), _ptr(A_im), _ptr(C_re), _ptr(C_im), M, N, K, batch_count,
+            int(output), alpha, beta, _stream_ptr(stream), int(bool(tune)), int(input)),
+            "gemm_prepared")
+
+    def gemm_prepared_direct(self, A_re, A_im, C_re, C_im, M, N, K, batch_count, alpha=1.0,
+                             beta=0.0, stream=None) -> int:
+        return self._gemm_rc(lib().ftdx_gemm_prepared_direct(
+            self._h, _ptr(A_re), _ptr(A_im), _ptr(C_re), _ptr(C_im), M, N, K, batch_count,
+            alpha, beta, _stream_ptr(stream)), "gemm_prepared_direct")
+
+    def gemm_prepared_direct_int4(self, A_int4, C_re, C_im, M, N, K, batch_count, alpha=1.0,
+                                  beta=0.0, stream=None) -> int:
+        return self._gemm_rc(lib().ftdx_gemm_prepared_direct_int4(
+            self._h, _ptr(A_int4), _ptr(C_re), _ptr(C_im), M, N, K, batch_count, alpha, beta,
+            _stream_ptr(stream)), "gemm_prepared_direct_int4")
+
+    def gemm_prepared_power(self, A_re, A_im, C_power, M, N, K, batch_count, alpha=1.0, beta=0.0,
+                            stream=None) -> int:
+        return self._gemm_rc(lib().ftdx_gemm_prepared_power(
+            self._h, _ptr(A_re), _ptr(A_im), _ptr(C_power), M, N, K, batch_count, alpha, beta,
+            _stream_ptr(stream)), "gemm_prepared_power")
+
+    def gemm_prepared_power_fp8(self, A_fp8, C_power, M, N, K, batch_count, alpha=1.0, beta=0.0,
+                                stream=None) -> int:
+        return self._gemm_rc(lib().ftdx_gemm_prepared_power_fp8(
+            self._h, _ptr(A_fp8), _ptr(C_power), M, N, K, batch_count, alpha, beta,
+            _stream_ptr(stream)), "gemm_prepared_power_fp8")
+
+    def gemm_prepared_power_int4(self, A_int4, C_power, M, N, K, batch_count, alpha=1.0,
+                                 beta=0.0, stream=None) -> int:
+        return self._gemm_rc(lib().ftdx_gemm_prepared_power_int4(
+            self._h, _ptr(A_int4), _ptr(C_power), M, N, K, batch_count, alpha, beta,
+            _stream_ptr(stream)), "gemm_prepared_power_int4")
+
+    def gemm(self, A_re, A_im, B_re, B_im, C_re, C_im, M, N, K, batch_count,
+             compute: ComputePrecision = ComputePrecision.FP8,
+             output: OutputPrecision = OutputPrecision.FP16, alpha=1.0, beta=0.0, stream=None,
+             tune: bool = True, input: InputPrecision = InputPrecision.FP16) -> int:
+        return self._gemm_rc(lib().ftdx_gemm(
+            self._h, _ptr(A_re), _ptr(A_im), _ptr(B_re), _ptr(B_im), _ptr(C_re), _ptr(C_im), M, N,
+            K, batch_count, int(compute), int(output), alpha, beta, _stream_ptr(stream),
+            int(bool(tune)), int(input)), "gemm")
 
     # ---- gfx950 launch configuration and tuning cache ----
     def set_launch_config(self, tile_config: int = -1, xcd_remap: int = -1) -> None:
@@ -738,3 +808,40 @@ class Pipeline:
             self.close()
         except Exception:
             pass
+
+
+# ---- visibility consumers: polarization sum and pillbox gridding ----
+def pol_reduce_triangle(tri, N: int, n_ch: int, n_pol: int, n_time_inner: int, out=None, stream=None):
+    """[n_ch*n_pol*n_ti, N(N+1)/2, 2] triangles -> [n_ch*n_ti, N(N+1)/2, 2] summed over pol."""
+    import torch
+    if out is None:
+        out = torch.empty((n_ch * n_time_inner, packed_len(N), 2), dtype=tri.dtype, device=tri.device)
+    o = int(OutputPrecision.FP16 if tri.dtype == torch.float16 else OutputPrecision.FP32)
+    _check(lib().ftdx_pol_reduce_triangle(_ptr(tri), _ptr(out), N, n_ch, n_pol, n_time_inner, o,
+                                          _stream_ptr(stream)), "pol_reduce_triangle")
+    return out
+
+
+def herk_pol_reduce(ctx: "CutlassComplexGemm", A, N: int, K: int, n_ch: int, n_pol: int,
+                    n_time_inner: int, output: str = "fp32", C=None, alpha: float = 1.0,
+                    beta: float = 0.0, stream=None):
+    """Fused HERK + polarization sum (ftdx_herk_pol_reduce)."""
+    import torch
+    dt = torch.float32 if output == "fp32" else torch.float16
+    if C is None:
+        C = torch.empty((n_ch * n_time_inner, packed_len(N), 2), dtype=dt, device=A.device)
+    o = int(OutputPrecision.FP32 if output == "fp32" else OutputPrecision.FP16)
+    _check(lib().ftdx_herk_pol_reduce(ctx.handle, _ptr(A), _ptr(C), N, K, n_ch, n_pol, n_time_inner, o,
+                                      alpha, beta, _stream_ptr(stream)), "herk_pol_reduce")
+    return C
+
+
+def pillbox_grid(vis_tri, baseline_uv_m, freq_hz, grid, n_baselines: int, Ng: int, Nf: int,
+                 cell_size_rad: float, freq_offset: int = 0, stream=None):
+    """Scatter-add Nf packed triangles into grid [Nf, Ng, Ng, 2] float32 (ftdx_pillbox_grid)."""
+    import torch
+    o = int(OutputPrecision.FP16 if vis_tri.dtype == torch.float16 else OutputPrecision.FP32)
+    _check(lib().ftdx_pillbox_grid(_ptr(vis_tri), o, _ptr(baseline_uv_m), _ptr(freq_hz), _ptr(grid),
+                                   n_baselines, Ng, Nf, freq_offset, cell_size_rad, _stream_ptr(stream)),
+           "pillbox_grid")
+    return grid

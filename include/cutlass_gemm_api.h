@@ -86,6 +86,40 @@ public:
              hipStream_t stream = nullptr,
              bool tune = true);
 
+    // ---- complex GEMM C = alpha A B^T + beta C (cutlass_gemm_api.h:215-470) ----
+    // FP16 planar operands are passed as `const __half*` by reference callers; any
+    // pointer converts to the `const void*` parameters here.
+
+    /// One call, B converted each time (gemm, :215-225).  FP8 compute only.
+    int gemm(const void* A_re, const void* A_im, const void* B_re, const void* B_im, void* C_re,
+             void* C_im, int M, int N, int K, int batch_count,
+             ComputePrecision compute = ComputePrecision::FP8,
+             OutputPrecision output = OutputPrecision::FP16, float alpha = 1.0f, float beta = 0.0f,
+             hipStream_t stream = nullptr, bool tune = true,
+             InputPrecision input = InputPrecision::FP16);
+    /// Converts B once (prepare_b, :251-255); throws for FP6 / FP4.
+    void prepare_b(const void* B_re, const void* B_im, int N, int K, int batch_count,
+                   ComputePrecision compute, hipStream_t stream = nullptr);
+    int gemm_prepared(const void* A_re, const void* A_im, void* C_re, void* C_im, int M, int N,
+                      int K, int batch_count, OutputPrecision output = OutputPrecision::FP32,
+                      float alpha = 1.0f, float beta = 0.0f, hipStream_t stream = nullptr,
+                      bool tune = true, InputPrecision input = InputPrecision::FP16);
+    int gemm_prepared_direct(const void* A_re, const void* A_im, float* C_re, float* C_im, int M,
+                             int N, int K, int batch_count, float alpha = 1.0f, float beta = 0.0f,
+                             hipStream_t stream = nullptr);
+    int gemm_prepared_power(const void* A_re, const void* A_im, float* C_power, int M, int N, int K,
+                            int batch_count, float alpha = 1.0f, float beta = 0.0f,
+                            hipStream_t stream = nullptr);
+    int gemm_prepared_power_fp8(const void* A_fp8_interleaved, float* C_power, int M, int N, int K,
+                                int batch_count, float alpha = 1.0f, float beta = 0.0f,
+                                hipStream_t stream = nullptr);
+    int gemm_prepared_power_int4(const void* A_int4, float* C_power, int M, int N, int K,
+                                 int batch_count, float alpha = 1.0f, float beta = 0.0f,
+                                 hipStream_t stream = nullptr);
+    int gemm_prepared_direct_int4(const void* A_int4, float* C_re, float* C_im, int M, int N, int K,
+                                  int batch_count, float alpha = 1.0f, float beta = 0.0f,
+                                  hipStream_t stream = nullptr);
+
     static void print_build_info();
 
     /// Underlying C-ABI context (for callers mixing the C and C++ interfaces).

@@ -324,6 +324,69 @@ int ftdx_xengine_destroy(ftdx_xengine* xe);
  * stream, destroy. */
 int ftdx_xengine_chpc(void* output, const void* input, const ftdx_xengine_param* param);
 
+/* ---- consumers of the visibility triangle ----
+ * Polarization sum: input triangles in the HERK batch order [n_ch][n_pol][n_ti]
+ * (packed lower, N(N+1) scalars each), output [n_ch][n_ti] = sum over pol, FP32 or
+ * FP16 (output = FTDX_OUTPUT_*).  Replaces pol_reduce_triangle_kernel[_fp16]
+ * (lib/bench_pipeline.cu:254-302, n_pol = 2 there). */
+int ftdx_pol_reduce_triangle(const void* tri_in, void* tri_out, int N, int n_ch, int n_pol, int n_time_inner,
+                             int output, void* stream);
+/* HERK and polarization sum in one pass: A = QC bytes [n_ch][n_pol][n_ti][N][K], C =
+ * [n_ch][n_ti] packed lower triangles of sum_pol X X^H (= the HERK over the n_pol*K
+ * samples of both polarizations), with the HERK's alpha / beta / Im(diag) = 0 rules.
+ * Equal to ftdx_herk followed by ftdx_pol_reduce_triangle for integer voltages (exact
+ * sums), without writing the per-polarization triangles. */
+int ftdx_herk_pol_reduce(ftdx_context* ctx, const void* A, void* C, int N, int K, int n_ch, int n_pol,
+                         int n_time_inner, int output, float alpha, float beta, void* stream);
+/* Pillbox UV gridding (PillboxGridScatter, include/kernels/visibility_bf.cuh:259-370;
+ * pillbox_grid_scatter_fp32_fp32grid_kernel, include/kernels/imaging_kernels.cuh:219-270):
+ * for frequency f < Nf and baseline k (packed lower index, vis_tri [Nf][n_baselines][2]
+ * FP32 or FP16 per vis_output), UV in wavelengths = baseline_uv_m[k] * freq_hz[freq_offset
+ * + f] / c; cell (roundf(u / cell + Ng/2), roundf(v / cell + Ng/2)); the visibility is
+ * added to uv_grid [Nf][Ng][Ng][2] (FP32) there, and its conjugate at (-u, -v) for
+ * cross-correlations.  Accumulates: the caller zeroes the grid. */
+int ftdx_pillbox_grid(const void* vis_tri, int vis_output, const float* baseline_uv_m,
+                      const double* freq_hz, float* uv_grid, int n_baselines, int Ng, int Nf,
+                      int freq_offset, float cell_size_rad, void* stream);
+
+/* ---- complex GEMM with prepared B (the voltage beamformer's product) ----
+ * C = alpha * A B^T + beta * C per batch, A [batch][M][K], B [batch][N][K], C
+ * [batch][M][N] (cutlass_gemm_api.h:215-470; CPU form cpu_gemm, test_gemm_api.cu:50-67).
+ * FP8 E4M3 compute (FP16 inputs clamped to +-448, nearest-even), FP32 accumulate.
+ * Planar layouts: FP16 A_re / A_im, FP16 B_re / B_im, C_re / C_im FP32 (or FP16 via
+ * ftdx_gemm_prepared's output).  INT4 A: QC bytes [batch][M][K] in A_re (A_im unused).
+ * Power: C_power = alpha * (Re^2 + Im^2) + beta * C_power, FP32 [batch][M][N].
+ * ComputePrecision FP6 / FP4 return FTDX_ERR_UNSUPPORTED. */
+/* CutlassComplexGemm::prepare_b (cutlass_gemm_api.h:251-255): converts B once. */
+int ftdx_prepare_b(ftdx_context* ctx, const void* B_re, const void* B_im, int N, int K, int batch_count,
+                   int compute, void* stream);
+/* gemm_prepared (cutlass_gemm_api.h:290-299); input FTDX_INPUT_FP16 / _INT4,
+ * output FTDX_OUTPUT_FP32 / _FP16. */
+int ftdx_gemm_prepared(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_re, void* C_im,
+                       int M, int N, int K, int batch_count, int output, float alpha, float beta,
+                       void* stream, int tune, int input);
+/* gemm_prepared_direct (cutlass_gemm_api.h:321-326): FP16 A, FP32 planar C. */
+int ftdx_gemm_prepared_direct(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_re,
+                              void* C_im, int M, int N, int K, int batch_count, float alpha, float beta,
+                              void* stream);
+/* gemm_prepared_direct_int4 (cutlass_gemm_api.h:419-424). */
+int ftdx_gemm_prepared_direct_int4(ftdx_context* ctx, const void* A_int4, void* C_re, void* C_im, int M,
+                                   int N, int K, int batch_count, float alpha, float beta, void* stream);
+/* gemm_prepared_power (cutlass_gemm_api.h:348-353). */
+int ftdx_gemm_prepared_power(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_power, int M,
+                             int N, int K, int batch_count, float alpha, float beta, void* stream);
+/* gemm_prepared_power_fp8 (cutlass_gemm_api.h:376-381): A as FP8 [re, im] byte pairs. */
+int ftdx_gemm_prepared_power_fp8(ftdx_context* ctx, const void* A_fp8_interleaved, void* C_power, int M,
+                                 int N, int K, int batch_count, float alpha, float beta, void* stream);
+/* gemm_prepared_power_int4 (cutlass_gemm_api.h:398-401). */
+int ftdx_gemm_prepared_power_int4(ftdx_context* ctx, const void* A_int4, void* C_power, int M, int N,
+                                  int K, int batch_count, float alpha, float beta, void* stream);
+/* gemm (cutlass_gemm_api.h:215-225): one call with B converted each time (does not
+ * touch the prepared B). */
+int ftdx_gemm(ftdx_context* ctx, const void* A_re, const void* A_im, const void* B_re, const void* B_im,
+              void* C_re, void* C_im, int M, int N, int K, int batch_count, int compute, int output,
+              float alpha, float beta, void* stream, int tune, int input);
+
 /* ---- streaming correlation of host payloads ----
  * VisibilityPipeline::correlate / correlate_pipelined (lib/visibility_pipeline.cu:253-300,
  * 484-545): each payload is QC bytes [batch][N][K] in host memory (K = n_time /

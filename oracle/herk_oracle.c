@@ -19,6 +19,9 @@
  *   triangulateFromHerm   include/kernels/tri_from_herm.cuh:28-73
  *   verifyXEngine GEMM    tests/xengine_test.cpp:213-321 (M^H M, TRI / HERM)
  *   cpu_herk_ftd          tests/visibility_pipeline_test.cpp:54-96 (int64)
+ *   cpu_gemm              cutlass_interface/test_gemm_api.cu:50-67 (complex GEMM, B [N][K])
+ *   power_reduce          cutlass_interface/cutlass_gemm_api.cu:425-437 (float formula)
+ *   PillboxGridScatter    include/kernels/visibility_bf.cuh:259-370 (UV cells, conjugate)
  * Parallelism follows the reference's OpenMP "parallel for schedule(dynamic)"
  * over (batch, row) (test_herk_int4.cu:56, :323).
  *
@@ -660,4 +663,85 @@ void ftdo_herk_ftd_i64(const uint8_t* raw, int n_ant, int n_chan, int n_time, in
     (void)threads;
     free(re);
     free(im);
+}
+
+/* ------------------------------------------------------------------ */
+/* Complex GEMM (cpu_gemm, cutlass_interface/test_gemm_api.cu:50-67): per batch
+ * C[i][j] = sum_k A[i][k] B[j][k] (B stored [N][K], no conjugate), FP64, complex
+ * interleaved.  Exact for integer inputs while sums stay below 2^53. */
+void ftdo_gemm_c64(const double* A, const double* B, double* C, int M, int N, int K, int batch,
+                   int threads) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    const int64_t total = (int64_t)batch * M;
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+#endif
+    for (int64_t t = 0; t < total; ++t) {
+        const int64_t b = t / M, i = t % M;
+        const double* a = A + ((b * M + i) * (int64_t)K) * 2;
+        for (int j = 0; j < N; ++j) {
+            const double* bb = B + ((b * N + j) * (int64_t)K) * 2;
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < K; ++k) {
+                sr += a[2 * k] * bb[2 * k] - a[2 * k + 1] * bb[2 * k + 1];
+                si += a[2 * k] * bb[2 * k + 1] + a[2 * k + 1] * bb[2 * k];
+            }
+            C[((b * M + i) * (int64_t)N + j) * 2] = sr;
+            C[((b * M + i) * (int64_t)N + j) * 2 + 1] = si;
+        }
+    }
+    (void)threads;
+}
+
+/* Fused power detection of the GEMM epilogue, in the float operations the library
+ * evaluates: p = alpha * fmaf(re, re, im * im); with beta != 0, p = fmaf(beta, old, p)
+ * (the reference's alpha * (re^2 + im^2) + beta * old, cutlass_gemm_api.cu:425-437,
+ * with its contraction fixed). */
+void ftdo_power_f32(const float* re, const float* im, const float* old, int64_t n, float alpha,
+                    float beta, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        float p = alpha * fmaf(re[i], re[i], im[i] * im[i]);
+        if (beta != 0.0f) p = fmaf(beta, old[i], p);
+        out[i] = p;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Pillbox UV gridding (PillboxGridScatter, include/kernels/visibility_bf.cuh:259-370;
+ * uv_cell, include/kernels/imaging_kernels.cuh:33-45): sequential scatter-add of
+ * Nf packed triangles vis [Nf][n_bl][2] into grid [Nf][Ng][Ng][2]; the conjugate goes to
+ * (-u, -v) for cross-correlations.  The baseline index is decoded exactly. */
+void ftdo_pillbox_grid(const float* vis, const float* uv_m, const double* freq_hz, float* grid,
+                       int64_t n_bl, int Ng, int Nf, int freq_offset, float cell) {
+    const double C_LIGHT = 299792458.0;
+    const float half_ng = 0.5f * (float)Ng;
+    const int64_t plane = (int64_t)Ng * Ng;
+    for (int f = 0; f < Nf; ++f) {
+        const double lambda = C_LIGHT / freq_hz[freq_offset + f];
+        int64_t k = 0;
+        for (int row = 0; k < n_bl; ++row)
+            for (int col = 0; col <= row && k < n_bl; ++col, ++k) {
+                const float vre = vis[2 * ((int64_t)f * n_bl + k)];
+                const float vim = vis[2 * ((int64_t)f * n_bl + k) + 1];
+                const float u_lam = (float)((double)uv_m[2 * k] / lambda);
+                const float v_lam = (float)((double)uv_m[2 * k + 1] / lambda);
+                const int iu = (int)roundf(u_lam / cell + half_ng);
+                const int iv = (int)roundf(v_lam / cell + half_ng);
+                if (iu >= 0 && iu < Ng && iv >= 0 && iv < Ng) {
+                    float* g = grid + 2 * ((int64_t)f * plane + (int64_t)iv * Ng + iu);
+                    g[0] += vre;
+                    g[1] += vim;
+                }
+                if (row != col) {
+                    const int iuc = (int)roundf(-u_lam / cell + half_ng);
+                    const int ivc = (int)roundf(-v_lam / cell + half_ng);
+                    if (iuc >= 0 && iuc < Ng && ivc >= 0 && ivc < Ng) {
+                        float* g = grid + 2 * ((int64_t)f * plane + (int64_t)ivc * Ng + iuc);
+                        g[0] += vre;
+                        g[1] += -vim;
+                    }
+                }
+            }
+    }
 }

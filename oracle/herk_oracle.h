@@ -112,6 +112,17 @@ void ftdo_xengine_ggp_i64(const uint8_t* in, int N, int K, int batch, int herm, 
 void ftdo_herk_ftd_i64(const uint8_t* raw, int n_ant, int n_chan, int n_time, int n_ti, int n_pol,
                        int twos, int64_t* out, int threads);
 
+/* Complex GEMM sum_k A[i][k] B[j][k] (cpu_gemm), interleaved FP64. */
+void ftdo_gemm_c64(const double* A, const double* B, double* C, int M, int N, int K, int batch,
+                   int threads);
+/* The GEMM epilogue's power detection in float arithmetic. */
+void ftdo_power_f32(const float* re, const float* im, const float* old, int64_t n, float alpha,
+                    float beta, float* out);
+
+/* Pillbox UV gridding of Nf packed FP32 triangles into an FP32 grid (accumulates). */
+void ftdo_pillbox_grid(const float* vis, const float* uv_m, const double* freq_hz, float* grid,
+                       int64_t n_bl, int Ng, int Nf, int freq_offset, float cell);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,15 @@ def lib():
         L.ftdo_herk_ftd_i64.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, i64p, ctypes.c_int]
         L.ftdo_herk_ftd_i64.restype = None
+        L.ftdo_gemm_c64.argtypes = [dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int]
+        L.ftdo_gemm_c64.restype = None
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.ftdo_power_f32.argtypes = [fp, fp, fp, ctypes.c_int64, ctypes.c_float, ctypes.c_float, fp]
+        L.ftdo_power_f32.restype = None
+        L.ftdo_pillbox_grid.argtypes = [fp, fp, dp, fp, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_float]
+        L.ftdo_pillbox_grid.restype = None
         L.ftdo_herk_c64.restype = None
         L.ftdo_herk_c64.argtypes = [ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
@@ -336,3 +345,51 @@ def herk_ftd_i64(raw: np.ndarray, n_ant: int, n_chan: int, n_time: int, n_ti: in
     lib().ftdo_herk_ftd_i64(_ptr(d, ctypes.c_uint8), n_ant, n_chan, n_time, n_ti, n_pol, int(twos),
                             _ptr(out, ctypes.c_int64), threads)
     return out
+
+
+
+# ---- complex GEMM (cpu_gemm) and the power epilogue ----
+def gemm_c64(A: np.ndarray, B: np.ndarray, threads: int = 0) -> np.ndarray:
+    """A complex [batch, M, K], B complex [batch, N, K] -> C = A B^T [batch, M, N] (complex128)."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    B = np.ascontiguousarray(B, dtype=np.complex128)
+    bt, M, K = A.shape
+    N = B.shape[1]
+    C = np.zeros((bt, M, N), dtype=np.complex128)
+    lib().ftdo_gemm_c64(_ptr(A.view(np.float64), ctypes.c_double), _ptr(B.view(np.float64), ctypes.c_double),
+                        _ptr(C.view(np.float64), ctypes.c_double), M, N, K, bt, threads)
+    return C
+
+
+def power_f32(re: np.ndarray, im: np.ndarray, alpha: float = 1.0, beta: float = 0.0,
+              old: np.ndarray | None = None) -> np.ndarray:
+    re = np.ascontiguousarray(re, dtype=np.float32).reshape(-1)
+    im = np.ascontiguousarray(im, dtype=np.float32).reshape(-1)
+    o = np.zeros_like(re) if old is None else np.ascontiguousarray(old, dtype=np.float32).reshape(-1)
+    out = np.empty_like(re)
+    fp = ctypes.c_float
+    lib().ftdo_power_f32(_ptr(re, fp), _ptr(im, fp), _ptr(o, fp), re.size, alpha, beta, _ptr(out, fp))
+    return out
+
+
+
+# ---- visibility consumers ----
+def pol_reduce(tri: np.ndarray, n_ch: int, n_pol: int, n_ti: int) -> np.ndarray:
+    """[n_ch*n_pol*n_ti, P, 2] -> [n_ch*n_ti, P, 2]: sum over pol in float64, one rounding
+    to the input dtype (pol_reduce_triangle_kernel, lib/bench_pipeline.cu:254-302)."""
+    t = np.asarray(tri)
+    P = t.shape[1]
+    s = t.astype(np.float64).reshape(n_ch, n_pol, n_ti, P, 2).sum(axis=1)
+    return s.reshape(n_ch * n_ti, P, 2).astype(t.dtype)
+
+
+def pillbox_grid(vis: np.ndarray, uv_m: np.ndarray, freq_hz: np.ndarray, n_bl: int, Ng: int, Nf: int,
+                 cell: float, freq_offset: int = 0, grid: np.ndarray | None = None) -> np.ndarray:
+    v = np.ascontiguousarray(vis, dtype=np.float32).reshape(-1)
+    u = np.ascontiguousarray(uv_m, dtype=np.float32).reshape(-1)
+    fr = np.ascontiguousarray(freq_hz, dtype=np.float64).reshape(-1)
+    g = np.zeros((Nf, Ng, Ng, 2), dtype=np.float32) if grid is None else grid
+    fp = ctypes.c_float
+    lib().ftdo_pillbox_grid(_ptr(v, fp), _ptr(u, fp), _ptr(fr, ctypes.c_double), _ptr(g, fp), n_bl, Ng,
+                            Nf, freq_offset, cell)
+    return g

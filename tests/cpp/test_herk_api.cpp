@@ -170,6 +170,64 @@ int main(int argc, char** argv) {
         failures += threw2 ? 0 : 1;
     }
 
+    // Test 6: prepared-B complex GEMM with fused power (INT4 voltages) vs the oracle
+    {
+        const int gM = 64, gN = 96, gK = 80, gB = 2;
+        std::vector<uint8_t> qc((size_t)gM * gK * gB);
+        ftdo_gen_qc_uniform(seed + 7, (int64_t)qc.size(), qc.data());
+        std::vector<uint16_t> bre((size_t)gN * gK * gB), bim(bre.size());
+        std::vector<double> A((size_t)gM * gK * gB * 2), Bm((size_t)gN * gK * gB * 2);
+        for (size_t i = 0; i < qc.size(); ++i) {
+            int r, m;
+            ftdo_decode_qc(qc[i], &r, &m);
+            A[2 * i] = r;
+            A[2 * i + 1] = m;
+        }
+        for (size_t i = 0; i < bre.size(); ++i) {
+            const int r = (int)(i * 7 % 15) - 7, m = (int)(i * 11 % 15) - 7;
+            bre[i] = ftdo_f32_to_f16_rne((float)r);
+            bim[i] = ftdo_f32_to_f16_rne((float)m);
+            Bm[2 * i] = r;
+            Bm[2 * i + 1] = m;
+        }
+        std::vector<double> Cref((size_t)gM * gN * gB * 2);
+        ftdo_gemm_c64(A.data(), Bm.data(), Cref.data(), gM, gN, gK, gB, 0);
+        std::vector<float> re(Cref.size() / 2), im(re.size()), want(re.size()), got(re.size());
+        for (size_t i = 0; i < re.size(); ++i) {
+            re[i] = (float)Cref[2 * i];
+            im[i] = (float)Cref[2 * i + 1];
+        }
+        ftdo_power_f32(re.data(), im.data(), re.data() /* unused: beta = 0 */, (int64_t)re.size(),
+                       1.0f, 0.0f, want.data());
+        uint8_t* dq;
+        uint16_t *dbr, *dbi;
+        float* dp;
+        HIP_OK(hipMalloc(&dq, qc.size()));
+        HIP_OK(hipMalloc(&dbr, bre.size() * 2));
+        HIP_OK(hipMalloc(&dbi, bim.size() * 2));
+        HIP_OK(hipMalloc(&dp, got.size() * 4));
+        HIP_OK(hipMemcpy(dq, qc.data(), qc.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dbr, bre.data(), bre.size() * 2, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dbi, bim.data(), bim.size() * 2, hipMemcpyHostToDevice));
+        bool ok = true;
+        try {
+            api.prepare_b(dbr, dbi, gN, gK, gB, ComputePrecision::FP8);
+            ok = api.gemm_prepared_power_int4(dq, dp, gM, gN, gK, gB) == 0;
+            HIP_OK(hipDeviceSynchronize());
+            HIP_OK(hipMemcpy(got.data(), dp, got.size() * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < got.size(); ++i) ok = ok && std::memcmp(&got[i], &want[i], 4) == 0;
+        } catch (const std::runtime_error& e) {
+            std::printf("  GEMM threw: %s\n", e.what());
+            ok = false;
+        }
+        std::printf("  %-40s %s\n", "prepare_b + gemm_prepared_power_int4", ok ? "PASS" : "FAIL");
+        failures += ok ? 0 : 1;
+        HIP_OK(hipFree(dq));
+        HIP_OK(hipFree(dbr));
+        HIP_OK(hipFree(dbi));
+        HIP_OK(hipFree(dp));
+    }
+
     HIP_OK(hipFree(dA));
     HIP_OK(hipFree(dC));
     HIP_OK(hipFree(dC16));

@@ -160,3 +160,16 @@ def test_pipeline_validation_without_device():
     assert L.ftdx_pipeline_create(None, 8, 8, 1, 1, None) == ftdx.FTDX_ERR_INVALID_ARG
     assert L.ftdx_pipeline_correlate_many(None, None, None, 1) == ftdx.FTDX_ERR_INVALID_ARG
     assert L.ftdx_pipeline_destroy(None) == ftdx.FTDX_OK
+
+
+def test_gemm_validation_without_device():
+    from ftd_amd import ftdx
+    L = ftdx.lib()
+    assert L.ftdx_prepare_b(None, None, None, 8, 8, 1, 0, None) == ftdx.FTDX_ERR_INVALID_ARG
+    assert L.ftdx_gemm_prepared_direct(None, None, None, None, None, 8, 8, 8, 1, 1.0, 0.0,
+                                       None) == ftdx.FTDX_ERR_INVALID_ARG
+    assert "NULL context" in ftdx.last_error()
+    assert L.ftdx_gemm_prepared_power_int4(None, None, None, 8, 8, 8, 1, 1.0, 0.0,
+                                           None) == ftdx.FTDX_ERR_INVALID_ARG
+    assert L.ftdx_gemm(None, None, None, None, None, None, None, 8, 8, 8, 1, 0, 1, 1.0, 0.0, None,
+                       0, 0) == ftdx.FTDX_ERR_INVALID_ARG

@@ -1,0 +1,198 @@
+"""GPU parity of the complex GEMM with prepared B (prepare_b / gemm_prepared* /
+gemm, cutlass_gemm_api.h:215-470) against the oracle.  The reference tests draw
+integers in [-7, 7] (test_gemm_api.cu:93-110): FP8 holds them exactly and the FP32
+sums are exact, so those comparisons are bit-exact.  General FP16 values are
+compared with the FP64 product of their FP8 quantization (FP32 accumulation)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view({4: np.uint32, 2: np.uint16}[a.dtype.itemsize])
+
+
+def _int_planes(rng, shape):
+    re = rng.integers(-7, 8, shape).astype(np.float16)
+    im = rng.integers(-7, 8, shape).astype(np.float16)
+    return re, im
+
+
+def _dev(dev, *arrs):
+    import torch
+    return [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in arrs]
+
+
+SHAPES = [(64, 128, 96, 2), (100, 130, 50, 3), (1, 1, 1, 1), (257, 33, 300, 1), (128, 256, 2048, 2)]
+
+
+@pytest.mark.parametrize("M,N,K,B", SHAPES)
+def test_prepared_direct_integer_bit_exact(oracle, dev, M, N, K, B):
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(M + N + K)
+    a_re, a_im = _int_planes(rng, (B, M, K))
+    b_re, b_im = _int_planes(rng, (B, N, K))
+    dA_re, dA_im, dB_re, dB_im = _dev(dev, a_re, a_im, b_re, b_im)
+    C_re = torch.full((B, M, N), float("nan"), dtype=torch.float32, device=dev)
+    C_im = torch.full_like(C_re, float("nan"))
+    g = ftdx.CutlassComplexGemm()
+    print(f"gemm direct M={M} N={N} K={K} B={B}", flush=True)
+    g.prepare_b(dB_re, dB_im, N, K, B)
+    assert g.gemm_prepared_direct(dA_re, dA_im, C_re, C_im, M, N, K, B) == 0
+    torch.cuda.synchronize()
+    ref = oracle.gemm_c64(a_re.astype(np.float64) + 1j * a_im, b_re.astype(np.float64) + 1j * b_im)
+    assert np.array_equal(_bits(C_re.cpu().numpy()), _bits(ref.real.astype(np.float32)))
+    assert np.array_equal(_bits(C_im.cpu().numpy()), _bits(ref.imag.astype(np.float32)))
+
+
+@pytest.mark.parametrize("output", ["fp32", "fp16"])
+def test_gemm_prepared_outputs_and_alpha_beta(oracle, dev, output):
+    import torch
+    from ftd_amd import ftdx
+    M, N, K, B = 96, 80, 64, 2
+    rng = np.random.default_rng(9)
+    a_re, a_im = _int_planes(rng, (B, M, K))
+    b_re, b_im = _int_planes(rng, (B, N, K))
+    dA_re, dA_im, dB_re, dB_im = _dev(dev, a_re, a_im, b_re, b_im)
+    dt = torch.float32 if output == "fp32" else torch.float16
+    old_re = rng.integers(-50, 50, (B, M, N)).astype(np.float32)
+    old_im = rng.integers(-50, 50, (B, M, N)).astype(np.float32)
+    C_re = torch.from_numpy(old_re).to(dev).to(dt)
+    C_im = torch.from_numpy(old_im).to(dev).to(dt)
+    g = ftdx.CutlassComplexGemm()
+    g.prepare_b(dB_re, dB_im, N, K, B)
+    op = ftdx.OutputPrecision.FP32 if output == "fp32" else ftdx.OutputPrecision.FP16
+    assert g.gemm_prepared(dA_re, dA_im, C_re, C_im, M, N, K, B, output=op, alpha=0.5,
+                           beta=2.0) == 0
+    torch.cuda.synchronize()
+    ref = oracle.gemm_c64(a_re.astype(np.float64) + 1j * a_im, b_re.astype(np.float64) + 1j * b_im)
+    for got, r, o in ((C_re, ref.real, old_re), (C_im, ref.imag, old_im)):
+        want = (0.5 * r + 2.0 * o)  # exact in FP32 (small integers and halves)
+        assert np.array_equal(got.float().cpu().numpy(), want.astype(np.float32).astype(
+            np.float16 if output == "fp16" else np.float32).astype(np.float32))
+
+
+@pytest.mark.parametrize("M,N,K,B", [(64, 128, 96, 2), (100, 130, 50, 3)])
+def test_prepared_int4_direct_and_power(oracle, dev, M, N, K, B):
+    import torch
+    from ftd_amd import ftdx
+    qc = oracle.gen_bytes(M * 7 + K, M * K * B)  # raw QC bytes, -0 nibbles included
+    Aq = oracle.decode_qc_array(qc).reshape(B, M, K)
+    rng = np.random.default_rng(N)
+    b_re, b_im = _int_planes(rng, (B, N, K))
+    dA, dB_re, dB_im = _dev(dev, qc, b_re, b_im)
+    g = ftdx.CutlassComplexGemm()
+    g.prepare_b(dB_re, dB_im, N, K, B)
+    C_re = torch.empty((B, M, N), dtype=torch.float32, device=dev)
+    C_im = torch.empty_like(C_re)
+    P = torch.empty_like(C_re)
+    print(f"gemm int4 M={M} N={N} K={K} B={B}", flush=True)
+    assert g.gemm_prepared_direct_int4(dA, C_re, C_im, M, N, K, B) == 0
+    assert g.gemm_prepared_power_int4(dA, P, M, N, K, B) == 0
+    torch.cuda.synchronize()
+    ref = oracle.gemm_c64(Aq, b_re.astype(np.float64) + 1j * b_im)
+    re32, im32 = ref.real.astype(np.float32), ref.imag.astype(np.float32)
+    assert np.array_equal(_bits(C_re.cpu().numpy()), _bits(re32))
+    assert np.array_equal(_bits(C_im.cpu().numpy()), _bits(im32))
+    pw = oracle.power_f32(re32, im32).reshape(B, M, N)
+    assert np.array_equal(_bits(P.cpu().numpy()), _bits(pw))
+
+
+def test_power_fp16_and_fp8_inputs_with_beta(oracle, dev):
+    import torch
+    from ftd_amd import ftdx
+    M, N, K, B = 70, 96, 128, 2
+    rng = np.random.default_rng(21)
+    a_re, a_im = _int_planes(rng, (B, M, K))
+    b_re, b_im = _int_planes(rng, (B, N, K))
+    dA_re, dA_im, dB_re, dB_im = _dev(dev, a_re, a_im, b_re, b_im)
+    # FP8 interleaved A: the E4M3 bytes of the same integers, [re, im] pairs
+    qc = np.array([oracle.encode_qc(int(r), int(i)) for r, i in
+                   zip(a_re.reshape(-1).astype(int), a_im.reshape(-1).astype(int))], dtype=np.uint8)
+    fp8 = np.array([oracle.qc_to_fp8_pair(int(b)) for b in qc], dtype=np.uint8).reshape(-1)
+    dA8, = _dev(dev, fp8)
+    old = rng.integers(-100, 100, (B, M, N)).astype(np.float32)
+    P16 = torch.from_numpy(old).to(dev)
+    P8 = torch.from_numpy(old).to(dev)
+    g = ftdx.CutlassComplexGemm()
+    g.prepare_b(dB_re, dB_im, N, K, B)
+    assert g.gemm_prepared_power(dA_re, dA_im, P16, M, N, K, B, alpha=0.25, beta=3.0) == 0
+    assert g.gemm_prepared_power_fp8(dA8, P8, M, N, K, B, alpha=0.25, beta=3.0) == 0
+    torch.cuda.synchronize()
+    ref = oracle.gemm_c64(a_re.astype(np.float64) + 1j * a_im, b_re.astype(np.float64) + 1j * b_im)
+    pw = oracle.power_f32(ref.real.astype(np.float32), ref.imag.astype(np.float32), alpha=0.25,
+                          beta=3.0, old=old).reshape(B, M, N)
+    assert np.array_equal(_bits(P16.cpu().numpy()), _bits(pw))
+    assert np.array_equal(_bits(P8.cpu().numpy()), _bits(pw))
+
+
+def test_general_fp16_values_vs_fp64_of_fp8(oracle, dev):
+    """Non-integer FP16 operands: FP8 E4M3 quantization (clamp +-448, nearest-even),
+    then the product within FP32 accumulation of the FP64 result."""
+    import torch
+    from ftd_amd import ftdx
+    M, N, K, B = 64, 64, 512, 1
+    rng = np.random.default_rng(5)
+    a_re, a_im, b_re, b_im = [(rng.standard_normal((B, d, K)) * 3).astype(np.float16)
+                              for d in (M, M, N, N)]
+    a_re[0, 0, 0] = 600.0  # clamps to 448
+    q = lambda x: np.vectorize(lambda v: oracle.fp8e4m3_to_float(
+        oracle.f32_to_e4m3_clamped(float(v))))(x.astype(np.float32))
+    ref = oracle.gemm_c64(q(a_re) + 1j * q(a_im), q(b_re) + 1j * q(b_im))
+    dA_re, dA_im, dB_re, dB_im = _dev(dev, a_re, a_im, b_re, b_im)
+    C_re = torch.empty((B, M, N), dtype=torch.float32, device=dev)
+    C_im = torch.empty_like(C_re)
+    g = ftdx.CutlassComplexGemm()
+    assert g.gemm(dA_re, dA_im, dB_re, dB_im, C_re, C_im, M, N, K, B,
+                  output=ftdx.OutputPrecision.FP32) == 0
+    torch.cuda.synchronize()
+    got = C_re.cpu().numpy().astype(np.float64) + 1j * C_im.cpu().numpy()
+    scale = np.abs(ref).max()
+    # The FP8 MFMA sums each 64-product step at reduced internal precision: exact on
+    # integer data (every partial sum representable, tests above), about 1e-4 of the
+    # largest element here (measured 9e-5); 3e-4 bounds it.
+    assert np.abs(got - ref).max() <= 3e-4 * scale
+
+
+def test_errors_and_prepared_b_isolation(oracle, dev):
+    import torch
+    from ftd_amd import ftdx
+    M, N, K, B = 32, 48, 40, 1
+    rng = np.random.default_rng(6)
+    a_re, a_im = _int_planes(rng, (B, M, K))
+    b_re, b_im = _int_planes(rng, (B, N, K))
+    c_re, c_im = _int_planes(rng, (B, N, K))
+    dA_re, dA_im, dB_re, dB_im, dC_re, dC_im = _dev(dev, a_re, a_im, b_re, b_im, c_re, c_im)
+    out_re = torch.empty((B, M, N), dtype=torch.float32, device=dev)
+    out_im = torch.empty_like(out_re)
+    g = ftdx.CutlassComplexGemm()
+    # not prepared / mismatched shape: invalid argument, no launch
+    assert g.gemm_prepared_direct(dA_re, dA_im, out_re, out_im, M, N, K, B) == ftdx.FTDX_ERR_INVALID_ARG
+    g.prepare_b(dB_re, dB_im, N, K, B)
+    assert g.gemm_prepared_direct(dA_re, dA_im, out_re, out_im, M, N + 1, K, B) == \
+        ftdx.FTDX_ERR_INVALID_ARG
+    with pytest.raises(ftdx.UnsupportedError):
+        g.prepare_b(dB_re, dB_im, N, K, B, compute=ftdx.ComputePrecision.FP6)
+    g.prepare_b(dB_re, dB_im, N, K, B)
+    # a one-shot gemm with another B leaves the prepared B untouched
+    tmp_re = torch.empty((B, M, N), dtype=torch.float32, device=dev)
+    tmp_im = torch.empty_like(tmp_re)
+    assert g.gemm(dA_re, dA_im, dC_re, dC_im, tmp_re, tmp_im, M, N, K, B,
+                  output=ftdx.OutputPrecision.FP32) == 0
+    assert g.gemm_prepared_direct(dA_re, dA_im, out_re, out_im, M, N, K, B) == 0
+    torch.cuda.synchronize()
+    ref = oracle.gemm_c64(a_re.astype(np.float64) + 1j * a_im, b_re.astype(np.float64) + 1j * b_im)
+    assert np.array_equal(out_re.cpu().numpy(), ref.real.astype(np.float32))
+    ref2 = oracle.gemm_c64(a_re.astype(np.float64) + 1j * a_im, c_re.astype(np.float64) + 1j * c_im)
+    assert np.array_equal(tmp_im.cpu().numpy(), ref2.imag.astype(np.float32))
+    # M = 0 is a no-op; K = 0 leaves beta * C
+    assert g.gemm_prepared_direct(dA_re, dA_im, out_re, out_im, 0, N, K, B) == 0
+    g.prepare_b(dB_re, dB_im, N, 0, B)
+    out_re.fill_(4.0)
+    out_im.fill_(-2.0)
+    assert g.gemm_prepared_direct(dA_re, dA_im, out_re, out_im, M, N, 0, B, alpha=1.0, beta=0.5) == 0
+    torch.cuda.synchronize()
+    assert torch.all(out_re == 2.0) and torch.all(out_im == -1.0)
