@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -165,35 +166,59 @@ int autotune(ftdx_context* ctx, const HerkPlainArgs& a0, bool out_fp16, hipStrea
     hipEvent_t t0, t1;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
+    // mean ms of `reps` launches, or a negative value if the launch failed
+    auto time_cand = [&](int cfg, int remap, int reps) -> float {
+        (void)hipEventRecord(t0, s);
+        for (int r = 0; r < reps; ++r)
+            if (launch_herk_int4_plain(a, cfg, out_fp16, remap, s) != hipSuccess) {
+                (void)hipGetLastError();
+                return -1.0f;
+            }
+        (void)hipEventRecord(t1, s);
+        if (hipEventSynchronize(t1) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1.0f;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        return ms / reps;
+    };
+    // first pass: every (configuration, workgroup order), one warm-up launch each
+    struct Cand {
+        int cfg, remap;
+        float ms;
+    };
+    std::vector<Cand> cands;
+    for (int cfg = 0; cfg < NUM_TILE_CONFIGS; ++cfg)
+        for (int remap = 0; remap <= 1; ++remap) {
+            if (time_cand(cfg, remap, 1) < 0.0f) continue;
+            const float ms = time_cand(cfg, remap, 3);
+            if (ms < 0.0f) continue;
+            if (ctx->strategy_verbosity >= 2)
+                std::printf("ftdx tune:   %-16s xcd_remap=%d  %.4f ms\n", tile_config_name(cfg), remap, ms);
+            cands.push_back({cfg, remap, ms});
+        }
     best.ms = 1e30f;
     best.cfg = kDefaultCfg;
     best.xcd_remap = kDefaultRemap;
-    for (int cfg = 0; cfg < NUM_TILE_CONFIGS; ++cfg) {
-        for (int remap = 0; remap <= 1; ++remap) {
-            if (launch_herk_int4_plain(a, cfg, out_fp16, remap, s) != hipSuccess) {
-                (void)hipGetLastError();
-                continue;
+    if (!cands.empty()) {
+        // second pass: the three fastest again in mirrored order (A B C C B A), so a
+        // clock that drifts under sustained MFMA load weighs on every finalist alike
+        std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+        const size_t nf = cands.size() < 3 ? cands.size() : 3;
+        std::vector<float> sum(nf, 0.0f);
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t q = 0; q < nf; ++q) {
+                const size_t i = pass == 0 ? q : nf - 1 - q;
+                const float ms = time_cand(cands[i].cfg, cands[i].remap, 3);
+                sum[i] += ms < 0.0f ? 1e30f : ms;
             }
-            const int reps = 3;
-            (void)hipEventRecord(t0, s);
-            for (int r = 0; r < reps; ++r) (void)launch_herk_int4_plain(a, cfg, out_fp16, remap, s);
-            (void)hipEventRecord(t1, s);
-            if (hipEventSynchronize(t1) != hipSuccess) {
-                (void)hipGetLastError();
-                continue;
+        for (size_t i = 0; i < nf; ++i)
+            if (sum[i] / 2 < best.ms) {
+                best.ms = sum[i] / 2;
+                best.cfg = cands[i].cfg;
+                best.xcd_remap = cands[i].remap;
             }
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, t0, t1);
-            ms /= reps;
-            if (ctx->strategy_verbosity >= 2)
-                std::printf("ftdx tune:   %-14s xcd_remap=%d  %.4f ms\n", tile_config_name(cfg),
-                            remap, ms);
-            if (ms < best.ms) {
-                best.ms = ms;
-                best.cfg = cfg;
-                best.xcd_remap = remap;
-            }
-        }
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);

@@ -378,8 +378,6 @@ struct TileRunner {
         __syncthreads();
     }
 
-    // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
-    // live and would spill the second register set).
     // GEMM-layout epilogue: element (i, j) of C = alpha A B^T + beta C at i*ldc + j.
     template <typename OutT, int FMT>
     static __device__ __forceinline__ void put_gemm(const TileJob& j, int64_t idx, float re, float im) {
@@ -403,6 +401,132 @@ struct TileRunner {
         }
     }
 
+    // One stage of MFMAs on the staged rows in `st` (KC samples, all 32-sample steps).
+    template <int FMT>
+    static __device__ __forceinline__ void mfma_stage(const uint8_t* st, int a_off, int b_off,
+                                                      v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                      v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+#pragma unroll
+        for (int ks = 0; ks < Cfg::KC / 32; ++ks) {
+            v8i au[Cfg::FM], bu[Cfg::FN];
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a)
+                au[a] = lds_read32(st + a_off + a * 32 * Cfg::PITCH + ks * 64);
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b)
+                bu[b] = lds_read32(st + b_off + b * 32 * Cfg::PITCH + ks * 64);
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a) {
+                const v8i av = conj_planar(au[a]);
+#pragma unroll
+                for (int b = 0; b < Cfg::FN; ++b) {
+                    if constexpr (FMT == OUT_LOWER_PACKED || FMT == OUT_GEMM_PLANAR ||
+                                  FMT == OUT_GEMM_POWER) {
+                        acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
+                    } else {  // transposed accumulator, same values
+                        acc_re[a][b] = mfma_32x32x64_fp8(bu[b], au[a], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64_fp8(bu[b], av, acc_im[a][b]);
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- Interior tiles of the fused QC path ----
+    // Every row and column of the tile exists, K is a whole number of stages and the
+    // product runs over one K segment.  The loads then need no bounds checks, so each
+    // stage is one basic block -- global loads for a later stage, this stage's
+    // ds_read + MFMA, the next stage's unpack + ds_write -- which the scheduler can
+    // interleave.  A diagonal tile loads its column block as well (the same rows
+    // as part of its row block) to keep the body uniform; the MFMAs read the row copy.
+    static __device__ __forceinline__ bool full_tile(const TileJob& j) {
+        return j.nseg <= 1 && j.K > 0 && j.K % Cfg::KC == 0 && j.row0 + Cfg::TM <= j.N &&
+               j.col0 + Cfg::TN <= j.N && (int64_t)j.N * j.K <= 0xFFFFFFFFll;
+    }
+
+    // Byte offset of each 16-byte segment of this thread inside one problem's rows.
+    static __device__ __forceinline__ void full_offsets(const TileJob& j, int tid,
+                                                        uint32_t (&off)[Cfg::SEGS_PER_THREAD]) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS_PER_THREAD; ++q) {
+            int lrow, seg;
+            seg_coords(q, tid, lrow, seg);
+            const int grow = lrow >= Cfg::TM ? j.col0 + (lrow - Cfg::TM) : j.row0 + lrow;
+            off[q] = (uint32_t)grow * (uint32_t)j.K + (uint32_t)(seg * 16);
+        }
+    }
+
+    static __device__ __forceinline__ void full_load(const uint8_t* base,
+                                                     const uint32_t (&off)[Cfg::SEGS_PER_THREAD],
+                                                     uint4 (&g)[Cfg::SEGS_PER_THREAD]) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS_PER_THREAD; ++q)
+            g[q] = *reinterpret_cast<const uint4*>(base + off[q]);
+    }
+
+    static __device__ __forceinline__ void full_store(uint8_t* st, int tid,
+                                                      const uint4 (&g)[Cfg::SEGS_PER_THREAD]) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS_PER_THREAD; ++q) {
+            int lrow, seg;
+            seg_coords(q, tid, lrow, seg);
+            uint4 lo, hi;
+            qc4_to_fp8_planar(g[q].x, lo.x, lo.y);
+            qc4_to_fp8_planar(g[q].y, lo.z, lo.w);
+            qc4_to_fp8_planar(g[q].z, hi.x, hi.y);
+            qc4_to_fp8_planar(g[q].w, hi.z, hi.w);
+            uint8_t* dst = st + lrow * Cfg::PITCH + seg * 32;
+            *reinterpret_cast<uint4*>(dst) = lo;
+            *reinterpret_cast<uint4*>(dst + 16) = hi;
+        }
+    }
+
+    template <int FMT, bool ACTIVE, int DEPTH>
+    static __device__ __forceinline__ void full_tile_loop(const TileJob& j, uint8_t* __restrict__ lds,
+                                                          int tid, int nst, int a_off, int b_off,
+                                                          v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                          v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+        constexpr int NG = Cfg::SEGS_PER_THREAD;
+        uint32_t off[NG];
+        full_offsets(j, tid, off);
+        // stage s's first sample; past the end the last stage is loaded again (in
+        // bounds, never stored where it is read)
+        auto src = [&](int s) { return j.A + (int64_t)(s < nst ? s : nst - 1) * Cfg::KC; };
+        if constexpr (DEPTH == 2) {
+            uint4 ga[NG], gb[NG];
+            full_load(src(0), off, ga);
+            full_store(lds, tid, ga);
+            full_load(src(1), off, ga);
+            __syncthreads();
+            for (int s = 0; s < nst; s += 2) {
+                full_load(src(s + 2), off, gb);
+                if constexpr (ACTIVE) mfma_stage<FMT>(lds, a_off, b_off, acc_re, acc_im);
+                full_store(lds + Cfg::STAGE, tid, ga);
+                __syncthreads();
+                if (s + 1 >= nst) break;
+                full_load(src(s + 3), off, ga);
+                if constexpr (ACTIVE) mfma_stage<FMT>(lds + Cfg::STAGE, a_off, b_off, acc_re, acc_im);
+                full_store(lds, tid, gb);
+                __syncthreads();
+            }
+        } else {
+            uint4 g[NG];
+            full_load(src(0), off, g);
+            full_store(lds, tid, g);
+            __syncthreads();
+            for (int s = 0; s < nst; ++s) {
+                full_load(src(s + 1), off, g);
+                if constexpr (ACTIVE)
+                    mfma_stage<FMT>(lds + (s & 1) * Cfg::STAGE, a_off, b_off, acc_re, acc_im);
+                full_store(lds + ((s + 1) & 1) * Cfg::STAGE, tid, g);
+                __syncthreads();
+            }
+        }
+    }
+
+    // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
+    // live and would spill the second register set).
     // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
     // live and would spill the second register set).
     template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false>
@@ -410,7 +534,8 @@ struct TileRunner {
         constexpr bool GEMM = FMT == OUT_GEMM_PLANAR || FMT == OUT_GEMM_POWER;
         static_assert(!GEMM || MODE == LOAD_FP8, "GEMM layouts read pre-cast rows");
         const int tid = threadIdx.x;
-        const int lane = tid & 63, wave = tid >> 6;
+        // wave index in an SGPR: the per-wave conditions below are then scalar branches
+        const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         const int wm = wave / Cfg::WAVES_N, wn = wave % Cfg::WAVES_N;
         const int wrow0 = j.row0 + wm * Cfg::WM;  // first row of this wave's block
         const int wcol0 = j.col0 + wn * Cfg::WN;
@@ -452,38 +577,24 @@ struct TileRunner {
                           (wn * Cfg::WN + lr) * Cfg::PITCH + lh;
 
         auto compute = [&](const uint8_t* st) {
-            if (active) {
-#pragma unroll
-                for (int ks = 0; ks < Cfg::KC / 32; ++ks) {
-                    v8i au[Cfg::FM], bu[Cfg::FN];
-#pragma unroll
-                    for (int a = 0; a < Cfg::FM; ++a)
-                        au[a] = lds_read32(st + a_off + a * 32 * Cfg::PITCH + ks * 64);
-#pragma unroll
-                    for (int b = 0; b < Cfg::FN; ++b)
-                        bu[b] = lds_read32(st + b_off + b * 32 * Cfg::PITCH + ks * 64);
-#pragma unroll
-                    for (int a = 0; a < Cfg::FM; ++a) {
-                        const v8i av = conj_planar(au[a]);
-#pragma unroll
-                        for (int b = 0; b < Cfg::FN; ++b) {
-                            if constexpr (FMT == OUT_LOWER_PACKED || GEMM) {
-                                acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
-                                acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
-                            } else {  // transposed accumulator, same values
-                                acc_re[a][b] = mfma_32x32x64_fp8(bu[b], au[a], acc_re[a][b]);
-                                acc_im[a][b] = mfma_32x32x64_fp8(bu[b], av, acc_im[a][b]);
-                            }
-                        }
-                    }
-                }
-            }
+            if (active) mfma_stage<FMT>(st, a_off, b_off, acc_re, acc_im);
         };
 
         // Depth 2 only where its second register set fits without spilling: the
         // fused 16-byte load mode (pre-cast stages are twice as wide).
         constexpr int DEPTH = (MODE == LOAD_VEC16 && !SHALLOW) ? Cfg::DEPTH : 1;
-        if constexpr (DEPTH == 1) {
+        bool looped = false;
+        if constexpr (MODE == LOAD_VEC16 && !GEMM) {
+            if (full_tile(j)) {
+                if (active)
+                    full_tile_loop<FMT, true, DEPTH>(j, lds, tid, nst, a_off, b_off, acc_re, acc_im);
+                else
+                    full_tile_loop<FMT, false, DEPTH>(j, lds, tid, nst, a_off, b_off, acc_re, acc_im);
+                looped = true;
+            }
+        }
+        if (looped) {
+        } else if constexpr (DEPTH == 1) {
             uint4 g[NG];
             if (nst > 0) {
                 load(0, g);

@@ -34,13 +34,13 @@ def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B):
     assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref))
 
 
-@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
-def test_tile_configs_fp16_beta(oracle, dev, cfg):
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("N,K,B", [(200, 96, 2), (300, 256, 2)])  # edge loop; interior tiles
+def test_tile_configs_fp16_beta(oracle, dev, cfg, N, K, B):
     import torch
     from ftd_amd import ftdx
     g = ftdx.CutlassComplexGemm()
     g.set_launch_config(cfg, 1)
-    N, K, B = 200, 96, 2
     A_np = oracle.gen_qc_uniform(5 + cfg, N * K * B)
     rng = np.random.default_rng(cfg)
     C0 = rng.integers(-100, 100, size=(B, N * (N + 1) // 2, 2)).astype(np.float16)
