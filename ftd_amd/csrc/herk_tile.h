@@ -123,6 +123,8 @@ template <typename OutT>
 struct OutIO;
 
 typedef float v2f __attribute__((ext_vector_type(2)));
+typedef float v4f_a8 __attribute__((ext_vector_type(4), aligned(8)));
+typedef uint32_t v2u_a4 __attribute__((ext_vector_type(2), aligned(4)));
 
 // Output is written once and never re-read by the kernel: stores are
 // non-temporal so the L2 keeps the re-read input rows instead.
@@ -138,6 +140,18 @@ struct OutIO<float> {
         v[0] = re;
         v[1] = im;
         __builtin_nontemporal_store(v, reinterpret_cast<v2f*>(C) + idx);
+    }
+    // elements idx and idx + 1 in one 16-byte access (8-byte aligned)
+    static __device__ __forceinline__ void load2(const void* C, int64_t idx, float& r0, float& i0,
+                                                 float& r1, float& i1) {
+        const v4f_a8 v = *reinterpret_cast<const v4f_a8*>(reinterpret_cast<const v2f*>(C) + idx);
+        r0 = v[0]; i0 = v[1]; r1 = v[2]; i1 = v[3];
+    }
+    static __device__ __forceinline__ void store2(void* C, int64_t idx, float r0, float i0, float r1,
+                                                  float i1) {
+        v4f_a8 v;
+        v[0] = r0; v[1] = i0; v[2] = r1; v[3] = i1;
+        __builtin_nontemporal_store(v, reinterpret_cast<v4f_a8*>(reinterpret_cast<v2f*>(C) + idx));
     }
 };
 
@@ -160,6 +174,21 @@ struct OutIO<_Float16> {
         __builtin_nontemporal_store((uint32_t)lo | ((uint32_t)hi << 16),
                                     reinterpret_cast<uint32_t*>(C) + idx);
     }
+    static __device__ __forceinline__ void load2(const void* C, int64_t idx, float& r0, float& i0,
+                                                 float& r1, float& i1) {
+        load(C, idx, r0, i0);
+        load(C, idx + 1, r1, i1);
+    }
+    static __device__ __forceinline__ void store2(void* C, int64_t idx, float r0, float i0, float r1,
+                                                  float i1) {
+        _Float16 h[4] = {(_Float16)r0, (_Float16)i0, (_Float16)r1, (_Float16)i1};
+        uint16_t u[4];
+        __builtin_memcpy(u, h, 8);
+        v2u_a4 v;
+        v[0] = (uint32_t)u[0] | ((uint32_t)u[1] << 16);
+        v[1] = (uint32_t)u[2] | ((uint32_t)u[3] << 16);
+        __builtin_nontemporal_store(v, reinterpret_cast<v2u_a4*>(reinterpret_cast<uint32_t*>(C) + idx));
+    }
 };
 
 typedef double v2d __attribute__((ext_vector_type(2)));
@@ -178,6 +207,16 @@ struct OutIO<double> {
         v[0] = re;
         v[1] = im;
         __builtin_nontemporal_store(v, reinterpret_cast<v2d*>(C) + idx);
+    }
+    static __device__ __forceinline__ void load2(const void* C, int64_t idx, double& r0, double& i0,
+                                                 double& r1, double& i1) {
+        load(C, idx, r0, i0);
+        load(C, idx + 1, r1, i1);
+    }
+    static __device__ __forceinline__ void store2(void* C, int64_t idx, double r0, double i0,
+                                                  double r1, double i1) {
+        store(C, idx, r0, i0);
+        store(C, idx + 1, r1, i1);
     }
 };
 
@@ -321,6 +360,24 @@ struct TileRunner {
         OutIO<OutT>::store(j.C, idx, re, im);
     }
 
+    // put() of elements idx and idx + 1 with one load / store each.
+    template <typename OutT>
+    static __device__ __forceinline__ void put2(const TileJob& j, int64_t idx, float r0, float i0,
+                                                float r1, float i1) {
+        typedef typename EpiType<OutT>::type E;
+        const E alpha = (E)j.alpha, beta = (E)j.beta;
+        E a0 = (E)r0 * alpha, b0 = (E)i0 * alpha, a1 = (E)r1 * alpha, b1 = (E)i1 * alpha;
+        if (beta != (E)0) {
+            E o0, p0, o1, p1;
+            OutIO<OutT>::load2(j.C, idx, o0, p0, o1, p1);
+            a0 = fma_e(beta, o0, a0);
+            b0 = fma_e(beta, p0, b0);
+            a1 = fma_e(beta, o1, a1);
+            b1 = fma_e(beta, p1, b1);
+        }
+        OutIO<OutT>::store2(j.C, idx, a0, b0, a1, b1);
+    }
+
     // Lower-packed epilogue through LDS.  Packed-triangle rows start at arbitrary
     // 8-byte offsets, so the accumulator layout's stores (two 256-byte runs of two
     // rows per instruction) leave partial cache lines at every run end.  Here each
@@ -357,18 +414,24 @@ struct TileRunner {
                             *reinterpret_cast<v2f*>(region + lrow * EPW + (b * 32 + n) * 8) = v;
                         }
                     __builtin_amdgcn_wave_barrier();
+                    // two rows per instruction: lanes 0-31 row lr0, 32-63 row lr0 + 1, each
+                    // lane two consecutive columns (one 16-byte LDS read, one store)
+                    const int hl = lane >> 5, cp = (lane & 31) * 2;
 #pragma unroll 1
-                    for (int lr = 0; lr < RP; ++lr) {
+                    for (int lr0 = 0; lr0 < RP; lr0 += 2) {
+                        const int lr = lr0 + hl;
                         const int gi = wrow0 + a * 32 + h * RP + lr;
-                        if (gi >= j.N) break;
                         const int64_t rbase = tri_index(gi, 0);
 #pragma unroll
                         for (int c0 = 0; c0 < Cfg::WN; c0 += 64) {
-                            const int c = c0 + lane;
+                            const int c = c0 + cp;
                             const int gj = wcol0 + c;
-                            if (gj <= gi) {
-                                const v2f v = *reinterpret_cast<const v2f*>(region + lr * EPW + c * 8);
-                                put<OutT>(j, rbase + gj, v[0], gi == gj ? 0.0f : v[1]);
+                            if (gi < j.N && gj <= gi) {
+                                const v4f_a8 v = *reinterpret_cast<const v4f_a8*>(region + lr * EPW + c * 8);
+                                if (gj < gi)  // (gi, gj) off the diagonal; (gi, gj + 1) may be on it
+                                    put2<OutT>(j, rbase + gj, v[0], v[1], v[2], gj + 1 == gi ? 0.0f : v[3]);
+                                else
+                                    put<OutT>(j, rbase + gj, v[0], 0.0f);
                             }
                         }
                     }
@@ -525,8 +588,119 @@ struct TileRunner {
         }
     }
 
-    // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
-    // live and would spill the second register set).
+    // ---- Interior tiles of pre-cast rows: global -> LDS without registers ----
+    // Stage layout: TM + TN rows of 2*KC bytes, unpadded; 16-byte chunk c of row r is
+    // stored at chunk position c ^ f(r).  The LDS serves a ds_read_b128 16 lanes (16
+    // rows of one operand) at a time from 64 four-byte banks, i.e. 16 chunk slots per
+    // 256 bytes; f(r) = (r / RPP) mod DCPR, RPP rows sharing one 256-byte bank period,
+    // gives those 16 rows 16 distinct slots.  One global_load_lds_dwordx4 per wave fills 1 KB of
+    // consecutive LDS (lane L -> byte 16 L); each lane fetches the global chunk that
+    // belongs at its position.  No VGPR staging, no ds_write, no unpack VALU.
+    static constexpr int DROW = 2 * Cfg::KC;                   // bytes per staged row
+    static constexpr int DCPR = DROW / 16;                     // 16-byte chunks per row
+    static constexpr int RPP = DROW < 256 ? 256 / DROW : 1;    // rows per 256-byte bank period
+    static __device__ __forceinline__ int swz(int row) { return (row / RPP) & (DCPR - 1); }
+    static constexpr int DSTAGE = (Cfg::TM + Cfg::TN) * DROW;  // bytes per stage
+    static constexpr int NWAVES = Cfg::WAVES_M * Cfg::WAVES_N;
+    static constexpr int DOPS = DSTAGE / (1024 * NWAVES);      // loads per wave per stage
+    static constexpr bool DMA_OK =
+        DSTAGE % (1024 * NWAVES) == 0 && 2 * DSTAGE <= Cfg::LDS_BYTES && DCPR * RPP == 16;
+
+    static __device__ __forceinline__ bool dma_tile(const TileJob& j) {
+        return j.K > 0 && j.K % Cfg::KC == 0 && j.row0 + Cfg::TM <= j.N && j.col0 + Cfg::TN <= j.N &&
+               (reinterpret_cast<uintptr_t>(j.A) & 15u) == 0 && j.lda % 16 == 0 &&
+               (int64_t)j.N * j.lda <= 0xFFFFFFFFll;
+    }
+
+    // Global byte offset (inside the problem's rows, stage 0) of each load of this lane.
+    static __device__ __forceinline__ void dma_offsets(const TileJob& j, int lane, int wave,
+                                                       uint32_t (&off)[DOPS]) {
+#pragma unroll
+        for (int q = 0; q < DOPS; ++q) {
+            const int byte = (q * NWAVES + wave) * 1024 + lane * 16;  // position in the stage
+            const int row = byte / DROW, pc = (byte % DROW) / 16;
+            const int c = pc ^ swz(row);                               // logical chunk held there
+            const int grow = row < Cfg::TM ? j.row0 + row : j.col0 + (row - Cfg::TM);
+            off[q] = (uint32_t)grow * (uint32_t)j.lda + (uint32_t)(c * 16);
+        }
+    }
+
+    static __device__ __forceinline__ void dma_stage(const uint8_t* base, uint8_t* st, int wave,
+                                                     const uint32_t (&off)[DOPS]) {
+#pragma unroll
+        for (int q = 0; q < DOPS; ++q)
+            __builtin_amdgcn_global_load_lds(
+                (const void*)(base + off[q]),
+                (void __attribute__((address_space(3)))*)(st + (q * NWAVES + wave) * 1024), 16, 0, 0);
+    }
+
+    // One stage of MFMAs on the swizzled layout.
+    template <int FMT>
+    static __device__ __forceinline__ void mfma_stage_dma(const uint8_t* st, int a_row, int b_row,
+                                                          int lane, v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                          v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+        // operand rows are (multiple of 32) + (lane & 31): one swizzle for all of them
+        const int sw = swz(lane & 31), half = (lane >> 5) * 2;
+#pragma unroll
+        for (int ks = 0; ks < Cfg::KC / 32; ++ks) {
+            const int p0 = ((ks * 4 + half) ^ sw) * 16, p1 = p0 ^ 16;
+            v8i au[Cfg::FM], bu[Cfg::FN];
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a) {
+                const uint8_t* r = st + (a_row + a * 32) * DROW;
+                const uint4 x = *reinterpret_cast<const uint4*>(r + p0);
+                const uint4 y = *reinterpret_cast<const uint4*>(r + p1);
+                au[a][0] = (int)x.x; au[a][1] = (int)x.y; au[a][2] = (int)x.z; au[a][3] = (int)x.w;
+                au[a][4] = (int)y.x; au[a][5] = (int)y.y; au[a][6] = (int)y.z; au[a][7] = (int)y.w;
+            }
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b) {
+                const uint8_t* r = st + (b_row + b * 32) * DROW;
+                const uint4 x = *reinterpret_cast<const uint4*>(r + p0);
+                const uint4 y = *reinterpret_cast<const uint4*>(r + p1);
+                bu[b][0] = (int)x.x; bu[b][1] = (int)x.y; bu[b][2] = (int)x.z; bu[b][3] = (int)x.w;
+                bu[b][4] = (int)y.x; bu[b][5] = (int)y.y; bu[b][6] = (int)y.z; bu[b][7] = (int)y.w;
+            }
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a) {
+                const v8i av = conj_planar(au[a]);
+#pragma unroll
+                for (int b = 0; b < Cfg::FN; ++b) {
+                    if constexpr (FMT == OUT_LOWER_PACKED || FMT == OUT_GEMM_PLANAR ||
+                                  FMT == OUT_GEMM_POWER) {
+                        acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
+                    } else {
+                        acc_re[a][b] = mfma_32x32x64_fp8(bu[b], au[a], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64_fp8(bu[b], av, acc_im[a][b]);
+                    }
+                }
+            }
+        }
+    }
+
+    // Two stage buffers, loads one stage ahead, one barrier per stage: the barrier
+    // that publishes stage s also retires every wave's reads of stage s - 1, whose
+    // buffer the loads of stage s + 1 then overwrite.  Ends with a barrier.
+    template <int FMT, bool ACTIVE>
+    static __device__ __forceinline__ void dma_tile_loop(const TileJob& j, uint8_t* __restrict__ lds,
+                                                         int lane, int wave, int nst, int a_row,
+                                                         int b_row, v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                         v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+        uint32_t off[DOPS];
+        dma_offsets(j, lane, wave, off);
+        dma_stage(j.A, lds, wave, off);
+        for (int s = 0; s < nst; ++s) {
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's loads of stage s landed
+            __syncthreads();
+            uint8_t* cur = lds + (s & 1) * DSTAGE;
+            if (s + 1 < nst)
+                dma_stage(j.A + (int64_t)(s + 1) * 2 * Cfg::KC, lds + ((s + 1) & 1) * DSTAGE, wave, off);
+            if constexpr (ACTIVE) mfma_stage_dma<FMT>(cur, a_row, b_row, lane, acc_re, acc_im);
+        }
+        __syncthreads();
+    }
+
     // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
     // live and would spill the second register set).
     template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false>
@@ -590,6 +764,17 @@ struct TileRunner {
                     full_tile_loop<FMT, true, DEPTH>(j, lds, tid, nst, a_off, b_off, acc_re, acc_im);
                 else
                     full_tile_loop<FMT, false, DEPTH>(j, lds, tid, nst, a_off, b_off, acc_re, acc_im);
+                looped = true;
+            }
+        }
+        if constexpr (MODE == LOAD_FP8 && !GEMM && DMA_OK) {
+            if (dma_tile(j)) {
+                const int a_row = wm * Cfg::WM + (lane & 31);
+                const int b_row = (share_cols ? (j.col0 - j.row0) : Cfg::TM) + wn * Cfg::WN + (lane & 31);
+                if (active)
+                    dma_tile_loop<FMT, true>(j, lds, lane, wave, nst, a_row, b_row, acc_re, acc_im);
+                else
+                    dma_tile_loop<FMT, false>(j, lds, lane, wave, nst, a_row, b_row, acc_re, acc_im);
                 looped = true;
             }
         }

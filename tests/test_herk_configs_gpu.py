@@ -19,11 +19,16 @@ def _bits(a):
 @pytest.mark.parametrize("N,K,B", [(1, 1, 2), (63, 70, 3), (129, 64, 2), (257, 100, 1),
                                    (300, 256, 2), (520, 128, 1),
                                    (97, 150, 2), (64, 320, 1)])  # 3 and 5 stages of 64
-def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B):
+@pytest.mark.parametrize("mode", ["fused", "precast"])
+def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B, mode):
+    """mode "precast": FP8 rows converted first (HerkMode.ForceBaseline), whose
+    interior tiles are loaded global -> LDS directly."""
     import torch
     from ftd_amd import ftdx
     g = ftdx.CutlassComplexGemm()
     g.set_launch_config(cfg, remap)
+    if mode == "precast":
+        g.set_herk_mode(ftdx.HerkMode.ForceBaseline)
     A_np = oracle.gen_qc_uniform(31 * N + K + cfg, N * K * B)
     A = torch.from_numpy(A_np).to(dev)
     C = torch.full((B, N * (N + 1) // 2, 2), float("nan"), dtype=torch.float32, device=dev)
@@ -36,11 +41,14 @@ def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B):
 
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("N,K,B", [(200, 96, 2), (300, 256, 2)])  # edge loop; interior tiles
-def test_tile_configs_fp16_beta(oracle, dev, cfg, N, K, B):
+@pytest.mark.parametrize("mode", ["fused", "precast"])
+def test_tile_configs_fp16_beta(oracle, dev, cfg, N, K, B, mode):
     import torch
     from ftd_amd import ftdx
     g = ftdx.CutlassComplexGemm()
     g.set_launch_config(cfg, 1)
+    if mode == "precast":
+        g.set_herk_mode(ftdx.HerkMode.ForceBaseline)
     A_np = oracle.gen_qc_uniform(5 + cfg, N * K * B)
     rng = np.random.default_rng(cfg)
     C0 = rng.integers(-100, 100, size=(B, N * (N + 1) // 2, 2)).astype(np.float16)

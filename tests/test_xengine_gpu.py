@@ -52,10 +52,11 @@ def test_xengine_host_buffers_match_oracle(oracle, dev, shape, prec, fmt):
 
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("fmt", ["tri", "herm"])
-def test_xengine_every_tile_config(oracle, dev, cfg, fmt):
+@pytest.mark.parametrize("n_time", [80, 256])  # K = 40 (edge loop only); K = 128 (interior tiles)
+def test_xengine_every_tile_config(oracle, dev, cfg, fmt, n_time):
     import torch
     from ftd_amd import ftdx
-    n_ant, n_chan, n_time, n_ti, n_pol = 300, 2, 80, 2, 1
+    n_ant, n_chan, n_ti, n_pol = 300, 2, 2, 1
     mf = ftdx.XENGINE_MAT_TRI if fmt == "tri" else ftdx.XENGINE_MAT_HERM
     xe = ftdx.XEngine(_param(ftdx, n_ant, n_chan, n_time, n_ti, n_pol, ftdx.QPREC_SINGLE, mf,
                              loc=ftdx.LOCATION_DEVICE))
