@@ -263,6 +263,43 @@ def beamformer_study(torch, ftdx, dev, M: int, N: int, K: int, B: int, seed: int
             "pct_fp8_peak": round(100.0 * tf / FP8_DENSE_PEAK_TFLOPS, 2)}
 
 
+def pol_reduce_study(torch, ftdx, dev, N: int, K: int, n_ch: int, n_pol: int, n_ti: int, seed: int,
+                     reps: int) -> dict:
+    """HERK + polarization sum: one fused pass (ftdx_herk_pol_reduce) against the HERK of
+    every polarization followed by the stand-alone sum; outputs compared byte for byte."""
+    B = n_ch * n_pol * n_ti
+    A = synth_qc(torch, N * K * B, seed, dev)
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_strategy_tune_verbosity(0)
+    full = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    red = torch.empty((n_ch * n_ti, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    fused = torch.empty_like(red)
+
+    def unfused():
+        ctx.herk(A, full, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                 ftdx.OutputPrecision.FP32, tune=False)
+        ftdx.pol_reduce_triangle(full, N, n_ch, n_pol, n_ti, out=red)
+
+    def one_pass():
+        ftdx.herk_pol_reduce(ctx, A, N, K, n_ch, n_pol, n_ti, C=fused)
+
+    res = {}
+    for name, fn in (("unfused_ms", unfused), ("fused_ms", one_pass)):
+        fn()
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = round(e0.elapsed_time(e1) / reps, 4)
+    same = bool(torch.equal(red.view(torch.int32), fused.view(torch.int32)))
+    return {"inputs": N, "K": K, "channels": n_ch, "pols": n_pol, "time_inner": n_ti, "seed": seed,
+            **res, "speedup": round(res["unfused_ms"] / res["fused_ms"], 3), "identical_output": same}
+
+
 def streaming_study(torch, ftdx, dev, N: int, K: int, B: int, payloads: int, seed: int) -> dict:
     """Host payloads through ftdx_pipeline_* (H2D, HERK, D2H double-buffered): the
     PCIe-inclusive rate next to the HERK alone on resident data."""
@@ -387,6 +424,7 @@ def main() -> int:
     ap.add_argument("--no-calib", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-beamformer", action="store_true")
+    ap.add_argument("--no-polsum", action="store_true")
     ap.add_argument("--batched-problems", type=int, default=512)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed steps, one plain device copy of this many MiB "
@@ -488,6 +526,14 @@ def main() -> int:
         log(f"bench: beamformer {bf['ms']:.3f} ms, {bf['tflops_4m_equiv']:.0f} TFLOP/s 4M-equiv")
         torch.cuda.empty_cache()
 
+    polsum = None
+    if not args.no_polsum:
+        log("bench: HERK + polarization sum, 2048 inputs x 16 ch x 2 pol x 2 ti, K=1024 ...")
+        polsum = pol_reduce_study(torch, ftdx, dev, 2048, 1024, 16, 2, 2, 8, 5)
+        log(f"bench: pol sum fused {polsum['fused_ms']:.3f} ms vs unfused {polsum['unfused_ms']:.3f} ms "
+            f"({polsum['speedup']:.2f}x, identical={polsum['identical_output']})")
+        torch.cuda.empty_cache()
+
     stream_res = None
     if not args.no_stream:
         log("bench: streaming host payloads 1024 x 16 x K=2048, 8 payloads ...")
@@ -537,6 +583,7 @@ def main() -> int:
         "gain_calibration": calib,
         "streaming_host_payloads": stream_res,
         "beamformer_gemm_power": bf,
+        "herk_pol_sum_fusion": polsum,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
