@@ -425,6 +425,7 @@ def main() -> int:
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-beamformer", action="store_true")
     ap.add_argument("--no-polsum", action="store_true")
+    ap.add_argument("--no-bf6", action="store_true")
     ap.add_argument("--batched-problems", type=int, default=512)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed steps, one plain device copy of this many MiB "
@@ -484,6 +485,38 @@ def main() -> int:
     achieved = flop / (kern_ms * 1e-3) / 1e12
     log(f"bench: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms "
         f"(median {per[len(per) // 2]:.3f}), {value:.1f} TFLOP/s 4M-equiv")
+
+    bf6 = None
+    if not args.no_bf6:
+        # the same tile fed by E3M2 operand rows (one extra pass over the input)
+        ctx6 = ftdx.CutlassComplexGemm()
+        ctx6.set_launch_config(cfg | ftdx.TILE_BF6, remap)
+        C6 = torch.empty_like(C)
+
+        def step6():
+            rc = ctx6.herk(A, C6, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                           ftdx.OutputPrecision.FP32, stream=stream, tune=False)
+            if rc != 0:
+                raise RuntimeError(f"herk (E3M2) failed: {rc} {ftdx.last_error()}")
+
+        for _ in range(2):
+            step6()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(5):
+            step6()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms6 = e0.elapsed_time(e1) / 5
+        same = bool(torch.equal(C.view(torch.int32), C6.view(torch.int32)))
+        bf6 = {"tile_config": cfg | ftdx.TILE_BF6, "ms": round(ms6, 4),
+               "tflops_4m_equiv": round(flop / (ms6 * 1e-3) / 1e12, 2),
+               "fp8_headline_ms": round(kern_ms, 4), "identical_output": same,
+               "note": "E3M2 conversion pass + E3M2 MFMA tile; opt-in (FTDX_TILE_BF6)"}
+        log(f"bench: E3M2 operands {ms6:.3f} ms vs FP8 {kern_ms:.3f} ms, identical={same}")
+        del C6, ctx6
+        torch.cuda.empty_cache()
 
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib * (1 << 20)
@@ -584,6 +617,7 @@ def main() -> int:
         "streaming_host_payloads": stream_res,
         "beamformer_gemm_power": bf,
         "herk_pol_sum_fusion": polsum,
+        "e3m2_operands": bf6,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -255,7 +255,7 @@ int grow_precast(ftdx_context* ctx, size_t need, const char* what) {
     int rc = ensure_scratch(ctx->precast, ctx->precast_cap, need, what);
     if (rc == FTDX_OK && before && before != ctx->precast) {
         for (auto& gph : ctx->graphs)
-            if (gph.used && gph.mode == 1) {
+            if (gph.used && gph.mode != 0) {
                 (void)hipGraphExecDestroy(gph.exec);
                 gph = ftdx_context::GraphEntry();
             }
@@ -315,7 +315,7 @@ int gemm_run(ftdx_context* ctx, const uint8_t* brows, GemmInput kind, const void
     g.ldc = N;
     g.c_bstride = (int64_t)M * N;
     // 256-row tiles once M fills them: each column block of B is then read once
-    const int cfg = ctx->forced_cfg >= 0 ? ctx->forced_cfg
+    const int cfg = ctx->forced_cfg >= 0 ? tile_base(ctx->forced_cfg)
                                          : (M >= 192 ? TILE_256x128_K64 : TILE_128x128_K64);
     const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : kDefaultRemap;
     e = launch_gemm_fp8(g, cfg, fmt, out_fp16, remap, s);
@@ -383,8 +383,7 @@ int ftdx_set_direct_tile_config(ftdx_context* ctx, int config) {
 
 int ftdx_set_launch_config(ftdx_context* ctx, int tile_config, int xcd_remap) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_launch_config: NULL context");
-    if (tile_config < -1 || tile_config >= ftdx::NUM_TILE_CONFIGS || xcd_remap < -1 ||
-        xcd_remap > 1)
+    if ((tile_config != -1 && !ftdx::valid_tile_config(tile_config)) || xcd_remap < -1 || xcd_remap > 1)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_launch_config: out of range");
     ctx->forced_cfg = tile_config;
     ctx->forced_remap = xcd_remap;
@@ -703,27 +702,11 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     ftdx::pick_config(ctx, key, cfg, remap, cached);
     const bool explicit_cfg = ctx->forced_cfg >= 0 || ctx->direct_tile != FTDX_DIRECT_TILE_AUTO;
     const int Kp = ftdx::precast_row_samples(K);
+    const size_t rows_need = (size_t)N * batch_count * 2 * Kp;
     if (precast) {
-        const size_t need = (size_t)N * batch_count * 2 * Kp;
-        int rc = ftdx::grow_precast(ctx, need, "ftdx_herk: scratch");
+        int rc = ftdx::grow_precast(ctx, rows_need, "ftdx_herk: scratch");
         if (rc != FTDX_OK) return rc;
     }
-    auto run = [&](hipStream_t st) -> hipError_t {
-        ftdx::HerkPlainArgs b = a;
-        if (precast) {
-            hipError_t pe = input == FTDX_INPUT_FP16
-                ? ftdx::launch_promote_fp16_fp8_planar(static_cast<const uint16_t*>(A), ctx->precast,
-                                                       (int64_t)N * batch_count, K, st)
-                : ftdx::launch_promote_qc_fp8_planar(static_cast<const uint8_t*>(A), ctx->precast,
-                                                     (int64_t)N * batch_count, K, st);
-            if (pe != hipSuccess) return pe;
-            b.A = ctx->precast;
-            b.precast = 1;
-            b.lda = (int64_t)2 * Kp;
-            b.a_bstride = (int64_t)N * 2 * Kp;
-        }
-        return ftdx::launch_herk_int4_plain(b, cfg, fp16, remap, st);
-    };
     if (tune && !cached && !explicit_cfg && !precast) {
         ftdx::TuneEntry best;
         if (ftdx::autotune(ctx, a, fp16, s, key, best) == FTDX_OK) {
@@ -731,10 +714,36 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
             if (ctx->forced_remap < 0) remap = best.xcd_remap;
         }
     }
+    // INT4 input on the fused path may use the E3M2 variant of its tile when the
+    // configuration asks for it (FTDX_TILE_BF6 flag: an explicit choice or a cache entry)
+    const bool bf6 = !precast && (cfg & ftdx::TILE_BF6);
+    cfg = ftdx::tile_base(cfg);
+    if (bf6) {
+        int rc = ftdx::grow_precast(ctx, rows_need, "ftdx_herk: scratch");
+        if (rc != FTDX_OK) return rc;
+    }
+    auto run = [&](hipStream_t st) -> hipError_t {
+        ftdx::HerkPlainArgs b = a;
+        if (precast || bf6) {
+            hipError_t pe = input == FTDX_INPUT_FP16
+                ? ftdx::launch_promote_fp16_fp8_planar(static_cast<const uint16_t*>(A), ctx->precast,
+                                                       (int64_t)N * batch_count, K, st)
+                : bf6 ? ftdx::launch_promote_qc_bf6_planar(static_cast<const uint8_t*>(A), ctx->precast,
+                                                           (int64_t)N * batch_count, K, st)
+                      : ftdx::launch_promote_qc_fp8_planar(static_cast<const uint8_t*>(A), ctx->precast,
+                                                           (int64_t)N * batch_count, K, st);
+            if (pe != hipSuccess) return pe;
+            b.A = ctx->precast;
+            b.precast = bf6 ? 2 : 1;
+            b.lda = (int64_t)2 * Kp;
+            b.a_bstride = (int64_t)N * 2 * Kp;
+        }
+        return ftdx::launch_herk_int4_plain(b, cfg, fp16, remap, st);
+    };
     // Graph replay of identical calls (cutlass_gemm_api.cu:1738-1807): capture the
     // launch sequence once per (shape, scalars, pointers), then replay it.
     if (ctx->graph && ctx->herk_mode != FTDX_HERK_AUTO) {
-        const int mode = precast ? 1 : 0;
+        const int mode = bf6 ? 2 : precast ? 1 : 0;
         ftdx_context::GraphEntry* hit = nullptr;
         for (auto& gph : ctx->graphs)
             if (gph.used && gph.N == N && gph.K == K && gph.batch == batch_count &&
@@ -879,7 +888,8 @@ int ftdx_correlate_ftd(ftdx_context* ctx, const void* raw, void* tri_fp32, int n
     if (!tri_fp32 || (!raw && K > 0)) return fail(FTDX_ERR_INVALID_ARG, "ftdx_correlate_ftd: NULL pointer");
     int rc = ftdx::grow_precast(ctx, ftdx::rows_bytes(n_ant, K, (int)batch) + 16, "ftdx_correlate_ftd: rows");
     if (rc != FTDX_OK) return rc;
-    const int cfg = ftdx::xengine_cfg(ctx->forced_cfg >= 0 ? ctx->forced_cfg : ftdx::direct_tile_to_cfg(ctx->direct_tile),
+    const int cfg = ftdx::xengine_cfg(ctx->forced_cfg >= 0 ? ftdx::tile_base(ctx->forced_cfg)
+                                                           : ftdx::direct_tile_to_cfg(ctx->direct_tile),
                                       n_ant, (int)batch);
     const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : 1;
     hipError_t e = ftdx::run_ftd_payload(static_cast<const uint8_t*>(raw), static_cast<float*>(tri_fp32),
@@ -926,7 +936,7 @@ int ftdx_herk_pol_reduce(ftdx_context* ctx, const void* A, void* C, int N, int K
     a.c_bstride = (int64_t)N * (N + 1);
     a.nseg = n_pol;
     a.seg_group = n_time_inner;
-    const int cfg = ctx->forced_cfg >= 0 ? ctx->forced_cfg
+    const int cfg = ctx->forced_cfg >= 0 ? ftdx::tile_base(ctx->forced_cfg)
                                          : (ftdx::direct_tile_to_cfg(ctx->direct_tile) >= 0
                                                 ? ftdx::direct_tile_to_cfg(ctx->direct_tile)
                                                 : ftdx::kDefaultCfg);

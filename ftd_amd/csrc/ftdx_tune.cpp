@@ -30,10 +30,11 @@ void TuneCache::put(const TuneKey& k, const TuneEntry& e) { map_[k] = e; }
 
 std::string TuneCache::fingerprint() {
     std::string f = "gfx950-abi1";
-    for (int c = 0; c < NUM_TILE_CONFIGS; ++c) {
-        f += '-';
-        f += tile_config_name(c);
-    }
+    for (int flag : {0, TILE_BF6})
+        for (int c = 0; c < NUM_TILE_CONFIGS; ++c) {
+            f += '-';
+            f += tile_config_name(c | flag);
+        }
     return f;
 }
 
@@ -60,7 +61,7 @@ int TuneCache::load(const std::string& path, int* skipped) {
         TuneEntry e{};
         if (!(ss >> kind >> k.N >> k.K >> k.batch >> k.out_fp16 >> k.vec >> e.cfg >> e.xcd_remap >>
               e.ms) ||
-            kind != "herk" || e.cfg < 0 || e.cfg >= NUM_TILE_CONFIGS || k.N < 0 || k.K < 0 ||
+            kind != "herk" || !valid_tile_config(e.cfg) || k.N < 0 || k.K < 0 ||
             k.batch < 0) {
             ++bad;
             continue;

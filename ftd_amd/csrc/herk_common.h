@@ -52,6 +52,18 @@ __device__ __forceinline__ v16f mfma_32x32x64_fp8(v8i a, v8i b, v16f c) {
     return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
 }
 
+// Operand formats of the unscaled f8f6f4 MFMA (cbsz / blgp codes): OCP E4M3, and E3M2
+// ("BF6": sign, 3 exponent bits, bias 3, 2 mantissa bits), 32 six-bit values in the low
+// 192 bits of the 8-VGPR operand.  E3M2 holds every integer in [-8, 8] exactly, so integer
+// QC products sum exactly in FP32 either way; the 6-bit format runs the MFMA at twice
+// the FP8 rate.
+enum OperandFormat { OPF_FP8 = 0, OPF_BF6 = 3 };
+
+template <int OPF>
+__device__ __forceinline__ v16f mfma_32x32x64(v8i a, v8i b, v16f c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, OPF, OPF, 0, 0, 0, 0);
+}
+
 // lower-triangle tile index t -> (bi, bj), bj <= bi, t = bi(bi+1)/2 + bj
 __device__ __forceinline__ void tri_decode(int t, int& bi, int& bj) {
     int r = (int)((__builtin_sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);

@@ -17,6 +17,14 @@ enum TileConfigId {
     NUM_TILE_CONFIGS = 6
 };
 
+// Flag on a tile configuration id (ftdx_herk, INT4 input): the same tile fed by
+// pre-cast E3M2 operand rows.  Exact for QC data, MFMAs at twice the FP8 rate, one
+// extra pass over the input.
+constexpr int TILE_BF6 = 16;
+inline bool valid_tile_config(int c) { return c >= 0 && (c & ~TILE_BF6) < NUM_TILE_CONFIGS; }
+inline int tile_base(int c) { return c & ~TILE_BF6; }
+
+
 const char* tile_config_name(int cfg);
 int tile_config_rows(int cfg);   // TM
 int tile_config_cols(int cfg);   // TN
@@ -44,7 +52,8 @@ struct HerkPlainArgs {
     float alpha, beta;
     int64_t a_bstride;  // bytes of A per batch (N*K)
     int64_t c_bstride;  // scalars of C per batch (N*(N+1))
-    int precast = 0;    // 1: A is pre-cast FP8 (dword-planar), row pitch lda bytes
+    int precast = 0;    // 1: A is pre-cast FP8 (dword-planar) rows, 2: pre-cast E3M2 rows
+                        // (32 bytes per 16 samples); row pitch lda bytes
     int64_t lda = 0;
     // Sum over K segments (QC input only): output batch b = ob * seg_group + ib sums
     // the HERKs of input batches (ob * nseg + s) * seg_group + ib, s < nseg, as one

@@ -124,6 +124,15 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
     const bool vec = (a.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15u) == 0);
     dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    if (a.precast == 2) {
+        if (out_fp16)
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_BF6>), grid, block, 0,
+                               stream, a, tiles, xcd_remap);
+        else
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_BF6>), grid, block, 0,
+                               stream, a, tiles, xcd_remap);
+        return hipGetLastError();
+    }
     if (a.precast) {
         if (out_fp16)
             hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_FP8>), grid, block, 0,
@@ -248,15 +257,13 @@ hipError_t launch_grouped_cfg(const GroupProblemDev* probs, const GroupItemDev* 
 }  // namespace
 
 const char* tile_config_name(int cfg) {
-    switch (cfg) {
-        case TILE_128x128_K64: return "t128x128_k64";
-        case TILE_128x128_K128: return "t128x128_k128";
-        case TILE_64x64_K64: return "t64x64_k64";
-        case TILE_256x128_K64: return "t256x128_k64";
-        case TILE_128x128_K64_P2: return "t128x128_k64_p2";
-        case TILE_256x128_K64_P2: return "t256x128_k64_p2";
-        default: return "unknown";
-    }
+    static const char* const names[2][NUM_TILE_CONFIGS] = {
+        {"t128x128_k64", "t128x128_k128", "t64x64_k64", "t256x128_k64", "t128x128_k64_p2",
+         "t256x128_k64_p2"},
+        {"t128x128_k64_bf6", "t128x128_k128_bf6", "t64x64_k64_bf6", "t256x128_k64_bf6",
+         "t128x128_k64_p2_bf6", "t256x128_k64_p2_bf6"}};
+    if (!valid_tile_config(cfg)) return "unknown";
+    return names[(cfg & TILE_BF6) ? 1 : 0][tile_base(cfg)];
 }
 
 int tile_config_rows(int cfg) {

@@ -60,10 +60,19 @@ struct TileCfg {
     static_assert(LDS_BYTES * WG_PER_CU <= 160 * 1024, "LDS of resident workgroups exceeds 160 KB");
 };
 
+// s_waitcnt vmcnt(N) (other counters unconstrained), gfx9 encoding
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | (((N >> 4) & 0x3) << 14));
+}
+
 // LOAD_VEC16 / LOAD_BYTES: A holds QC bytes [N][K] (row pitch K), unpacked in
 // the load stage.  LOAD_FP8: A holds pre-cast operand rows (row pitch lda bytes,
 // 8 bytes per 4 samples in the dword-planar order), copied to LDS as they are.
-enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1, LOAD_FP8 = 2 };
+// LOAD_BF6: pre-cast E3M2 rows with the same geometry (one 32-byte slot per 16
+// samples: 12 bytes of Re codes, 12 of Im codes, 8 of padding), MFMAs in E3M2.
+enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1, LOAD_FP8 = 2, LOAD_BF6 = 3 };
 
 struct TileJob {
     const uint8_t* A;  // this problem's operand rows
@@ -105,6 +114,28 @@ __device__ __forceinline__ v8i conj_planar(v8i a) {
         r[2 * q + 1] = a[2 * q] ^ (int)0x80808080u;
     }
     return r;
+}
+
+// The same for an E3M2 operand slot: dwords 0-2 hold the 16 six-bit Re codes, 3-5 the
+// 16 Im codes (code k at bits 6k..6k+5 of the 96-bit half), 6-7 padding.  v = [im | -re]:
+// swap the halves and flip bit 5 of every Re code.
+__device__ __forceinline__ v8i conj_planar_bf6(v8i a) {
+    v8i r;
+    r[0] = a[3];
+    r[1] = a[4];
+    r[2] = a[5];
+    r[3] = a[0] ^ (int)0x20820820u;
+    r[4] = a[1] ^ (int)0x08208208u;
+    r[5] = a[2] ^ (int)0x82082082u;
+    r[6] = 0;
+    r[7] = 0;
+    return r;
+}
+
+template <int OPF>
+__device__ __forceinline__ v8i conj_op(v8i a) {
+    if constexpr (OPF == OPF_BF6) return conj_planar_bf6(a);
+    else return conj_planar(a);
 }
 
 // Output layouts of one problem's Gram matrix (element (i, j) below is
@@ -239,7 +270,7 @@ struct TileRunner {
 
     // Pre-cast FP8 rows: segment slot q of thread tid covers 8 samples of one row.
     // GEMM: column rows come from j.B (pitch ldb, Ncol rows) instead of j.A.
-    template <bool GEMM = false>
+    template <bool GEMM = false, bool BF6 = false>
     static __device__ __forceinline__ void load_stage_fp8(const TileJob& j, int k0, bool share_cols,
                                                           int tid, uint4 (&g)[Cfg::SEGS8_PER_THREAD]) {
 #pragma unroll
@@ -254,8 +285,10 @@ struct TileRunner {
             const int64_t pitch = from_b ? j.ldb : j.lda;
             const int lim = from_b ? j.Ncol : j.N;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            // pre-cast rows are zero-padded to a multiple of 16 samples
-            if (!(is_col && share_cols) && grow < lim && kk < j.K)
+            // pre-cast rows are zero-padded to a multiple of 16 samples; an E3M2 slot's
+            // two 16-byte halves both carry all 16 of its samples
+            const int kslot = BF6 ? (kk & ~15) : kk;
+            if (!(is_col && share_cols) && grow < lim && kslot < j.K)
                 v = *reinterpret_cast<const uint4*>(base + (int64_t)grow * pitch + (int64_t)kk * 2);
             g[q] = v;
         }
@@ -465,7 +498,7 @@ struct TileRunner {
     }
 
     // One stage of MFMAs on the staged rows in `st` (KC samples, all 32-sample steps).
-    template <int FMT>
+    template <int FMT, int OPF = OPF_FP8>
     static __device__ __forceinline__ void mfma_stage(const uint8_t* st, int a_off, int b_off,
                                                       v16f (&acc_re)[Cfg::FM][Cfg::FN],
                                                       v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
@@ -480,16 +513,16 @@ struct TileRunner {
                 bu[b] = lds_read32(st + b_off + b * 32 * Cfg::PITCH + ks * 64);
 #pragma unroll
             for (int a = 0; a < Cfg::FM; ++a) {
-                const v8i av = conj_planar(au[a]);
+                const v8i av = conj_op<OPF>(au[a]);
 #pragma unroll
                 for (int b = 0; b < Cfg::FN; ++b) {
                     if constexpr (FMT == OUT_LOWER_PACKED || FMT == OUT_GEMM_PLANAR ||
                                   FMT == OUT_GEMM_POWER) {
-                        acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
-                        acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
+                        acc_re[a][b] = mfma_32x32x64<OPF>(au[a], bu[b], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64<OPF>(av, bu[b], acc_im[a][b]);
                     } else {  // transposed accumulator, same values
-                        acc_re[a][b] = mfma_32x32x64_fp8(bu[b], au[a], acc_re[a][b]);
-                        acc_im[a][b] = mfma_32x32x64_fp8(bu[b], av, acc_im[a][b]);
+                        acc_re[a][b] = mfma_32x32x64<OPF>(bu[b], au[a], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64<OPF>(bu[b], av, acc_im[a][b]);
                     }
                 }
             }
@@ -635,7 +668,7 @@ struct TileRunner {
     }
 
     // One stage of MFMAs on the swizzled layout.
-    template <int FMT>
+    template <int FMT, int OPF>
     static __device__ __forceinline__ void mfma_stage_dma(const uint8_t* st, int a_row, int b_row,
                                                           int lane, v16f (&acc_re)[Cfg::FM][Cfg::FN],
                                                           v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
@@ -663,16 +696,16 @@ struct TileRunner {
             }
 #pragma unroll
             for (int a = 0; a < Cfg::FM; ++a) {
-                const v8i av = conj_planar(au[a]);
+                const v8i av = conj_op<OPF>(au[a]);
 #pragma unroll
                 for (int b = 0; b < Cfg::FN; ++b) {
                     if constexpr (FMT == OUT_LOWER_PACKED || FMT == OUT_GEMM_PLANAR ||
                                   FMT == OUT_GEMM_POWER) {
-                        acc_re[a][b] = mfma_32x32x64_fp8(au[a], bu[b], acc_re[a][b]);
-                        acc_im[a][b] = mfma_32x32x64_fp8(av, bu[b], acc_im[a][b]);
+                        acc_re[a][b] = mfma_32x32x64<OPF>(au[a], bu[b], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64<OPF>(av, bu[b], acc_im[a][b]);
                     } else {
-                        acc_re[a][b] = mfma_32x32x64_fp8(bu[b], au[a], acc_re[a][b]);
-                        acc_im[a][b] = mfma_32x32x64_fp8(bu[b], av, acc_im[a][b]);
+                        acc_re[a][b] = mfma_32x32x64<OPF>(bu[b], au[a], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64<OPF>(bu[b], av, acc_im[a][b]);
                     }
                 }
             }
@@ -681,8 +714,9 @@ struct TileRunner {
 
     // Two stage buffers, loads one stage ahead, one barrier per stage: the barrier
     // that publishes stage s also retires every wave's reads of stage s - 1, whose
-    // buffer the loads of stage s + 1 then overwrite.  Ends with a barrier.
-    template <int FMT, bool ACTIVE>
+    // buffer the loads of stage s + 1 then overwrite.  Only these loads are in flight
+    // in the loop, so vmcnt(0) waits for exactly them.  Ends with a barrier.
+    template <int FMT, bool ACTIVE, int OPF>
     static __device__ __forceinline__ void dma_tile_loop(const TileJob& j, uint8_t* __restrict__ lds,
                                                          int lane, int wave, int nst, int a_row,
                                                          int b_row, v16f (&acc_re)[Cfg::FM][Cfg::FN],
@@ -691,12 +725,12 @@ struct TileRunner {
         dma_offsets(j, lane, wave, off);
         dma_stage(j.A, lds, wave, off);
         for (int s = 0; s < nst; ++s) {
-            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's loads of stage s landed
+            wait_vmcnt<0>();  // this wave's loads of stage s landed
             __syncthreads();
             uint8_t* cur = lds + (s & 1) * DSTAGE;
             if (s + 1 < nst)
                 dma_stage(j.A + (int64_t)(s + 1) * 2 * Cfg::KC, lds + ((s + 1) & 1) * DSTAGE, wave, off);
-            if constexpr (ACTIVE) mfma_stage_dma<FMT>(cur, a_row, b_row, lane, acc_re, acc_im);
+            if constexpr (ACTIVE) mfma_stage_dma<FMT, OPF>(cur, a_row, b_row, lane, acc_re, acc_im);
         }
         __syncthreads();
     }
@@ -706,7 +740,9 @@ struct TileRunner {
     template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false>
     static __device__ __forceinline__ void run(const TileJob& j, uint8_t* __restrict__ lds) {
         constexpr bool GEMM = FMT == OUT_GEMM_PLANAR || FMT == OUT_GEMM_POWER;
-        static_assert(!GEMM || MODE == LOAD_FP8, "GEMM layouts read pre-cast rows");
+        static_assert(!GEMM || MODE == LOAD_FP8, "GEMM layouts read pre-cast FP8 rows");
+        constexpr bool PRE = MODE == LOAD_FP8 || MODE == LOAD_BF6;  // pre-cast operand rows
+        constexpr int OPF = MODE == LOAD_BF6 ? OPF_BF6 : OPF_FP8;
         const int tid = threadIdx.x;
         // wave index in an SGPR: the per-wave conditions below are then scalar branches
         const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -735,13 +771,13 @@ struct TileRunner {
 
         const int Kv = j.nseg > 1 ? j.K * j.nseg : j.K;  // samples the product runs over
         const int nst = (Kv + Cfg::KC - 1) / Cfg::KC;
-        constexpr int NG = MODE == LOAD_FP8 ? Cfg::SEGS8_PER_THREAD : Cfg::SEGS_PER_THREAD;
+        constexpr int NG = PRE ? Cfg::SEGS8_PER_THREAD : Cfg::SEGS_PER_THREAD;
         auto load = [&](int k0, uint4 (&g)[NG]) {
-            if constexpr (MODE == LOAD_FP8) load_stage_fp8<GEMM>(j, k0, share_cols, tid, g);
+            if constexpr (PRE) load_stage_fp8<GEMM, MODE == LOAD_BF6>(j, k0, share_cols, tid, g);
             else load_stage<MODE>(j, k0, share_cols, tid, g);
         };
         auto store = [&](uint8_t* st, const uint4 (&g)[NG]) {
-            if constexpr (MODE == LOAD_FP8) store_stage_fp8(st, share_cols, tid, g);
+            if constexpr (PRE) store_stage_fp8(st, share_cols, tid, g);
             else store_stage(st, share_cols, tid, g);
         };
 
@@ -751,7 +787,7 @@ struct TileRunner {
                           (wn * Cfg::WN + lr) * Cfg::PITCH + lh;
 
         auto compute = [&](const uint8_t* st) {
-            if (active) mfma_stage<FMT>(st, a_off, b_off, acc_re, acc_im);
+            if (active) mfma_stage<FMT, OPF>(st, a_off, b_off, acc_re, acc_im);
         };
 
         // Depth 2 only where its second register set fits without spilling: the
@@ -767,14 +803,14 @@ struct TileRunner {
                 looped = true;
             }
         }
-        if constexpr (MODE == LOAD_FP8 && !GEMM && DMA_OK) {
+        if constexpr (PRE && !GEMM && DMA_OK) {
             if (dma_tile(j)) {
                 const int a_row = wm * Cfg::WM + (lane & 31);
                 const int b_row = (share_cols ? (j.col0 - j.row0) : Cfg::TM) + wn * Cfg::WN + (lane & 31);
                 if (active)
-                    dma_tile_loop<FMT, true>(j, lds, lane, wave, nst, a_row, b_row, acc_re, acc_im);
+                    dma_tile_loop<FMT, true, OPF>(j, lds, lane, wave, nst, a_row, b_row, acc_re, acc_im);
                 else
-                    dma_tile_loop<FMT, false>(j, lds, lane, wave, nst, a_row, b_row, acc_re, acc_im);
+                    dma_tile_loop<FMT, false, OPF>(j, lds, lane, wave, nst, a_row, b_row, acc_re, acc_im);
                 looped = true;
             }
         }

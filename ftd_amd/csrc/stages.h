@@ -12,6 +12,9 @@ hipError_t launch_promote_qc_fp8_interleaved(const uint8_t* in, uint8_t* out, in
                                              hipStream_t s);
 hipError_t launch_promote_qc_fp8_planar(const uint8_t* in, uint8_t* out, int64_t rows, int K,
                                         hipStream_t s);
+// QC rows -> E3M2 ("BF6") operand rows for LOAD_BF6: 32 bytes per 16 samples, pitch 2*Kp.
+hipError_t launch_promote_qc_bf6_planar(const uint8_t* in, uint8_t* out, int64_t rows, int K,
+                                        hipStream_t s);
 hipError_t launch_promote_fp16_fp8_planar(const uint16_t* in, uint8_t* out, int64_t rows, int K,
                                           hipStream_t s);
 hipError_t launch_promote_qc_fp16_planar(const uint8_t* in, uint16_t* re, uint16_t* im, int64_t n,

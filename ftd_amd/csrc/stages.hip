@@ -112,6 +112,67 @@ promote_qc_fp8_planar_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict
     }
 }
 
+// 4 QC bytes -> E3M2 codes of their Re and Im nibbles, one code per byte (bits 5:0).
+// Magnitudes {0..7} -> {0x00, 0x0C, 0x10, 0x12, 0x14, 0x15, 0x16, 0x17} (two table dwords
+// indexed by v_perm_b32); the nibble's sign bit becomes bit 5.
+__device__ __forceinline__ void qc4_to_bf6_bytes(uint32_t x, uint32_t& re, uint32_t& im) {
+    const uint32_t TLO = 0x12100C00u;  // e3m2(0..3)
+    const uint32_t THI = 0x17161514u;  // e3m2(4..7)
+    re = __builtin_amdgcn_perm(THI, TLO, (x >> 4) & 0x07070707u) | ((x >> 2) & 0x20202020u);
+    im = __builtin_amdgcn_perm(THI, TLO, x & 0x07070707u) | ((x << 2) & 0x20202020u);
+}
+
+// four byte-wide codes -> 24 bits, code k at bits 6k..6k+5
+__device__ __forceinline__ uint32_t pack4_bf6(uint32_t c) {
+    return (c & 0x3Fu) | ((c >> 2) & 0xFC0u) | ((c >> 4) & 0x3F000u) | ((c >> 6) & 0xFC0000u);
+}
+
+// 16 byte-wide codes (c0: codes 0-3, ..., c3: codes 12-15) -> 96 bits in d0..d2
+__device__ __forceinline__ void pack16_bf6(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                           uint32_t& d0, uint32_t& d1, uint32_t& d2) {
+    const uint32_t p0 = pack4_bf6(c0), p1 = pack4_bf6(c1), p2 = pack4_bf6(c2), p3 = pack4_bf6(c3);
+    d0 = p0 | (p1 << 24);
+    d1 = (p1 >> 8) | (p2 << 16);
+    d2 = (p2 >> 16) | (p3 << 8);
+}
+
+// QC rows -> E3M2 operand rows, one 32-byte slot per 16 samples: dwords 0-2 the Re
+// codes of samples 0..15, dwords 3-5 the Im codes, dwords 6-7 zero.  Same row pitch
+// (2 * Kp bytes) and zero padding as the FP8 planar rows.
+template <bool VEC>
+__global__ void __launch_bounds__(ST_THREADS)
+promote_qc_bf6_planar_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                             int64_t rows, int K, int Kp) {
+    const int segs = Kp / 16;
+    const int64_t total = rows * segs;
+    const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
+    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride) {
+        const int64_t r = t / segs;
+        const int s = (int)(t - r * segs);
+        const int k0 = s * 16;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        const uint8_t* src = in + r * K + k0;
+        if (VEC) {
+            const uint4 v = *reinterpret_cast<const uint4*>(src);
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+            for (int e = 0; e < 16; ++e)
+                if (k0 + e < K) w[e >> 2] |= (uint32_t)src[e] << (8 * (e & 3));
+        }
+        uint32_t re[4], im[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) qc4_to_bf6_bytes(w[q], re[q], im[q]);
+        uint4 a, b;
+        pack16_bf6(re[0], re[1], re[2], re[3], a.x, a.y, a.z);
+        pack16_bf6(im[0], im[1], im[2], im[3], a.w, b.x, b.y);
+        b.z = 0u;
+        b.w = 0u;
+        uint4* dst = reinterpret_cast<uint4*>(out + r * (int64_t)Kp * 2 + (int64_t)k0 * 2);
+        dst[0] = a;
+        dst[1] = b;
+    }
+}
+
 // float -> OCP E4M3 byte, nearest-even, inputs already clamped to +-448
 __device__ __forceinline__ uint32_t f32_to_e4m3(float x) {
     const uint32_t sign = __builtin_signbitf(x) ? 0x80u : 0u;
@@ -545,6 +606,21 @@ hipError_t launch_promote_qc_fp8_planar(const uint8_t* in, uint8_t* out, int64_t
                            0, s, in, out, rows, K, Kp);
     else
         hipLaunchKernelGGL((promote_qc_fp8_planar_kernel<false>), dim3(grid_for(work)), dim3(ST_THREADS),
+                           0, s, in, out, rows, K, Kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_promote_qc_bf6_planar(const uint8_t* in, uint8_t* out, int64_t rows, int K,
+                                        hipStream_t s) {
+    if (rows <= 0 || K <= 0) return hipSuccess;
+    const int Kp = precast_row_samples(K);
+    const int64_t work = rows * (Kp / 16);
+    const bool vec = (K % 16 == 0) && ((reinterpret_cast<uintptr_t>(in) & 15u) == 0);
+    if (vec)
+        hipLaunchKernelGGL((promote_qc_bf6_planar_kernel<true>), dim3(grid_for(work)), dim3(ST_THREADS),
+                           0, s, in, out, rows, K, Kp);
+    else
+        hipLaunchKernelGGL((promote_qc_bf6_planar_kernel<false>), dim3(grid_for(work)), dim3(ST_THREADS),
                            0, s, in, out, rows, K, Kp);
     return hipGetLastError();
 }

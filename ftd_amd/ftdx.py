@@ -160,6 +160,7 @@ TILE_128x128_K64_P2 = 4
 TILE_256x128_K64_P2 = 5
 TILE_CONFIGS = (TILE_128x128_K64, TILE_128x128_K128, TILE_64x64_K64, TILE_256x128_K64,
                 TILE_128x128_K64_P2, TILE_256x128_K64_P2)
+TILE_BF6 = 16  # flag: the tile with pre-cast E3M2 operands (ftdx_herk, INT4 input)
 
 
 class HerkProblem(ctypes.Structure):

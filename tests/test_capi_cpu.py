@@ -80,6 +80,7 @@ def test_tune_cache_file_parsing(tmp_path):
         header,
         "herk\t4096\t2048\t96\t0\t1\t3\t1\t7.5",
         "herk\t128\t1024\t16\t0\t1\t0\t0\t0.1",
+        "herk\t256\t512\t4\t0\t1\t20\t1\t0.5",         # tile 4, E3M2 operands
         "herk\t1\t2",                                  # too short
         "herk\t10\t10\t1\t0\t1\t9\t0\t1.0",            # unknown tile config
         "gemm\t10\t10\t1\t0\t1\t0\t0\t1.0",            # other kind
@@ -88,9 +89,10 @@ def test_tune_cache_file_parsing(tmp_path):
         "herk\t64\t64\t1\t0\t1\t1\t0\t0.01",           # foreign fingerprint
     ]) + "\n")
     n, skipped = ctx.tune_load(str(f))
-    assert (n, skipped) == (2, 5)
+    assert (n, skipped) == (3, 5)
     assert ctx.query_config(4096, 2048, 96) == (3, 1)
     assert ctx.query_config(128, 1024, 16) == (0, 0)
+    assert ctx.query_config(256, 512, 4) == (4 | ftdx.TILE_BF6, 1)
     assert ctx.query_config(64, 64, 1) == (0, 1)       # default
     ctx.set_launch_config(2, 0)
     assert ctx.query_config(4096, 2048, 96) == (2, 0)  # explicit setting wins
@@ -100,7 +102,7 @@ def test_tune_cache_file_parsing(tmp_path):
     out = tmp_path / "round.txt"
     ctx.tune_save(str(out))
     ctx2 = ftdx.CutlassComplexGemm()
-    assert ctx2.tune_load(str(out)) == (2, 0)
+    assert ctx2.tune_load(str(out)) == (3, 0)
     with pytest.raises(ftdx.FtdxError):
         ctx2.tune_load(str(tmp_path / "missing.txt"))
 

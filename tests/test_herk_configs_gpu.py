@@ -19,14 +19,15 @@ def _bits(a):
 @pytest.mark.parametrize("N,K,B", [(1, 1, 2), (63, 70, 3), (129, 64, 2), (257, 100, 1),
                                    (300, 256, 2), (520, 128, 1),
                                    (97, 150, 2), (64, 320, 1)])  # 3 and 5 stages of 64
-@pytest.mark.parametrize("mode", ["fused", "precast"])
+@pytest.mark.parametrize("mode", ["fused", "precast", "bf6"])
 def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B, mode):
     """mode "precast": FP8 rows converted first (HerkMode.ForceBaseline), whose
-    interior tiles are loaded global -> LDS directly."""
+    interior tiles are loaded global -> LDS directly; mode "bf6": rows converted to
+    E3M2 and the MFMAs run in E3M2 (FTDX_TILE_BF6)."""
     import torch
     from ftd_amd import ftdx
     g = ftdx.CutlassComplexGemm()
-    g.set_launch_config(cfg, remap)
+    g.set_launch_config(cfg | (ftdx.TILE_BF6 if mode == "bf6" else 0), remap)
     if mode == "precast":
         g.set_herk_mode(ftdx.HerkMode.ForceBaseline)
     A_np = oracle.gen_qc_uniform(31 * N + K + cfg, N * K * B)
@@ -41,12 +42,12 @@ def test_tile_configs_bit_exact(oracle, dev, cfg, remap, N, K, B, mode):
 
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("N,K,B", [(200, 96, 2), (300, 256, 2)])  # edge loop; interior tiles
-@pytest.mark.parametrize("mode", ["fused", "precast"])
+@pytest.mark.parametrize("mode", ["fused", "precast", "bf6"])
 def test_tile_configs_fp16_beta(oracle, dev, cfg, N, K, B, mode):
     import torch
     from ftd_amd import ftdx
     g = ftdx.CutlassComplexGemm()
-    g.set_launch_config(cfg, 1)
+    g.set_launch_config(cfg | (ftdx.TILE_BF6 if mode == "bf6" else 0), 1)
     if mode == "precast":
         g.set_herk_mode(ftdx.HerkMode.ForceBaseline)
     A_np = oracle.gen_qc_uniform(5 + cfg, N * K * B)
@@ -176,3 +177,21 @@ def test_autotune_and_cache_file(oracle, dev, tmp_path):
     torch.cuda.synchronize()
     ref = oracle.herk_f32(A_np, N, K, B, alpha=1.0, beta=1.0, C=C0)
     assert np.array_equal(_bits(C2.cpu().numpy()), _bits(ref))
+
+
+@pytest.mark.parametrize("N,K,B", [(300, 256, 2), (129, 70, 3)])
+def test_bf6_signed_zero_and_extremes(oracle, dev, N, K, B):
+    """E3M2 operands on data heavy in -0 nibbles (0x8 / 0x80) and +-7: every sign and
+    magnitude code, bit-exact against the oracle (FP32 sums stay exact)."""
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(N + K)
+    nib = rng.choice(np.array([0x0, 0x8, 0x7, 0xF, 0x1, 0x9, 0x5, 0xD], np.uint8), size=(2, N * K * B))
+    A_np = ((nib[0] << 4) | nib[1]).astype(np.uint8)
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(ftdx.TILE_128x128_K64_P2 | ftdx.TILE_BF6, 1)
+    C = torch.full((B, N * (N + 1) // 2, 2), float("nan"), dtype=torch.float32, device=dev)
+    assert g.herk(torch.from_numpy(A_np).to(dev), C, N, K, B, ftdx.InputPrecision.INT4,
+                  ftdx.ComputePrecision.FP8, ftdx.OutputPrecision.FP32, tune=False) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(_bits(C.cpu().numpy()), _bits(oracle.herk_f32(A_np, N, K, B)))

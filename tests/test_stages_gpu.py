@@ -155,7 +155,7 @@ def test_force_baseline_equals_fused(oracle, dev, N, K, B):
     g.end_herk()
 
 
-@pytest.mark.parametrize("mode", ["ForceDirect", "ForceBaseline"])
+@pytest.mark.parametrize("mode", ["ForceDirect", "ForceBaseline", "ForceDirectBF6"])
 def test_graph_replay_lru(oracle, dev, mode):
     """set_direct_herk_graph: the first call of a shape/pointer set is captured,
     later ones replay the graph (cutlass_gemm_api.cu:1738-1807); five distinct
@@ -163,7 +163,9 @@ def test_graph_replay_lru(oracle, dev, mode):
     import torch
     from ftd_amd import ftdx
     g = ftdx.CutlassComplexGemm()
-    g.set_herk_mode(getattr(ftdx.HerkMode, mode))
+    g.set_herk_mode(getattr(ftdx.HerkMode, mode.replace("BF6", "")))
+    if mode.endswith("BF6"):
+        g.set_launch_config(ftdx.TILE_128x128_K64 | ftdx.TILE_BF6, 1)
     g.set_direct_herk_graph(True)
     N, K, B = 96, 128, 2
     g.init_herk(N, K, B)
