@@ -27,6 +27,10 @@ FP8_DENSE_PEAK_TFLOPS = 5033.0   # 256 CU x 2.4 GHz x 8192 FLOP/CU/clk (AMD publ
 HBM_PEAK_GBS = 8000.0
 MX_DENSE_PEAK_TFLOPS = 10066.0   # MXFP6/MXFP4 dense (AMD published), 2x the FP8 rate
 METRIC = "HERK TFLOP/s (4M-equiv) & % FP8 MFMA peak, 4096 inputs; batched vs plain"
+# HBM bytes per launch of the headline kernel (4096 inputs, 96 channels, K = 2048, FP32
+# output), per tile configuration, from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the
+# factors a 4096 MiB device copy gives those counters); summary and commands under profiles/.
+MEASURED_TRAFFIC_BYTES = {5: 11.630e9, 4: 15.518e9}
 
 
 def log(msg: str) -> None:
@@ -606,7 +610,9 @@ def main() -> int:
             "peak": FP8_DENSE_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP8_DENSE_PEAK_TFLOPS, 4),
-            "traffic": None,
+            # measured for the headline shape only; null for other shapes or configurations
+            "traffic": (MEASURED_TRAFFIC_BYTES.get(cfg)
+                        if (N, K, B) == (4096, 2048, 96) else None),
             "kernel": "herk_int4_fp8_plain_kernel",
             "kernel_ms": round(kern_ms, 4),
             "tile_config": cfg, "xcd_remap": remap,
