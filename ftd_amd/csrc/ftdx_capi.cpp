@@ -146,8 +146,10 @@ void pick_config(ftdx_context* ctx, const TuneKey& key, int& cfg, int& remap, bo
 // stores the fastest (cutlass_interface/strategy_cache.hpp:504-800: warm-up, then
 // event-timed repeats, failures skipped).  With beta != 0 the timing runs use a
 // scratch output so the caller's C is not accumulated into.
+// flag = TILE_BF6: the E3M2 variants, each timed with its conversion pass into the
+// context's pre-cast rows (already sized by the caller).
 int autotune(ftdx_context* ctx, const HerkPlainArgs& a0, bool out_fp16, hipStream_t s,
-             const TuneKey& key, TuneEntry& best) {
+             const TuneKey& key, TuneEntry& best, int flag) {
     HerkPlainArgs a = a0;
     void* scratch = nullptr;
     hipError_t e;
@@ -166,11 +168,22 @@ int autotune(ftdx_context* ctx, const HerkPlainArgs& a0, bool out_fp16, hipStrea
     hipEvent_t t0, t1;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
+    const int Kp = precast_row_samples(a.K);
+    HerkPlainArgs a6 = a;
+    a6.A = ctx->precast;
+    a6.precast = 2;
+    a6.lda = (int64_t)2 * Kp;
+    a6.a_bstride = (int64_t)a.N * 2 * Kp;
+    auto launch = [&](int cfg, int remap) -> hipError_t {
+        if (!(cfg & TILE_BF6)) return launch_herk_int4_plain(a, cfg, out_fp16, remap, s);
+        hipError_t e2 = launch_promote_qc_bf6_planar(a.A, ctx->precast, (int64_t)a.N * a.batch, a.K, s);
+        return e2 != hipSuccess ? e2 : launch_herk_int4_plain(a6, tile_base(cfg), out_fp16, remap, s);
+    };
     // mean ms of `reps` launches, or a negative value if the launch failed
     auto time_cand = [&](int cfg, int remap, int reps) -> float {
         (void)hipEventRecord(t0, s);
         for (int r = 0; r < reps; ++r)
-            if (launch_herk_int4_plain(a, cfg, out_fp16, remap, s) != hipSuccess) {
+            if (launch(cfg, remap) != hipSuccess) {
                 (void)hipGetLastError();
                 return -1.0f;
             }
@@ -189,8 +202,9 @@ int autotune(ftdx_context* ctx, const HerkPlainArgs& a0, bool out_fp16, hipStrea
         float ms;
     };
     std::vector<Cand> cands;
-    for (int cfg = 0; cfg < NUM_TILE_CONFIGS; ++cfg)
+    for (int base = 0; base < NUM_TILE_CONFIGS; ++base)
         for (int remap = 0; remap <= 1; ++remap) {
+            const int cfg = base | flag;
             if (time_cand(cfg, remap, 1) < 0.0f) continue;
             const float ms = time_cand(cfg, remap, 3);
             if (ms < 0.0f) continue;
@@ -199,7 +213,7 @@ int autotune(ftdx_context* ctx, const HerkPlainArgs& a0, bool out_fp16, hipStrea
             cands.push_back({cfg, remap, ms});
         }
     best.ms = 1e30f;
-    best.cfg = kDefaultCfg;
+    best.cfg = kDefaultCfg | flag;
     best.xcd_remap = kDefaultRemap;
     if (!cands.empty()) {
         // second pass: the three fastest again in mirrored order (A B C C B A), so a
@@ -666,8 +680,11 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: C must be aligned to one complex element");
     if ((int64_t)N * (N + 1) / 2 > (int64_t)1 << 40)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk: N too large");
-    // FP6/FP4 compute: the block-scaled MX path (FP16 output, as the reference)
-    if (compute != FTDX_COMPUTE_FP8)
+    // FP6 compute on INT4 input is lossless (cutlass_gemm_api.h:45: E3M2 holds every QC
+    // value): it runs the E3M2 operand tiles.  Other FP6 / FP4 cases take the block-scaled
+    // MX path (FP16 output, as the reference).
+    const bool e3m2 = compute == FTDX_COMPUTE_FP6 && input == FTDX_INPUT_INT4;
+    if (compute != FTDX_COMPUTE_FP8 && !e3m2)
         return ftdx_herk_mx(ctx, A, C, N, K, batch_count, input, compute, output, alpha, beta,
                             stream);
     if (input == FTDX_INPUT_FP16 && (reinterpret_cast<uintptr_t>(A) & 3u))
@@ -693,9 +710,11 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     hipStream_t s = static_cast<hipStream_t>(stream);
 
     ftdx::ensure_tune_loaded(ctx);
-    // Pre-cast (unfused) path: FP16 input always; INT4 input under ForceBaseline.
-    const bool precast = (input == FTDX_INPUT_FP16) || (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE);
-    const int vec = precast ? 2 : ((K % 16 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15u) == 0));
+    // Pre-cast FP8 (unfused) path: FP16 input always; INT4 input under ForceBaseline.
+    // E3M2 compute always converts first (its own tune-cache entries: vec = 3).
+    const bool precast = !e3m2 && ((input == FTDX_INPUT_FP16) || (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE));
+    const int vec = e3m2 ? 3
+                  : precast ? 2 : ((K % 16 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15u) == 0));
     ftdx::TuneKey key{N, K, batch_count, fp16 ? 1 : 0, vec};
     int cfg, remap;
     bool cached;
@@ -707,16 +726,20 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
         int rc = ftdx::grow_precast(ctx, rows_need, "ftdx_herk: scratch");
         if (rc != FTDX_OK) return rc;
     }
+    if (e3m2) {
+        int rc = ftdx::grow_precast(ctx, rows_need, "ftdx_herk: scratch");
+        if (rc != FTDX_OK) return rc;
+    }
     if (tune && !cached && !explicit_cfg && !precast) {
         ftdx::TuneEntry best;
-        if (ftdx::autotune(ctx, a, fp16, s, key, best) == FTDX_OK) {
+        if (ftdx::autotune(ctx, a, fp16, s, key, best, e3m2 ? ftdx::TILE_BF6 : 0) == FTDX_OK) {
             cfg = best.cfg;
             if (ctx->forced_remap < 0) remap = best.xcd_remap;
         }
     }
-    // INT4 input on the fused path may use the E3M2 variant of its tile when the
-    // configuration asks for it (FTDX_TILE_BF6 flag: an explicit choice or a cache entry)
-    const bool bf6 = !precast && (cfg & ftdx::TILE_BF6);
+    // E3M2 operand tiles: FP6 compute on INT4 input, or an INT4 FP8 call whose tile
+    // configuration carries the FTDX_TILE_BF6 flag (explicit choice or cache entry)
+    const bool bf6 = e3m2 || (!precast && (cfg & ftdx::TILE_BF6));
     cfg = ftdx::tile_base(cfg);
     if (bf6) {
         int rc = ftdx::grow_precast(ctx, rows_need, "ftdx_herk: scratch");

@@ -639,31 +639,45 @@ struct TileRunner {
     static constexpr bool DMA_OK =
         DSTAGE % (1024 * NWAVES) == 0 && 2 * DSTAGE <= Cfg::LDS_BYTES && DCPR * RPP == 16;
 
+    // Column rows: HERK reads them from A (pitch lda), the GEMM layouts from B (pitch ldb,
+    // Ncol rows).  Every load instruction covers 1024 / DROW whole rows of one block.
+    template <bool GEMM>
     static __device__ __forceinline__ bool dma_tile(const TileJob& j) {
-        return j.K > 0 && j.K % Cfg::KC == 0 && j.row0 + Cfg::TM <= j.N && j.col0 + Cfg::TN <= j.N &&
-               (reinterpret_cast<uintptr_t>(j.A) & 15u) == 0 && j.lda % 16 == 0 &&
-               (int64_t)j.N * j.lda <= 0xFFFFFFFFll;
+        const uint8_t* B = GEMM ? j.B : j.A;
+        const int64_t ldb = GEMM ? j.ldb : j.lda;
+        const int ncol = GEMM ? j.Ncol : j.N;
+        return j.K > 0 && j.K % Cfg::KC == 0 && j.row0 + Cfg::TM <= j.N && j.col0 + Cfg::TN <= ncol &&
+               ((reinterpret_cast<uintptr_t>(j.A) | reinterpret_cast<uintptr_t>(B)) & 15u) == 0 &&
+               j.lda % 16 == 0 && ldb % 16 == 0 && (int64_t)j.N * j.lda <= 0xFFFFFFFFll &&
+               (int64_t)ncol * ldb <= 0xFFFFFFFFll;
+    }
+    static_assert(Cfg::TM % (1024 / DROW) == 0, "a load instruction must not straddle the blocks");
+    static __device__ __forceinline__ bool dma_op_is_col(int q, int wave) {
+        return (q * NWAVES + wave) * (1024 / DROW) >= Cfg::TM;
     }
 
-    // Global byte offset (inside the problem's rows, stage 0) of each load of this lane.
+    // Global byte offset (inside its block's rows, stage 0) of each load of this lane.
+    template <bool GEMM>
     static __device__ __forceinline__ void dma_offsets(const TileJob& j, int lane, int wave,
                                                        uint32_t (&off)[DOPS]) {
+        const int64_t ldb = GEMM ? j.ldb : j.lda;
 #pragma unroll
         for (int q = 0; q < DOPS; ++q) {
             const int byte = (q * NWAVES + wave) * 1024 + lane * 16;  // position in the stage
             const int row = byte / DROW, pc = (byte % DROW) / 16;
             const int c = pc ^ swz(row);                               // logical chunk held there
-            const int grow = row < Cfg::TM ? j.row0 + row : j.col0 + (row - Cfg::TM);
-            off[q] = (uint32_t)grow * (uint32_t)j.lda + (uint32_t)(c * 16);
+            off[q] = row < Cfg::TM ? (uint32_t)(j.row0 + row) * (uint32_t)j.lda + (uint32_t)(c * 16)
+                                   : (uint32_t)(j.col0 + row - Cfg::TM) * (uint32_t)ldb + (uint32_t)(c * 16);
         }
     }
 
-    static __device__ __forceinline__ void dma_stage(const uint8_t* base, uint8_t* st, int wave,
-                                                     const uint32_t (&off)[DOPS]) {
+    // baseA / baseB: this stage's first byte of the row / column operand rows
+    static __device__ __forceinline__ void dma_stage(const uint8_t* baseA, const uint8_t* baseB,
+                                                     uint8_t* st, int wave, const uint32_t (&off)[DOPS]) {
 #pragma unroll
         for (int q = 0; q < DOPS; ++q)
             __builtin_amdgcn_global_load_lds(
-                (const void*)(base + off[q]),
+                (const void*)((dma_op_is_col(q, wave) ? baseB : baseA) + off[q]),
                 (void __attribute__((address_space(3)))*)(st + (q * NWAVES + wave) * 1024), 16, 0, 0);
     }
 
@@ -721,15 +735,19 @@ struct TileRunner {
                                                          int lane, int wave, int nst, int a_row,
                                                          int b_row, v16f (&acc_re)[Cfg::FM][Cfg::FN],
                                                          v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+        constexpr bool GEMM = FMT == OUT_GEMM_PLANAR || FMT == OUT_GEMM_POWER;
+        const uint8_t* B = GEMM ? j.B : j.A;
         uint32_t off[DOPS];
-        dma_offsets(j, lane, wave, off);
-        dma_stage(j.A, lds, wave, off);
+        dma_offsets<GEMM>(j, lane, wave, off);
+        dma_stage(j.A, B, lds, wave, off);
         for (int s = 0; s < nst; ++s) {
             wait_vmcnt<0>();  // this wave's loads of stage s landed
             __syncthreads();
             uint8_t* cur = lds + (s & 1) * DSTAGE;
-            if (s + 1 < nst)
-                dma_stage(j.A + (int64_t)(s + 1) * 2 * Cfg::KC, lds + ((s + 1) & 1) * DSTAGE, wave, off);
+            if (s + 1 < nst) {
+                const int64_t k1 = (int64_t)(s + 1) * 2 * Cfg::KC;
+                dma_stage(j.A + k1, B + k1, lds + ((s + 1) & 1) * DSTAGE, wave, off);
+            }
             if constexpr (ACTIVE) mfma_stage_dma<FMT, OPF>(cur, a_row, b_row, lane, acc_re, acc_im);
         }
         __syncthreads();
@@ -803,8 +821,8 @@ struct TileRunner {
                 looped = true;
             }
         }
-        if constexpr (PRE && !GEMM && DMA_OK) {
-            if (dma_tile(j)) {
+        if constexpr (PRE && DMA_OK) {
+            if (dma_tile<GEMM>(j)) {
                 const int a_row = wm * Cfg::WM + (lane & 31);
                 const int b_row = (share_cols ? (j.col0 - j.row0) : Cfg::TM) + wn * Cfg::WN + (lane & 31);
                 if (active)

@@ -25,7 +25,9 @@ def _dev(dev, *arrs):
     return [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in arrs]
 
 
-SHAPES = [(64, 128, 96, 2), (100, 130, 50, 3), (1, 1, 1, 1), (257, 33, 300, 1), (128, 256, 2048, 2)]
+# the last three hold interior tiles (direct global-to-LDS loads) of the 128- and 256-row tiles
+SHAPES = [(64, 128, 96, 2), (100, 130, 50, 3), (1, 1, 1, 1), (257, 33, 300, 1), (128, 256, 2048, 2),
+          (256, 384, 256, 2), (300, 200, 192, 1)]
 
 
 @pytest.mark.parametrize("M,N,K,B", SHAPES)
@@ -49,10 +51,10 @@ def test_prepared_direct_integer_bit_exact(oracle, dev, M, N, K, B):
 
 
 @pytest.mark.parametrize("output", ["fp32", "fp16"])
-def test_gemm_prepared_outputs_and_alpha_beta(oracle, dev, output):
+@pytest.mark.parametrize("M,N,K,B", [(96, 80, 64, 2), (256, 256, 128, 1)])
+def test_gemm_prepared_outputs_and_alpha_beta(oracle, dev, output, M, N, K, B):
     import torch
     from ftd_amd import ftdx
-    M, N, K, B = 96, 80, 64, 2
     rng = np.random.default_rng(9)
     a_re, a_im = _int_planes(rng, (B, M, K))
     b_re, b_im = _int_planes(rng, (B, N, K))
@@ -75,7 +77,7 @@ def test_gemm_prepared_outputs_and_alpha_beta(oracle, dev, output):
             np.float16 if output == "fp16" else np.float32).astype(np.float32))
 
 
-@pytest.mark.parametrize("M,N,K,B", [(64, 128, 96, 2), (100, 130, 50, 3)])
+@pytest.mark.parametrize("M,N,K,B", [(64, 128, 96, 2), (100, 130, 50, 3), (256, 256, 128, 2)])
 def test_prepared_int4_direct_and_power(oracle, dev, M, N, K, B):
     import torch
     from ftd_amd import ftdx

@@ -153,3 +153,25 @@ def test_herk_dispatches_mx_and_keeps_reference_rules(oracle, dev):
     with pytest.raises(ftdx.UnsupportedError):
         g.herk(X, C32, N, K, B, ftdx.InputPrecision.FP16, ftdx.ComputePrecision.FP4,
                ftdx.OutputPrecision.FP32)
+
+
+@pytest.mark.parametrize("N,K,B", [(300, 256, 2), (129, 70, 3)])
+@pytest.mark.parametrize("tune", [False, True])
+def test_herk_int4_fp6_is_lossless(oracle, dev, N, K, B, tune):
+    """INT4 input with FP6 compute (lossless in the reference's E3M2 convention,
+    cutlass_gemm_api.h:45) runs the E3M2 operand tiles: bit-exact FP16 output with
+    alpha and beta, through the autotuner too."""
+    import torch
+    from ftd_amd import ftdx
+    A = oracle.gen_qc_uniform(N + K + B, N * K * B)
+    rng = np.random.default_rng(N)
+    C0 = rng.integers(-50, 50, size=(B, N * (N + 1) // 2, 2)).astype(np.float16)
+    g = ftdx.CutlassComplexGemm()
+    g.set_strategy_tune_verbosity(0)
+    C = torch.from_numpy(C0.copy()).to(dev)
+    assert g.herk(torch.from_numpy(A).to(dev), C, N, K, B, ftdx.InputPrecision.INT4,
+                  ftdx.ComputePrecision.FP6, ftdx.OutputPrecision.FP16, alpha=0.5, beta=2.0,
+                  tune=tune) == 0
+    torch.cuda.synchronize()
+    ref = oracle.herk_f16(A, N, K, B, alpha=0.5, beta=2.0, C=C0)
+    assert np.array_equal(C.cpu().numpy().view(np.uint16), ref)
