@@ -65,11 +65,17 @@ struct ftdx_context {
     hipStream_t cap_stream = nullptr;
     // complex GEMM: prepared conj(B) rows (prepare_b) and a separate buffer for the
     // one-shot gemm(), so it never clobbers a prepared B
+    // (FP8 compute: pre-cast rows; FP6 / FP4 compute: MX codes in *_b, scales in *_bs)
     uint8_t* prep_b = nullptr;
     size_t prep_b_cap = 0;
+    uint8_t* prep_bs = nullptr;
+    size_t prep_bs_cap = 0;
     int prep_N = -1, prep_K = -1, prep_batch = -1;
+    int prep_fmt = -1;  // -1: FP8 rows, else the MxFormat of the codes
     uint8_t* gemm_b = nullptr;
     size_t gemm_b_cap = 0;
+    uint8_t* gemm_bs = nullptr;
+    size_t gemm_bs_cap = 0;
 };
 
 namespace ftdx {
@@ -328,9 +334,10 @@ int gemm_run(ftdx_context* ctx, const uint8_t* brows, GemmInput kind, const void
     g.b_bstride = (int64_t)N * Kp * 2;
     g.ldc = N;
     g.c_bstride = (int64_t)M * N;
-    // 256-row tiles once M fills them: each column block of B is then read once
-    const int cfg = ctx->forced_cfg >= 0 ? tile_base(ctx->forced_cfg)
-                                         : (M >= 192 ? TILE_256x128_K64 : TILE_128x128_K64);
+    // 128x128 tiles, two workgroups per CU: fastest at the beamformer shape (M = 256,
+    // N = 4000, K = 1664: 1.01 ms against 1.08 ms for 256-row tiles, which read each
+    // column block of B once but run one workgroup per CU)
+    const int cfg = ctx->forced_cfg >= 0 ? tile_base(ctx->forced_cfg) : TILE_128x128_K64;
     const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : kDefaultRemap;
     e = launch_gemm_fp8(g, cfg, fmt, out_fp16, remap, s);
     return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
@@ -370,10 +377,12 @@ int ftdx_create(ftdx_context** out) {
 int ftdx_destroy(ftdx_context* ctx) {
     if (!ctx) return FTDX_OK;
     ftdx_end_herk(ctx);
-    if (ctx->prep_b || ctx->gemm_b) {
+    if (ctx->prep_b || ctx->gemm_b || ctx->prep_bs || ctx->gemm_bs) {
         (void)hipDeviceSynchronize();
         if (ctx->prep_b) (void)hipFree(ctx->prep_b);
         if (ctx->gemm_b) (void)hipFree(ctx->gemm_b);
+        if (ctx->prep_bs) (void)hipFree(ctx->prep_bs);
+        if (ctx->gemm_bs) (void)hipFree(ctx->gemm_bs);
     }
     delete ctx;
     return FTDX_OK;
@@ -988,16 +997,30 @@ int ftdx_pillbox_grid(const void* vis_tri, int vis_output, const float* baseline
 // ---- complex GEMM with prepared B ----
 namespace {
 
-int prepare_rows(ftdx_context* ctx, uint8_t*& buf, size_t& cap, const void* B_re, const void* B_im,
-                 int N, int K, int batch, int compute, void* stream, const char* what) {
+// FP8: conj(B) as pre-cast rows (fmt = -1).  FP6 / FP4: MX codes and scales of conj(B)
+// (fmt = the MxFormat), the block-scaled path of cutlass_gemm_api.h:40-47.
+int prepare_rows(ftdx_context* ctx, uint8_t*& buf, size_t& cap, uint8_t*& sbuf, size_t& scap,
+                 const void* B_re, const void* B_im, int N, int K, int batch, int compute, void* stream,
+                 const char* what, int& fmt) {
     const std::string w(what);
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, w + ": NULL context");
     if (N < 0 || K < 0 || batch < 0) return fail(FTDX_ERR_INVALID_ARG, w + ": negative size");
-    if (compute != FTDX_COMPUTE_FP8)
-        return fail(FTDX_ERR_UNSUPPORTED,
-                    w + ": only ComputePrecision::FP8 is supported for the complex GEMM");
+    if (compute < FTDX_COMPUTE_FP8 || compute > FTDX_COMPUTE_FP4)
+        return fail(FTDX_ERR_INVALID_ARG, w + ": unknown compute precision");
     if (K > 0 && (int64_t)N * batch > 0 && (!B_re || !B_im))
         return fail(FTDX_ERR_INVALID_ARG, w + ": NULL B pointer");
+    if (compute != FTDX_COMPUTE_FP8) {
+        fmt = ftdx::mx_format(compute);
+        const int nblkp = ftdx::mx_padded_blocks(K);
+        const int64_t rows = (int64_t)N * batch;
+        int rc = ftdx::ensure_scratch(buf, cap, (size_t)rows * nblkp * 2 * ftdx::mx_code_bytes(fmt) + 16, what);
+        if (rc == FTDX_OK) rc = ftdx::ensure_scratch(sbuf, scap, (size_t)rows * nblkp * 2 + 16, what);
+        if (rc != FTDX_OK) return rc;
+        hipError_t e = ftdx::launch_mx_quantize_ex(B_re, B_im, ftdx::MX_IN_FP16_PLANAR, fmt, true, buf, sbuf,
+                                                   rows, K, static_cast<hipStream_t>(stream));
+        return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
+    }
+    fmt = -1;
     const int Kp = ftdx::precast_row_samples(K);
     const size_t need = (size_t)N * batch * Kp * 2 + 16;
     int rc = ftdx::ensure_scratch(buf, cap, need, what);
@@ -1009,12 +1032,58 @@ int prepare_rows(ftdx_context* ctx, uint8_t*& buf, size_t& cap, const void* B_re
     return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
 }
 
-int check_prepared(ftdx_context* ctx, int N, int K, int batch, const char* what) {
+int check_prepared(ftdx_context* ctx, int N, int K, int batch, const char* what, bool fp8_only = false) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, std::string(what) + ": NULL context");
     if (!ctx->prep_b || ctx->prep_N != N || ctx->prep_K != K || ctx->prep_batch != batch)
         return fail(FTDX_ERR_INVALID_ARG,
                     std::string(what) + ": prepare_b() must have been called with matching N, K, batch");
+    // the direct and power variants need FP8-prepared B (cutlass_gemm_api.h:319, 346, 374, 396)
+    if (fp8_only && ctx->prep_fmt >= 0)
+        return fail(FTDX_ERR_UNSUPPORTED,
+                    std::string(what) + ": prepare_b() must have been called with ComputePrecision::FP8");
     return FTDX_OK;
+}
+
+// Block-scaled complex GEMM: quantize A (FP16 planar or INT4) into the MX scratch, then
+// C = alpha A B^T + beta C against the MX codes of conj(B).
+int gemm_mx_run(ftdx_context* ctx, const uint8_t* bcodes, const uint8_t* bscales, int fmt, bool int4,
+                const void* a0, const void* a1, void* c0, void* c1, int M, int N, int K, int batch,
+                bool out_fp16, float alpha, float beta, void* stream, const char* what) {
+    const std::string w(what);
+    if (M < 0) return fail(FTDX_ERR_INVALID_ARG, w + ": M must be >= 0");
+    if (M == 0 || N == 0 || batch == 0) return FTDX_OK;
+    if (!c0 || !c1) return fail(FTDX_ERR_INVALID_ARG, w + ": NULL output pointer");
+    if (K > 0 && (!a0 || (!int4 && !a1))) return fail(FTDX_ERR_INVALID_ARG, w + ": NULL A pointer");
+    if ((int64_t)M * N > ((int64_t)1 << 40) || (int64_t)M * K > ((int64_t)1 << 40))
+        return fail(FTDX_ERR_INVALID_ARG, w + ": sizes too large");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int nblkp = ftdx::mx_padded_blocks(K);
+    const int64_t rows = (int64_t)M * batch;
+    int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap,
+                                  (size_t)rows * nblkp * 2 * ftdx::mx_code_bytes(fmt) + 16, what);
+    if (rc == FTDX_OK)
+        rc = ftdx::ensure_scratch(ctx->mx_scales, ctx->mx_scales_cap, (size_t)rows * nblkp * 2 + 16, what);
+    if (rc != FTDX_OK) return rc;
+    hipError_t e = ftdx::launch_mx_quantize_ex(a0, a1, int4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16_PLANAR,
+                                               fmt, false, ctx->mx_codes, ctx->mx_scales, rows, K, s);
+    if (e != hipSuccess) return hip_fail(e, what);
+    ftdx::MxGemmArgs g;
+    g.acodes = ctx->mx_codes;
+    g.ascales = ctx->mx_scales;
+    g.bcodes = bcodes;
+    g.bscales = bscales;
+    g.Cre = c0;
+    g.Cim = c1;
+    g.M = M;
+    g.N = N;
+    g.batch = batch;
+    g.nblkp = nblkp;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.ldc = N;
+    g.c_bstride = (int64_t)M * N;
+    e = ftdx::launch_gemm_mx(g, fmt, out_fp16, s);
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
 }
 
 }  // namespace
@@ -1022,12 +1091,14 @@ int check_prepared(ftdx_context* ctx, int N, int K, int batch, const char* what)
 int ftdx_prepare_b(ftdx_context* ctx, const void* B_re, const void* B_im, int N, int K, int batch_count,
                    int compute, void* stream) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_prepare_b: NULL context");
-    int rc = prepare_rows(ctx, ctx->prep_b, ctx->prep_b_cap, B_re, B_im, N, K, batch_count, compute, stream,
-                          "ftdx_prepare_b");
+    int fmt = -1;
+    int rc = prepare_rows(ctx, ctx->prep_b, ctx->prep_b_cap, ctx->prep_bs, ctx->prep_bs_cap, B_re, B_im, N,
+                          K, batch_count, compute, stream, "ftdx_prepare_b", fmt);
     if (rc != FTDX_OK) {
         ctx->prep_N = ctx->prep_K = ctx->prep_batch = -1;
         return rc;
     }
+    ctx->prep_fmt = fmt;
     ctx->prep_N = N;
     ctx->prep_K = K;
     ctx->prep_batch = batch_count;
@@ -1044,6 +1115,10 @@ int ftdx_gemm_prepared(ftdx_context* ctx, const void* A_re, const void* A_im, vo
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm_prepared: output must be FP32 or FP16");
     if (input != FTDX_INPUT_FP16 && input != FTDX_INPUT_INT4)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm_prepared: input must be FP16 or INT4");
+    if (ctx->prep_fmt >= 0)
+        return gemm_mx_run(ctx, ctx->prep_b, ctx->prep_bs, ctx->prep_fmt, input == FTDX_INPUT_INT4, A_re, A_im,
+                           C_re, C_im, M, N, K, batch_count, output == FTDX_OUTPUT_FP16, alpha, beta, stream,
+                           "ftdx_gemm_prepared");
     return ftdx::gemm_run(ctx, ctx->prep_b, input == FTDX_INPUT_INT4 ? ftdx::GEMM_A_INT4 : ftdx::GEMM_A_FP16_PLANAR,
                           A_re, A_im, C_re, C_im, M, N, K, batch_count, ftdx::OUT_GEMM_PLANAR,
                           output == FTDX_OUTPUT_FP16, alpha, beta, stream, "ftdx_gemm_prepared");
@@ -1052,7 +1127,7 @@ int ftdx_gemm_prepared(ftdx_context* ctx, const void* A_re, const void* A_im, vo
 int ftdx_gemm_prepared_direct(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_re,
                               void* C_im, int M, int N, int K, int batch_count, float alpha, float beta,
                               void* stream) {
-    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_direct");
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_direct", true);
     if (rc != FTDX_OK) return rc;
     return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_FP16_PLANAR, A_re, A_im, C_re, C_im, M, N, K,
                           batch_count, ftdx::OUT_GEMM_PLANAR, false, alpha, beta, stream,
@@ -1061,7 +1136,7 @@ int ftdx_gemm_prepared_direct(ftdx_context* ctx, const void* A_re, const void* A
 
 int ftdx_gemm_prepared_direct_int4(ftdx_context* ctx, const void* A_int4, void* C_re, void* C_im, int M,
                                    int N, int K, int batch_count, float alpha, float beta, void* stream) {
-    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_direct_int4");
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_direct_int4", true);
     if (rc != FTDX_OK) return rc;
     return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_INT4, A_int4, nullptr, C_re, C_im, M, N, K,
                           batch_count, ftdx::OUT_GEMM_PLANAR, false, alpha, beta, stream,
@@ -1070,7 +1145,7 @@ int ftdx_gemm_prepared_direct_int4(ftdx_context* ctx, const void* A_int4, void* 
 
 int ftdx_gemm_prepared_power(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_power, int M,
                              int N, int K, int batch_count, float alpha, float beta, void* stream) {
-    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power");
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power", true);
     if (rc != FTDX_OK) return rc;
     return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_FP16_PLANAR, A_re, A_im, C_power, nullptr, M, N,
                           K, batch_count, ftdx::OUT_GEMM_POWER, false, alpha, beta, stream,
@@ -1079,7 +1154,7 @@ int ftdx_gemm_prepared_power(ftdx_context* ctx, const void* A_re, const void* A_
 
 int ftdx_gemm_prepared_power_fp8(ftdx_context* ctx, const void* A_fp8_interleaved, void* C_power, int M,
                                  int N, int K, int batch_count, float alpha, float beta, void* stream) {
-    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power_fp8");
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power_fp8", true);
     if (rc != FTDX_OK) return rc;
     return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_FP8_INTERLEAVED, A_fp8_interleaved, nullptr,
                           C_power, nullptr, M, N, K, batch_count, ftdx::OUT_GEMM_POWER, false, alpha, beta,
@@ -1088,7 +1163,7 @@ int ftdx_gemm_prepared_power_fp8(ftdx_context* ctx, const void* A_fp8_interleave
 
 int ftdx_gemm_prepared_power_int4(ftdx_context* ctx, const void* A_int4, void* C_power, int M, int N,
                                   int K, int batch_count, float alpha, float beta, void* stream) {
-    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power_int4");
+    int rc = check_prepared(ctx, N, K, batch_count, "ftdx_gemm_prepared_power_int4", true);
     if (rc != FTDX_OK) return rc;
     return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_INT4, A_int4, nullptr, C_power, nullptr, M, N, K,
                           batch_count, ftdx::OUT_GEMM_POWER, false, alpha, beta, stream,
@@ -1104,9 +1179,13 @@ int ftdx_gemm(ftdx_context* ctx, const void* A_re, const void* A_im, const void*
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm: output must be FP32 or FP16");
     if (input != FTDX_INPUT_FP16 && input != FTDX_INPUT_INT4)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm: input must be FP16 or INT4");
-    int rc = prepare_rows(ctx, ctx->gemm_b, ctx->gemm_b_cap, B_re, B_im, N, K, batch_count, compute, stream,
-                          "ftdx_gemm");
+    int fmt = -1;
+    int rc = prepare_rows(ctx, ctx->gemm_b, ctx->gemm_b_cap, ctx->gemm_bs, ctx->gemm_bs_cap, B_re, B_im, N, K,
+                          batch_count, compute, stream, "ftdx_gemm", fmt);
     if (rc != FTDX_OK) return rc;
+    if (fmt >= 0)
+        return gemm_mx_run(ctx, ctx->gemm_b, ctx->gemm_bs, fmt, input == FTDX_INPUT_INT4, A_re, A_im, C_re, C_im,
+                           M, N, K, batch_count, output == FTDX_OUTPUT_FP16, alpha, beta, stream, "ftdx_gemm");
     return ftdx::gemm_run(ctx, ctx->gemm_b, input == FTDX_INPUT_INT4 ? ftdx::GEMM_A_INT4 : ftdx::GEMM_A_FP16_PLANAR,
                           A_re, A_im, C_re, C_im, M, N, K, batch_count, ftdx::OUT_GEMM_PLANAR,
                           output == FTDX_OUTPUT_FP16, alpha, beta, stream, "ftdx_gemm");

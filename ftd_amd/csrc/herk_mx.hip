@@ -11,7 +11,7 @@
 //              scratch layout  codes  [row][blk][part][PB]  (part 0 = Re, 1 = Im)
 //                              scales [row][blk][part]      (E8M0 bytes)
 //              blk padded to an even count (one LDS stage = 2 blocks).
-//   HERK:      per 32-sample block one v_mfma_scale_f32_32x32x64_f8f6f4 step,
+//   HERK / GEMM: per 32-sample block one v_mfma_scale_f32_32x32x64_f8f6f4 step,
 //              lanes 0-31 carry the Re block, lanes 32-63 the Im block of a row:
 //                 Re C_ij = sum (re_i re_j + im_i im_j)     A = [re | im],  B = [re | im]
 //                 Im C_ij = sum (im_i re_j - re_i im_j)     A = [im | -re], B = [re | im]
@@ -107,11 +107,15 @@ __device__ __forceinline__ float qc_nibble_value(uint32_t n) {
 // One thread per (row, 32-sample block): reads 32 complex samples, writes the
 // Re and Im code blocks and their two scale bytes.  Blocks past K (and the
 // padding block) quantize as zeros with scale 0.
+// MX_IN_FP16_PLANAR: X holds the Re halves, X2 the Im halves, both [rows][K].  conj:
+// the Im codes are stored negated (sign bit flipped after encoding, which is exact
+// since the encoding is symmetric), i.e. the codes of conj(X).
 template <int FMT, int IN>
 __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict__ X,
+                                                          const void* __restrict__ X2,
                                                           uint8_t* __restrict__ codes,
                                                           uint8_t* __restrict__ scales,
-                                                          int64_t rows, int K, int nblkp) {
+                                                          int64_t rows, int K, int nblkp, int conj) {
     using F = MxFmt<FMT>;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= rows * nblkp) return;
@@ -127,6 +131,9 @@ __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict
                 const uint32_t b = static_cast<const uint8_t*>(X)[row * K + k];
                 re = qc_nibble_value(b >> 4);
                 im = qc_nibble_value(b & 15u);
+            } else if (IN == MX_IN_FP16_PLANAR) {
+                re = (float)static_cast<const _Float16*>(X)[row * K + k];
+                im = (float)static_cast<const _Float16*>(X2)[row * K + k];
             } else {
                 const _Float16* h = static_cast<const _Float16*>(X) + (row * K + k) * 2;
                 re = (float)h[0];
@@ -157,7 +164,7 @@ __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict
         }
         uint32_t* dst = reinterpret_cast<uint32_t*>(codes + ((row * nblkp + blk) * 2 + part) * F::PB);
 #pragma unroll
-        for (int d = 0; d < F::DW; ++d) dst[d] = w[d];
+        for (int d = 0; d < F::DW; ++d) dst[d] = (part == 1 && conj) ? w[d] ^ F::neg(d) : w[d];
         scales[(row * nblkp + blk) * 2 + part] = (uint8_t)se;
     }
 }
@@ -355,17 +362,184 @@ herk_mx_kernel(const uint8_t* __restrict__ codes, const uint8_t* __restrict__ sc
         }
 }
 
+// ---------------------------------------------------------------- complex GEMM
+// C = alpha A B^T + beta C (planar Re / Im outputs, row pitch ldc) on MX codes: the row
+// block from A's codes, the column block from the codes of conj(B), so the HERK
+// operand algebra above gives Re = sum(ar br - ai bi), Im = sum(ai br + ar bi).  Full
+// grid of 128x128 tiles, column-major tile order within a batch.
+template <int FMT, typename OutT>
+__global__ void __launch_bounds__(MX_THREADS, FMT == MX_FP4_E2M1 ? 2 : 1)
+gemm_mx_kernel(MxGemmArgs p, int tiles_m, int tiles) {
+    using Cfg = MxCfg<FMT>;
+    using F = MxFmt<FMT>;
+    __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
+
+    const int w = (int)blockIdx.x;
+    const int b = w / tiles;
+    const int t = w - b * tiles;
+    const int J = t / tiles_m, I = t - J * tiles_m;
+    const int row0 = I * MX_TILE, col0 = J * MX_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int wrow0 = row0 + wm * 64, wcol0 = col0 + wn * 64;
+    const bool active = wrow0 < p.M && wcol0 < p.N;
+    const int nblkp = p.nblkp;
+
+    const int64_t row_bytes = (int64_t)nblkp * 2 * F::PB;
+    const uint8_t* ac = p.acodes + (int64_t)b * p.M * row_bytes;
+    const uint8_t* as = p.ascales + (int64_t)b * p.M * nblkp * 2;
+    const uint8_t* bc = p.bcodes + (int64_t)b * p.N * row_bytes;
+    const uint8_t* bs = p.bscales + (int64_t)b * p.N * nblkp * 2;
+
+    v16f acc_re[2][2], acc_im[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                acc_re[x][y][r] = 0.0f;
+                acc_im[x][y][r] = 0.0f;
+            }
+
+    const int nst = nblkp / MX_BLK_PER_STAGE;
+    uint4 g[Cfg::SEGS_PER_THREAD];
+    uint32_t gs;
+    auto load = [&](int s) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS_PER_THREAD; ++q) {
+            const int seg = tid + MX_THREADS * q;
+            const int lrow = seg / Cfg::SEGS_PER_ROW, sg = seg % Cfg::SEGS_PER_ROW;
+            const bool is_col = lrow >= MX_TILE;
+            const int grow = is_col ? col0 + (lrow - MX_TILE) : row0 + lrow;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (grow < (is_col ? p.N : p.M))
+                v = *reinterpret_cast<const uint4*>((is_col ? bc : ac) + grow * row_bytes +
+                                                     (int64_t)s * Cfg::ROW_DATA + sg * 16);
+            g[q] = v;
+        }
+        {
+            const bool is_col = tid >= MX_TILE;
+            const int grow = is_col ? col0 + (tid - MX_TILE) : row0 + tid;
+            gs = 0u;
+            if (grow < (is_col ? p.N : p.M))
+                gs = *reinterpret_cast<const uint32_t*>((is_col ? bs : as) + (int64_t)grow * nblkp * 2 + s * 4);
+        }
+    };
+    auto store = [&](uint8_t* st) {
+#pragma unroll
+        for (int q = 0; q < Cfg::SEGS_PER_THREAD; ++q) {
+            const int seg = tid + MX_THREADS * q;
+            const int lrow = seg / Cfg::SEGS_PER_ROW, sg = seg % Cfg::SEGS_PER_ROW;
+            *reinterpret_cast<uint4*>(st + lrow * Cfg::PITCH + sg * 16) = g[q];
+        }
+        *reinterpret_cast<uint32_t*>(st + Cfg::CODE_BYTES + tid * 4) = gs;
+    };
+
+    if (nst > 0) {
+        load(0);
+        store(lds);
+    }
+    __syncthreads();
+
+    const int lr = lane & 31, h = lane >> 5;
+    const int a_row = wm * 64 + lr;
+    const int b_row = MX_TILE + wn * 64 + lr;
+    for (int s = 0; s < nst; ++s) {
+        const bool more = s + 1 < nst;
+        if (more) load(s + 1);
+        const uint8_t* st = lds + (s & 1) * Cfg::STAGE;
+        const uint8_t* ss = st + Cfg::CODE_BYTES;
+        if (active) {
+#pragma unroll
+            for (int blk = 0; blk < MX_BLK_PER_STAGE; ++blk) {
+                v8i bu[2];
+                int sbu[2];
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    const int c = b_row + y * 32;
+                    bu[y] = mx_read_operand<FMT>(st + c * Cfg::PITCH + (blk * 2 + h) * F::PB);
+                    sbu[y] = ss[c * 4 + blk * 2 + h];
+                }
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    const int r = a_row + x * 32;
+                    const v8i au = mx_read_operand<FMT>(st + r * Cfg::PITCH + (blk * 2 + h) * F::PB);
+                    v8i av = mx_read_operand<FMT>(st + r * Cfg::PITCH + (blk * 2 + 1 - h) * F::PB);
+                    const int sau = ss[r * 4 + blk * 2 + h];
+                    const int sav = ss[r * 4 + blk * 2 + 1 - h];
+                    if (h) {
+#pragma unroll
+                        for (int d = 0; d < F::DW; ++d) av[d] ^= (int)F::neg(d);
+                    }
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) {
+                        acc_re[x][y] = mx_mfma<FMT>(au, bu[y], acc_re[x][y], sau, sbu[y]);
+                        acc_im[x][y] = mx_mfma<FMT>(av, bu[y], acc_im[x][y], sav, sbu[y]);
+                    }
+                }
+            }
+        }
+        if (more) store(lds + ((s + 1) & 1) * Cfg::STAGE);
+        __syncthreads();
+    }
+
+    if (!active) return;
+    OutT* Cr = static_cast<OutT*>(p.Cre) + (int64_t)b * p.c_bstride;
+    OutT* Ci = static_cast<OutT*>(p.Cim) + (int64_t)b * p.c_bstride;
+    const int n = lane & 31, g4 = (lane >> 5) * 4;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int gj = wcol0 + y * 32 + n;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int gi = wrow0 + x * 32 + (r >> 2) * 8 + g4 + (r & 3);
+                if (gi < p.M && gj < p.N) {
+                    const int64_t idx = (int64_t)gi * p.ldc + gj;
+                    float re = acc_re[x][y][r] * p.alpha, im = acc_im[x][y][r] * p.alpha;
+                    if (p.beta != 0.0f) {
+                        re = __builtin_fmaf(p.beta, (float)Cr[idx], re);
+                        im = __builtin_fmaf(p.beta, (float)Ci[idx], im);
+                    }
+                    __builtin_nontemporal_store((OutT)re, Cr + idx);
+                    __builtin_nontemporal_store((OutT)im, Ci + idx);
+                }
+            }
+        }
+}
+
 template <int FMT>
-hipError_t quantize_fmt(const void* X, int input, uint8_t* codes, uint8_t* scales, int64_t rows,
-                        int K, int nblkp, hipStream_t s) {
+hipError_t quantize_fmt(const void* X, const void* X2, int input, bool conj, uint8_t* codes,
+                        uint8_t* scales, int64_t rows, int K, int nblkp, hipStream_t s) {
     const int64_t total = rows * nblkp;
     const unsigned blocks = (unsigned)((total + 255) / 256);
+    const int cj = conj ? 1 : 0;
     if (input == MX_IN_INT4)
-        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_INT4>), dim3(blocks), dim3(256), 0, s, X,
-                           codes, scales, rows, K, nblkp);
+        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_INT4>), dim3(blocks), dim3(256), 0, s, X, X2,
+                           codes, scales, rows, K, nblkp, cj);
+    else if (input == MX_IN_FP16_PLANAR)
+        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_FP16_PLANAR>), dim3(blocks), dim3(256), 0, s,
+                           X, X2, codes, scales, rows, K, nblkp, cj);
     else
-        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_FP16>), dim3(blocks), dim3(256), 0, s, X,
-                           codes, scales, rows, K, nblkp);
+        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_FP16>), dim3(blocks), dim3(256), 0, s, X, X2,
+                           codes, scales, rows, K, nblkp, cj);
+    return hipGetLastError();
+}
+
+template <int FMT>
+hipError_t gemm_fmt(const MxGemmArgs& a, bool out_fp16, hipStream_t s) {
+    const int tiles_m = (a.M + MX_TILE - 1) / MX_TILE, tiles_n = (a.N + MX_TILE - 1) / MX_TILE;
+    const int64_t tiles = (int64_t)tiles_m * tiles_n;
+    const int64_t blocks = tiles * a.batch;
+    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+    if (out_fp16)
+        hipLaunchKernelGGL((gemm_mx_kernel<FMT, _Float16>), dim3((unsigned)blocks), dim3(MX_THREADS), 0,
+                           s, a, tiles_m, (int)tiles);
+    else
+        hipLaunchKernelGGL((gemm_mx_kernel<FMT, float>), dim3((unsigned)blocks), dim3(MX_THREADS), 0, s,
+                           a, tiles_m, (int)tiles);
     return hipGetLastError();
 }
 
@@ -395,13 +569,27 @@ int mx_padded_blocks(int K) {
     return (nblk + MX_BLK_PER_STAGE - 1) / MX_BLK_PER_STAGE * MX_BLK_PER_STAGE;
 }
 
-hipError_t launch_mx_quantize(const void* X, int input, int fmt, uint8_t* codes, uint8_t* scales,
-                              int64_t rows, int K, hipStream_t s) {
+hipError_t launch_mx_quantize_ex(const void* X, const void* X2, int input, int fmt, bool conj,
+                                 uint8_t* codes, uint8_t* scales, int64_t rows, int K, hipStream_t s) {
     if (rows <= 0) return hipSuccess;
     const int nblkp = mx_padded_blocks(K);
     if (nblkp == 0) return hipSuccess;
-    if (fmt == MX_FP4_E2M1) return quantize_fmt<MX_FP4_E2M1>(X, input, codes, scales, rows, K, nblkp, s);
-    if (fmt == MX_FP6_E3M2) return quantize_fmt<MX_FP6_E3M2>(X, input, codes, scales, rows, K, nblkp, s);
+    if (fmt == MX_FP4_E2M1)
+        return quantize_fmt<MX_FP4_E2M1>(X, X2, input, conj, codes, scales, rows, K, nblkp, s);
+    if (fmt == MX_FP6_E3M2)
+        return quantize_fmt<MX_FP6_E3M2>(X, X2, input, conj, codes, scales, rows, K, nblkp, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_mx_quantize(const void* X, int input, int fmt, uint8_t* codes, uint8_t* scales,
+                              int64_t rows, int K, hipStream_t s) {
+    return launch_mx_quantize_ex(X, nullptr, input, fmt, false, codes, scales, rows, K, s);
+}
+
+hipError_t launch_gemm_mx(const MxGemmArgs& a, int fmt, bool out_fp16, hipStream_t s) {
+    if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return hipSuccess;
+    if (fmt == MX_FP4_E2M1) return gemm_fmt<MX_FP4_E2M1>(a, out_fp16, s);
+    if (fmt == MX_FP6_E3M2) return gemm_fmt<MX_FP6_E3M2>(a, out_fp16, s);
     return hipErrorInvalidValue;
 }
 

@@ -176,8 +176,11 @@ def test_errors_and_prepared_b_isolation(oracle, dev):
     g.prepare_b(dB_re, dB_im, N, K, B)
     assert g.gemm_prepared_direct(dA_re, dA_im, out_re, out_im, M, N + 1, K, B) == \
         ftdx.FTDX_ERR_INVALID_ARG
+    # FP6 / FP4 prepare_b is the block-scaled path; the direct and power variants need
+    # FP8-prepared B (cutlass_gemm_api.h:319, 346, 374, 396)
+    g.prepare_b(dB_re, dB_im, N, K, B, compute=ftdx.ComputePrecision.FP6)
     with pytest.raises(ftdx.UnsupportedError):
-        g.prepare_b(dB_re, dB_im, N, K, B, compute=ftdx.ComputePrecision.FP6)
+        g.gemm_prepared_direct(dA_re, dA_im, out_re, out_im, M, N, K, B)
     g.prepare_b(dB_re, dB_im, N, K, B)
     # a one-shot gemm with another B leaves the prepared B untouched
     tmp_re = torch.empty((B, M, N), dtype=torch.float32, device=dev)
@@ -198,3 +201,86 @@ def test_errors_and_prepared_b_isolation(oracle, dev):
     assert g.gemm_prepared_direct(dA_re, dA_im, out_re, out_im, M, N, 0, B, alpha=1.0, beta=0.5) == 0
     torch.cuda.synchronize()
     assert torch.all(out_re == 2.0) and torch.all(out_im == -1.0)
+
+
+def _gauss_planes(rng, shape, scale):
+    return ((rng.standard_normal(shape) * scale).astype(np.float16),
+            (rng.standard_normal(shape) * scale).astype(np.float16))
+
+
+def _mx_dequant(oracle, re, im, fmt):
+    """FP64 values the MFMA sees after MX quantization of [batch, rows, K] planes."""
+    Bt, R, K = re.shape
+    X = np.stack([re, im], axis=-1).astype(np.float32).reshape(Bt * R, K, 2)
+    codes, scales = oracle.mx_quantize(X, fmt)
+    return oracle.mx_dequantize(codes, scales, fmt, K).reshape(Bt, R, K)
+
+
+@pytest.mark.parametrize("compute", ["FP6", "FP4"])
+@pytest.mark.parametrize("output", ["fp32", "fp16"])
+@pytest.mark.parametrize("M,N,K,B", [(96, 130, 200, 2), (256, 256, 256, 1)])
+def test_mx_gemm_prepared_matches_fp64_of_quantized(oracle, dev, compute, output, M, N, K, B):
+    """FP6 / FP4 prepare_b + gemm_prepared (block-scaled, cutlass_gemm_api.h:40-47, 215-299):
+    the product equals the FP64 GEMM of the MX-dequantized A and B, to the MFMA's
+    accumulation (same bounds as the MX HERK), with alpha and beta applied."""
+    import torch
+    from ftd_amd import ftdx
+    fmt = 0 if compute == "FP4" else 1
+    rng = np.random.default_rng(M + N + K + fmt)
+    a_re, a_im = _gauss_planes(rng, (B, M, K), 1.0)
+    b_re, b_im = _gauss_planes(rng, (B, N, K), 0.3)
+    dA_re, dA_im, dB_re, dB_im = _dev(dev, a_re, a_im, b_re, b_im)
+    dt = torch.float32 if output == "fp32" else torch.float16
+    old_re = rng.integers(-4, 4, (B, M, N)).astype(np.float32)
+    old_im = rng.integers(-4, 4, (B, M, N)).astype(np.float32)
+    C_re = torch.from_numpy(old_re).to(dev).to(dt)
+    C_im = torch.from_numpy(old_im).to(dev).to(dt)
+    g = ftdx.CutlassComplexGemm()
+    g.prepare_b(dB_re, dB_im, N, K, B, compute=getattr(ftdx.ComputePrecision, compute))
+    op = ftdx.OutputPrecision.FP32 if output == "fp32" else ftdx.OutputPrecision.FP16
+    print(f"mx gemm {compute} {output} M={M} N={N} K={K} B={B}", flush=True)
+    assert g.gemm_prepared(dA_re, dA_im, C_re, C_im, M, N, K, B, output=op, alpha=0.5, beta=2.0) == 0
+    torch.cuda.synchronize()
+    Aq = _mx_dequant(oracle, a_re, a_im, fmt)
+    Bq = _mx_dequant(oracle, b_re, b_im, fmt)
+    ref = 0.5 * np.einsum("bmk,bnk->bmn", Aq, Bq) + 2.0 * (old_re + 1j * old_im)
+    got = C_re.double().cpu().numpy() + 1j * C_im.double().cpu().numpy()
+    scale = np.abs(ref).max()
+    tol = {("fp32", "FP4"): 2e-6, ("fp32", "FP6"): 5e-4}.get((output, compute), 2e-3)
+    assert np.abs(got - ref).max() <= tol * scale, np.abs(got - ref).max() / scale
+
+
+def test_mx_gemm_int4_a_and_one_shot(oracle, dev):
+    """INT4 A is exact in FP6 E3M2; B is block-scaled.  The one-shot gemm() with FP4
+    compute matches its prepared form and leaves a prepared FP6 B untouched."""
+    import torch
+    from ftd_amd import ftdx
+    M, N, K, B = 128, 160, 256, 2
+    qc = oracle.gen_qc_uniform(31, M * K * B)
+    Aq = oracle.decode_qc_array(qc).reshape(B, M, K)
+    rng = np.random.default_rng(5)
+    b_re, b_im = _gauss_planes(rng, (B, N, K), 0.3)
+    c_re, c_im = _gauss_planes(rng, (B, N, K), 0.3)
+    dA, dB_re, dB_im, dC_re, dC_im = _dev(dev, qc, b_re, b_im, c_re, c_im)
+    g = ftdx.CutlassComplexGemm()
+    g.prepare_b(dB_re, dB_im, N, K, B, compute=ftdx.ComputePrecision.FP6)
+    out = [torch.empty((B, M, N), dtype=torch.float32, device=dev) for _ in range(4)]
+    assert g.gemm_prepared(dA, None, out[0], out[1], M, N, K, B, input=ftdx.InputPrecision.INT4) == 0
+    # one-shot FP4 gemm on another B (FP16 A: the decoded integers)
+    a_re = Aq.real.astype(np.float16)
+    a_im = Aq.imag.astype(np.float16)
+    dA_re, dA_im = _dev(dev, a_re, a_im)
+    assert g.gemm(dA_re, dA_im, dC_re, dC_im, out[2], out[3], M, N, K, B,
+                  compute=ftdx.ComputePrecision.FP4, output=ftdx.OutputPrecision.FP32) == 0
+    torch.cuda.synchronize()
+    ref6 = np.einsum("bmk,bnk->bmn", Aq, _mx_dequant(oracle, b_re, b_im, 1))
+    got6 = out[0].double().cpu().numpy() + 1j * out[1].double().cpu().numpy()
+    assert np.abs(got6 - ref6).max() <= 5e-4 * np.abs(ref6).max()
+    ref4 = np.einsum("bmk,bnk->bmn", _mx_dequant(oracle, a_re, a_im, 0), _mx_dequant(oracle, c_re, c_im, 0))
+    got4 = out[2].double().cpu().numpy() + 1j * out[3].double().cpu().numpy()
+    assert np.abs(got4 - ref4).max() <= 2e-6 * np.abs(ref4).max()
+    # the prepared FP6 B is still in place
+    again = [torch.empty_like(out[0]) for _ in range(2)]
+    assert g.gemm_prepared(dA, None, again[0], again[1], M, N, K, B, input=ftdx.InputPrecision.INT4) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(again[0], out[0]) and torch.equal(again[1], out[1])
