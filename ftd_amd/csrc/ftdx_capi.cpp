@@ -118,6 +118,13 @@ int hip_fail(hipError_t e, const char* what) {
                                   hipGetErrorString(e) + ")");
 }
 
+const uint8_t* prepared_fp8_rows(const ftdx_context* ctx, int N, int K, int batch) {
+    if (!ctx || !ctx->prep_b || ctx->prep_fmt >= 0 || ctx->prep_N != N || ctx->prep_K != K ||
+        ctx->prep_batch != batch)
+        return nullptr;
+    return ctx->prep_b;
+}
+
 namespace {
 
 void ensure_tune_loaded(ftdx_context* ctx) {

@@ -16,6 +16,8 @@ int hip_fail(hipError_t e, const char* what);
 #include <stddef.h>
 #include <stdint.h>
 
+typedef struct ftdx_context ftdx_context;
+
 namespace ftdx {
 
 // Tile configuration for the XEngine / FTD paths: `forced` if >= 0, else a shape
@@ -27,6 +29,9 @@ size_t rows_bytes(int N, int K, int batch);
 // (OUT_UPPER_PACKED or OUT_HERM_FULL), FP32 or FP64.
 hipError_t run_ggp_payload(const uint8_t* in, void* out, uint8_t* rows, int N, int K, int batch,
                            int fmt, bool fp64, int cfg, int remap, hipStream_t s);
+// Pre-cast conj(B) rows of a context whose prepare_b ran with FP8 compute and this
+// (N, K, batch); nullptr otherwise.
+const uint8_t* prepared_fp8_rows(const ftdx_context* ctx, int N, int K, int batch);
 // One FTD raw-order payload -> packed lower triangle FP32.
 hipError_t run_ftd_payload(const uint8_t* raw, float* tri, uint8_t* rows, int n_ant, int n_chan,
                            int n_time, int n_time_inner, int n_pol, bool twos, int cfg, int remap,

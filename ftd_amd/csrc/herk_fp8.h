@@ -36,13 +36,22 @@ int tile_config_wg_per_cu(int cfg);
 //   OUT_HERM_FULL     GGP: full M^H M row-major, (i, j) at j*N + i, conj at i*N + j
 //   OUT_GEMM_PLANAR   C = alpha A B^T + beta C, Re and Im planes [M][ldc] (B pre-cast as conj)
 //   OUT_GEMM_POWER    P = alpha |A B^T|^2 + beta P, FP32 [M][ldc]
+//   OUT_GEMM_BEAM     beamformer detect + integrate: 128-row tile = 64 samples of pol 0
+//                     then the same 64 of pol 1; out[col][m / nt] = sum over the nt
+//                     samples of (|pol 0|^2 + |pol 1|^2), quantized (uint8) or FP32
 enum OutFormat {
     OUT_LOWER_PACKED = 0,
     OUT_UPPER_PACKED = 1,
     OUT_HERM_FULL = 2,
     OUT_GEMM_PLANAR = 3,
-    OUT_GEMM_POWER = 4
+    OUT_GEMM_POWER = 4,
+    OUT_GEMM_BEAM = 5
 };
+
+// Full M x N GEMM grid with B from its own rows (not the lower triangle of A A^H).
+constexpr bool gemm_layout(int fmt) {
+    return fmt == OUT_GEMM_PLANAR || fmt == OUT_GEMM_POWER || fmt == OUT_GEMM_BEAM;
+}
 
 struct HerkPlainArgs {
     const uint8_t* A;   // [batch][N][K] QC bytes, or pre-cast FP8 rows when precast != 0
@@ -87,11 +96,17 @@ struct GemmArgs {
     float alpha, beta;
     int64_t lda, ldb, a_bstride, b_bstride;  // bytes
     int64_t ldc, c_bstride;                  // elements
+    // OUT_GEMM_BEAM: samples per output (divides 64), valid samples per batch (rows
+    // of pol 0); alpha is the quantization scale.
+    int beam_nt = 0, beam_samples = 0;
 };
 
 // fmt = OUT_GEMM_PLANAR (float or, out_fp16, half planes) or OUT_GEMM_POWER (float).
 hipError_t launch_gemm_fp8(const GemmArgs& a, int cfg, int fmt, bool out_fp16, int xcd_remap,
                            hipStream_t stream);
+// OUT_GEMM_BEAM on the 128 x 128 tile: uint8 (out_u8, rint(acc * alpha) clamped to
+// [0, 255]) or FP32 acc; M = 128 * ceil(beam_samples / 64) rows per batch.
+hipError_t launch_gemm_beam(const GemmArgs& a, bool out_u8, int xcd_remap, hipStream_t stream);
 
 // ---- grouped persistent launch ----
 struct GroupProblemDev {

@@ -189,6 +189,8 @@ gemm_fp8_kernel(GemmArgs p, int tiles_m, int tiles, int xcd_remap) {
     j.ldb = p.ldb;
     j.ldc = p.ldc;
     j.nseg = 1;
+    j.beam_nt = p.beam_nt;
+    j.beam_samples = p.beam_samples;
     TileRunner<Cfg>::template run<OutT, LOAD_FP8, FMT>(j, lds);
 }
 
@@ -210,6 +212,25 @@ hipError_t launch_gemm_cfg(const GemmArgs& a, int fmt, bool out_fp16, int xcd_re
                            tiles, xcd_remap);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_beam_cfg0(const GemmArgs& a, bool out_u8, int xcd_remap, hipStream_t stream) {
+    using Cfg = Cfg0;
+    if (a.beam_nt <= 0 || 64 % a.beam_nt != 0 || a.M != (a.beam_samples + 63) / 64 * Cfg::TM)
+        return hipErrorInvalidValue;
+    const int64_t tiles_m = a.M / Cfg::TM, tiles_n = (a.N + Cfg::TN - 1) / Cfg::TN;
+    const int64_t blocks = tiles_m * tiles_n * a.batch;
+    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+    if (blocks == 0) return hipSuccess;
+    dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    const int tm = (int)tiles_m, tiles = (int)(tiles_m * tiles_n);
+    if (out_u8)
+        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, uint8_t, OUT_GEMM_BEAM>), grid, block, 0, stream, a, tm,
+                           tiles, xcd_remap);
+    else
+        hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_BEAM>), grid, block, 0, stream, a, tm,
+                           tiles, xcd_remap);
     return hipGetLastError();
 }
 
@@ -294,6 +315,10 @@ hipError_t launch_herk_ggp(const HerkPlainArgs& a, int cfg, int fmt, bool out_fp
     return with_cfg(cfg, [&](auto c) {
         return launch_ggp_cfg<decltype(c)>(a, fmt, out_fp64, xcd_remap, stream);
     });
+}
+
+hipError_t launch_gemm_beam(const GemmArgs& a, bool out_u8, int xcd_remap, hipStream_t stream) {
+    return launch_gemm_beam_cfg0(a, out_u8, xcd_remap, stream);
 }
 
 hipError_t launch_gemm_fp8(const GemmArgs& a, int cfg, int fmt, bool out_fp16, int xcd_remap,

@@ -93,6 +93,8 @@ struct TileJob {
     int Ncol;
     void* C2;
     int64_t ldc;
+    // OUT_GEMM_BEAM: samples per output and valid samples of this problem
+    int beam_nt, beam_samples;
 };
 
 // 4 QC bytes (4 samples) -> u dwords: re = fp8(Re0..Re3), im = fp8(Im0..Im3).
@@ -497,6 +499,60 @@ struct TileRunner {
         }
     }
 
+    // Beamformer epilogue (OUT_GEMM_BEAM).  Tile rows 0-63 are samples m0 .. m0 + 63 of
+    // polarization 0 and rows 64-127 the same samples of polarization 1 (m0 = 64 *
+    // row block); columns are beams.  Each element's power fma(re, re, im * im) goes to
+    // LDS; then each output (beam, group of nt samples) is
+    //     acc = 0; for dt: acc += (P0[m] + P1[m]),   m = m0 + g * nt + dt
+    // in that order, the reference's two power GEMMs with beta = 1 for pol 1 followed by
+    // its time integration (lib/voltage_pipeline.cu:283-318, 934-967), and is written
+    // to out[beam][m / nt] as FP32 acc or as the uint8 rint(acc * scale) clamped to
+    // [0, 255] (NaN -> 0, as __float2int_rn).  Groups lie inside one tile (nt divides 64).
+    template <typename OutT>
+    static __device__ __forceinline__ void epilogue_beam(
+        const TileJob& j, uint8_t* __restrict__ lds, const v16f (&acc_re)[Cfg::FM][Cfg::FN],
+        const v16f (&acc_im)[Cfg::FM][Cfg::FN], bool active, int wrow0, int wcol0, int lane, int tid) {
+        static_assert(Cfg::TM == 128, "two 64-sample polarization halves per row block");
+        constexpr int LP = Cfg::TN + 8;  // floats per LDS row: rows r and r + 4 on disjoint banks
+        static_assert(Cfg::TM * LP * 4 <= Cfg::LDS_BYTES, "power tile exceeds LDS");
+        float* P = reinterpret_cast<float*>(lds);
+        __syncthreads();  // every wave is done with the stage buffers
+        if (active) {
+            const int n = lane & 31, g4 = (lane >> 5) * 4;
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+                for (int b = 0; b < Cfg::FN; ++b) {
+                    const int lc = wcol0 - j.col0 + b * 32 + n;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int lr = wrow0 - j.row0 + a * 32 + (r >> 2) * 8 + g4 + (r & 3);
+                        const float re = acc_re[a][b][r], im = acc_im[a][b][r];
+                        P[lr * LP + lc] = __builtin_fmaf(re, re, im * im);
+                    }
+                }
+        }
+        __syncthreads();
+        const int nt = j.beam_nt, groups = 64 / nt;
+        const int m0 = (j.row0 >> 7) * 64;
+        const int col = tid % Cfg::TN;
+        const int beam = j.col0 + col;
+        if (beam >= j.Ncol) return;
+        OutT* out = static_cast<OutT*>(j.C) + (int64_t)beam * j.ldc + m0 / nt;
+        for (int g = tid / Cfg::TN; g < groups; g += Cfg::THREADS / Cfg::TN) {
+            if (m0 + g * nt >= j.beam_samples) break;
+            const float* p = P + g * nt * LP + col;
+            float acc = 0.0f;
+            for (int dt = 0; dt < nt; ++dt) acc += p[dt * LP] + p[(dt + 64) * LP];
+            if constexpr (sizeof(OutT) == 1) {
+                const float q = __builtin_fminf(__builtin_fmaxf(__builtin_rintf(acc * j.alpha), 0.0f), 255.0f);
+                out[g] = (OutT)(int)q;
+            } else {
+                out[g] = acc;
+            }
+        }
+    }
+
     // One stage of MFMAs on the staged rows in `st` (KC samples, all 32-sample steps).
     template <int FMT, int OPF = OPF_FP8>
     static __device__ __forceinline__ void mfma_stage(const uint8_t* st, int a_off, int b_off,
@@ -516,8 +572,7 @@ struct TileRunner {
                 const v8i av = conj_op<OPF>(au[a]);
 #pragma unroll
                 for (int b = 0; b < Cfg::FN; ++b) {
-                    if constexpr (FMT == OUT_LOWER_PACKED || FMT == OUT_GEMM_PLANAR ||
-                                  FMT == OUT_GEMM_POWER) {
+                    if constexpr (FMT == OUT_LOWER_PACKED || gemm_layout(FMT)) {
                         acc_re[a][b] = mfma_32x32x64<OPF>(au[a], bu[b], acc_re[a][b]);
                         acc_im[a][b] = mfma_32x32x64<OPF>(av, bu[b], acc_im[a][b]);
                     } else {  // transposed accumulator, same values
@@ -713,8 +768,7 @@ struct TileRunner {
                 const v8i av = conj_op<OPF>(au[a]);
 #pragma unroll
                 for (int b = 0; b < Cfg::FN; ++b) {
-                    if constexpr (FMT == OUT_LOWER_PACKED || FMT == OUT_GEMM_PLANAR ||
-                                  FMT == OUT_GEMM_POWER) {
+                    if constexpr (FMT == OUT_LOWER_PACKED || gemm_layout(FMT)) {
                         acc_re[a][b] = mfma_32x32x64<OPF>(au[a], bu[b], acc_re[a][b]);
                         acc_im[a][b] = mfma_32x32x64<OPF>(av, bu[b], acc_im[a][b]);
                     } else {
@@ -735,7 +789,7 @@ struct TileRunner {
                                                          int lane, int wave, int nst, int a_row,
                                                          int b_row, v16f (&acc_re)[Cfg::FM][Cfg::FN],
                                                          v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
-        constexpr bool GEMM = FMT == OUT_GEMM_PLANAR || FMT == OUT_GEMM_POWER;
+        constexpr bool GEMM = gemm_layout(FMT);
         const uint8_t* B = GEMM ? j.B : j.A;
         uint32_t off[DOPS];
         dma_offsets<GEMM>(j, lane, wave, off);
@@ -757,7 +811,7 @@ struct TileRunner {
     // live and would spill the second register set).
     template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false>
     static __device__ __forceinline__ void run(const TileJob& j, uint8_t* __restrict__ lds) {
-        constexpr bool GEMM = FMT == OUT_GEMM_PLANAR || FMT == OUT_GEMM_POWER;
+        constexpr bool GEMM = gemm_layout(FMT);
         static_assert(!GEMM || MODE == LOAD_FP8, "GEMM layouts read pre-cast FP8 rows");
         constexpr bool PRE = MODE == LOAD_FP8 || MODE == LOAD_BF6;  // pre-cast operand rows
         constexpr int OPF = MODE == LOAD_BF6 ? OPF_BF6 : OPF_FP8;
@@ -871,7 +925,10 @@ struct TileRunner {
             }
         }
 
-        if constexpr (GEMM) {
+        if constexpr (FMT == OUT_GEMM_BEAM) {
+            epilogue_beam<OutT>(j, lds, acc_re, acc_im, active, wrow0, wcol0, lane, tid);
+            return;
+        } else if constexpr (GEMM) {
             // B rows hold conj(B), so the HERK product A conj(B)^H = A B^T; lane =
             // column in the accumulator layout: consecutive lanes store consecutive j.
             if (!active) return;
@@ -917,7 +974,7 @@ struct TileRunner {
                             put<OutT>(j, tri_index(gi, gj), re, im);
                         } else if constexpr (FMT == OUT_UPPER_PACKED) {
                             put<OutT>(j, (int64_t)gj * j.N + gi - ((int64_t)gj * (gj + 1) >> 1), re, im);
-                        } else {
+                        } else if constexpr (FMT == OUT_HERM_FULL) {
                             put<OutT>(j, (int64_t)gj * j.N + gi, re, im);
                             if (gi != gj) put<OutT>(j, (int64_t)gi * j.N + gj, re, -im);
                         }

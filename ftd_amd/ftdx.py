@@ -146,6 +146,20 @@ _SIGNATURES = {
     "ftdx_pipeline_correlate": (_I, [_VP, _VP, _VP]),
     "ftdx_pipeline_correlate_many": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I]),
     "ftdx_pipeline_destroy": (_I, [_VP]),
+    "ftdx_voltage_config_init": (_I, [_VP]),
+    "ftdx_voltage_create": (_I, [_VP, ctypes.POINTER(_VP)]),
+    "ftdx_voltage_destroy": (_I, [_VP]),
+    "ftdx_voltage_set_weights": (_I, [_VP, _VP, _VP]),
+    "ftdx_voltage_beamform": (_I, [_VP, _VP, _VP, _F, _VP]),
+    "ftdx_voltage_beamform_batched": (_I, [_VP, _VP, ctypes.POINTER(_VP), _I, _F, _VP]),
+    "ftdx_voltage_compute_filterbank": (_I, [_VP, _VP, _VP]),
+    "ftdx_voltage_compute": (_I, [_VP, _VP, _VP, _VP, _F, _VP]),
+    "ftdx_voltage_corner_turn": (_I, [_VP, _VP, _VP, _VP]),
+    "ftdx_voltage_set_graph_enabled": (_I, [_VP, _I]),
+    "ftdx_voltage_query": (_I, [_VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    "ftdx_corner_turn": (_I, [_VP, _VP, _I, _I, _I, _VP]),
 }
 
 PREC_FP32 = 1
@@ -809,6 +823,128 @@ class Pipeline:
             self.close()
         except Exception:
             pass
+
+
+# ---- voltage beamformer (ftdx_voltage_*) ----
+VOLTAGE_FP8 = 0
+VOLTAGE_FP6 = 1
+
+
+class VoltageConfig(ctypes.Structure):
+    """ftdx_voltage_config (VoltagePipeline::Config, include/voltage_pipeline.h:121-156)."""
+    _fields_ = [("struct_size", ctypes.c_size_t), ("n_antennae", ctypes.c_uint64),
+                ("n_beams", ctypes.c_uint64), ("n_channels", ctypes.c_uint64), ("n_time", ctypes.c_uint64),
+                ("n_polarizations", ctypes.c_uint64), ("n_time_power_sum", ctypes.c_uint64),
+                ("compute_mode", ctypes.c_int), ("kernel_tune_verbosity", ctypes.c_int),
+                ("strategy_tune_verbosity", ctypes.c_int), ("gemm_tune", ctypes.c_int),
+                ("kernel_tune_cache_path", ctypes.c_char_p), ("strategy_tune_cache_path", ctypes.c_char_p),
+                ("n_dm_trials", ctypes.c_int), ("f_min_MHz", ctypes.c_float), ("f_max_MHz", ctypes.c_float),
+                ("max_dm", ctypes.c_float), ("total_obs_time_s", ctypes.c_float), ("profile", ctypes.c_int),
+                ("ch_fuse", ctypes.c_uint64), ("max_payloads", ctypes.c_uint64)]
+
+
+def voltage_config(**fields) -> VoltageConfig:
+    """Config defaults (include/voltage_pipeline.h:123-156), then `fields`."""
+    c = VoltageConfig()
+    _check(lib().ftdx_voltage_config_init(ctypes.byref(c)), "voltage_config_init")
+    for k, v in fields.items():
+        setattr(c, k, v)
+    return c
+
+
+class VoltagePipeline:
+    """ggp::VoltagePipeline over ftdx_voltage_* (include/voltage_pipeline.h:49-310).
+    Device tensors: QC input uint8 [n_ch, 2, n_ant, n_time], weights float32
+    [n_ch, n_beam, n_ant, 2], uint8 output [n_ch, n_beam, n_time_out]."""
+
+    def __init__(self, config: VoltageConfig | None = None, **fields):
+        self.config = config if config is not None else voltage_config(**fields)
+        h = _VP()
+        _check(lib().ftdx_voltage_create(ctypes.byref(self.config), ctypes.byref(h)), "VoltagePipeline")
+        self.handle = h.value
+        osz, isz = ctypes.c_int64(), ctypes.c_int64()
+        nto = ctypes.c_uint64()
+        direct, graph = _I(), _I()
+        fc, fb = _VP(), _VP()
+        _check(lib().ftdx_voltage_query(self.handle, ctypes.byref(osz), ctypes.byref(isz), ctypes.byref(nto),
+                                        ctypes.byref(direct), ctypes.byref(graph), ctypes.byref(fc),
+                                        ctypes.byref(fb)), "voltage_query")
+        self.output_size, self.input_size, self.n_time_out = osz.value, isz.value, nto.value
+        self.uses_direct_path = bool(direct.value)
+        self._fb_ptrs = (fc.value, fb.value)
+
+    def _need(self, t, nbytes: int, what: str) -> None:
+        if t.numel() * t.element_size() < nbytes:
+            raise ValueError(f"VoltagePipeline: {what} holds {t.numel() * t.element_size()} bytes, "
+                             f"needs {nbytes}")
+
+    def set_weights(self, weights, stream=None) -> None:
+        c = self.config
+        self._need(weights, 8 * c.n_channels * c.n_beams * c.n_antennae, "weights")
+        _check(lib().ftdx_voltage_set_weights(self.handle, _ptr(weights), _stream_ptr(stream)), "set_weights")
+
+    def beamform(self, output, qc_input, scale: float = 1.0, stream=None) -> None:
+        self._need(output, self.output_size, "output")
+        self._need(qc_input, self.input_size, "input")
+        _check(lib().ftdx_voltage_beamform(self.handle, _ptr(output), _ptr(qc_input), scale, _stream_ptr(stream)),
+               "beamform")
+
+    def beamform_batched(self, output, qc_inputs, scale: float = 1.0, stream=None) -> None:
+        n = len(qc_inputs)
+        for x in qc_inputs:
+            self._need(x, self.input_size, "input")
+        self._need(output, self.output_size * max(n, 1), "output")
+        arr = (_VP * max(1, n))(*[_ptr(x) for x in qc_inputs])
+        _check(lib().ftdx_voltage_beamform_batched(self.handle, _ptr(output), arr, n, scale, _stream_ptr(stream)),
+               "beamform_batched")
+
+    def compute_filterbank(self, qc_input, stream=None) -> None:
+        self._need(qc_input, self.input_size, "input")
+        _check(lib().ftdx_voltage_compute_filterbank(self.handle, _ptr(qc_input), _stream_ptr(stream)),
+               "compute_filterbank")
+
+    def compute(self, output, qc_input, dedisp_out=None, scale: float = 1.0, stream=None) -> None:
+        self._need(output, self.output_size, "output")
+        self._need(qc_input, self.input_size, "input")
+        _check(lib().ftdx_voltage_compute(self.handle, _ptr(output), _ptr(qc_input), _ptr(dedisp_out), scale,
+                                          _stream_ptr(stream)), "compute")
+
+    def corner_turn(self, out, inp, stream=None) -> None:
+        _check(lib().ftdx_voltage_corner_turn(self.handle, _ptr(out), _ptr(inp), _stream_ptr(stream)),
+               "corner_turn")
+
+    def set_graph_enabled(self, enabled: bool) -> None:
+        _check(lib().ftdx_voltage_set_graph_enabled(self.handle, int(bool(enabled))), "set_graph_enabled")
+
+    def filterbank(self, which: str = "chanmajor"):
+        """Copy of the FP32 filterbank buffer ([n_ch, n_beam, n_time_out] channel-major or
+        [n_beam, n_ch, n_time_out] beam-major) as a device tensor (stream-ordered on the
+        current stream)."""
+        import torch
+        c = self.config
+        n = c.n_channels * c.n_beams * self.n_time_out
+        out = torch.empty(n, dtype=torch.float32, device="cuda")
+        src = self._fb_ptrs[0 if which == "chanmajor" else 1]
+        # a 1 x 1 corner turn of n values is a plain device copy
+        _check(lib().ftdx_corner_turn(_ptr(out), src, 1, 1, n, _stream_ptr(None)), "filterbank copy")
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().ftdx_voltage_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def corner_turn(out, inp, n_channels: int, n_beams: int, n_time: int, stream=None) -> None:
+    """ggp::corner_turn_nf_nb (lib/corner_turn.cu:63-75): [Nf, Nb, Nt] -> [Nb, Nf, Nt] FP32."""
+    _check(lib().ftdx_corner_turn(_ptr(out), _ptr(inp), n_channels, n_beams, n_time, _stream_ptr(stream)),
+           "corner_turn")
 
 
 # ---- visibility consumers: polarization sum and pillbox gridding ----

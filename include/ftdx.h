@@ -412,6 +412,93 @@ int ftdx_pipeline_correlate_many(ftdx_pipeline* pl, void* const* vis_outputs,
                                  const void* const* raw_inputs, int n_payloads);
 int ftdx_pipeline_destroy(ftdx_pipeline* pl);
 
+/* ---- voltage beamformer (ggp::VoltagePipeline, include/voltage_pipeline.h:49-310,
+ * lib/voltage_pipeline.cu:505-1338) ----
+ * QC voltages [n_channels][2 pol][n_antennae][n_time] bytes -> beam power
+ * |W V_pol0|^2 + |W V_pol1|^2 summed over n_time_power_sum samples, as uint8
+ * rint(power * scale) clamped to [0, 255], layout [n_channels][n_beams][n_time_out],
+ * or FP32 (compute_filterbank).  Weights: FP32 [re, im] [n_channels][n_beams][n_antennae],
+ * rounded to FP16 then to the compute format (the reference's set_weights).
+ * FP8 compute: one pass turns the voltages into GEMM rows; one GEMM over both
+ * polarizations detects, sums the polarizations, integrates and quantizes in its
+ * epilogue (FP32 power never reaches memory when n_time_power_sum divides 64).
+ * FP6 compute: block-scaled E3M2 GEMM to planar FP32 beams, then the power sum.
+ * All data pointers are device pointers.  Errors the reference raises through
+ * VP_ERROR return FTDX_ERR_INVALID_ARG / _UNSUPPORTED. */
+#define FTDX_VOLTAGE_FP8 0 /* VoltageComputeMode::FP8 */
+#define FTDX_VOLTAGE_FP6 1 /* VoltageComputeMode::FP6 */
+
+/* VoltagePipeline::Config (include/voltage_pipeline.h:121-156), same fields and defaults. */
+typedef struct ftdx_voltage_config {
+    size_t struct_size; /* sizeof(ftdx_voltage_config) */
+    uint64_t n_antennae;
+    uint64_t n_beams;
+    uint64_t n_channels;
+    uint64_t n_time;
+    uint64_t n_polarizations;
+    uint64_t n_time_power_sum;
+    int compute_mode; /* FTDX_VOLTAGE_* */
+    int kernel_tune_verbosity;
+    int strategy_tune_verbosity;
+    int gemm_tune;
+    const char* kernel_tune_cache_path;
+    const char* strategy_tune_cache_path;
+    int n_dm_trials; /* dedispersion settings: kept for the layout; no dedispersion library here */
+    float f_min_MHz;
+    float f_max_MHz;
+    float max_dm;
+    float total_obs_time_s;
+    int profile;
+    uint64_t ch_fuse;
+    uint64_t max_payloads;
+} ftdx_voltage_config;
+
+typedef struct ftdx_voltage ftdx_voltage;
+
+/* Config defaults (include/voltage_pipeline.h:123-156). */
+int ftdx_voltage_config_init(ftdx_voltage_config* cfg);
+/* VoltagePipeline::VoltagePipeline (lib/voltage_pipeline.cu:509-575): validates
+ * (n_polarizations == 2, n_time % n_time_power_sum == 0, ch_fuse a power of 2 dividing
+ * n_channels, max_payloads in 1..16) and allocates. */
+int ftdx_voltage_create(const ftdx_voltage_config* cfg, ftdx_voltage** out);
+int ftdx_voltage_destroy(ftdx_voltage* vp);
+/* set_weights (lib/voltage_pipeline.cu:737-784); prepares n_channels / ch_fuse weight
+ * sets (the first ones), drops a captured graph. */
+int ftdx_voltage_set_weights(ftdx_voltage* vp, const float* d_weights, void* stream);
+/* beamform (lib/voltage_pipeline.cu:786-989): one payload -> uint8 [n_ch][n_beam][n_time_out].
+ * With ch_fuse > 1 the prepared weights do not match n_channels and it fails, as the
+ * reference's GEMM check makes it. */
+int ftdx_voltage_beamform(ftdx_voltage* vp, uint8_t* d_output, const uint8_t* d_qc_input, float scale,
+                          void* stream);
+/* beamform_batched (lib/voltage_pipeline.cu:991-1120): d_qc_inputs is a host array of
+ * n_payloads device pointers.  FP8 only; needs ch_fuse > 1 or n_payloads > 1.  Output
+ * [n_ch / ch_fuse][n_beam][n_time * ch_fuse * n_payloads / n_time_power_sum], the
+ * inner index running over (payload, channel in group, time / n_time_power_sum). */
+int ftdx_voltage_beamform_batched(ftdx_voltage* vp, uint8_t* d_output, const uint8_t* const* d_qc_inputs,
+                                  int n_payloads, float scale, void* stream);
+/* compute_filterbank (lib/voltage_pipeline.cu:1122-1226): FP32 [n_ch][n_beam][n_time_out]
+ * into the pipeline's channel-major filterbank buffer. */
+int ftdx_voltage_compute_filterbank(ftdx_voltage* vp, const uint8_t* d_qc_input, void* stream);
+/* compute (lib/voltage_pipeline.cu:1228-1263): filterbank, corner turn to the beam-major
+ * buffer, uint8 quantization of the channel-major filterbank into d_output.  With no
+ * dedispersion library d_dedisp_out is not written (the reference's dedisperse()
+ * returns -1 and compute() goes on). */
+int ftdx_voltage_compute(ftdx_voltage* vp, uint8_t* d_output, const uint8_t* d_qc_input, float* d_dedisp_out,
+                         float scale, void* stream);
+/* corner_turn (lib/voltage_pipeline.cu:1265-1273): [n_ch][n_beam][n_time_out] ->
+ * [n_beam][n_ch][n_time_out]. */
+int ftdx_voltage_corner_turn(ftdx_voltage* vp, float* d_out, const float* d_in, void* stream);
+/* set_graph_enabled (include/voltage_pipeline.h:237): FP8 beamform() calls with the same
+ * input, output and scale replay one captured graph. */
+int ftdx_voltage_set_graph_enabled(ftdx_voltage* vp, int enabled);
+/* output_size, input_size, n_time_out, uses_direct_path, graph_enabled, the filterbank
+ * buffers (include/voltage_pipeline.h:208-303); NULL outputs are skipped. */
+int ftdx_voltage_query(const ftdx_voltage* vp, int64_t* output_size, int64_t* input_size, uint64_t* n_time_out,
+                       int* uses_direct_path, int* graph_enabled, const float** filterbank_chanmajor,
+                       const float** filterbank_beammajor);
+/* ggp::corner_turn_nf_nb (lib/corner_turn.cu:63-75): [Nf][Nb][Nt] -> [Nb][Nf][Nt] FP32. */
+int ftdx_corner_turn(float* d_out, const float* d_in, int n_channels, int n_beams, int n_time, void* stream);
+
 /* Replaces CutlassComplexGemm::print_build_info (cutlass_gemm_api.h:773-775). */
 int ftdx_print_build_info(void);
 

@@ -708,6 +708,51 @@ void ftdo_power_f32(const float* re, const float* im, const float* old, int64_t 
 }
 
 /* ------------------------------------------------------------------ */
+/* Voltage beamformer tail: from exact beams re / im [batch][2 pol][M][n_beam] (FP32) to
+ * the filterbank out [batch][n_beam][M / nt], uint8 (out_u8) or FP32.
+ * mode 0, the FP8 path (lib/voltage_pipeline.cu:886-910, 934-967): the power GEMM of
+ *   pol 0 with beta = 0, of pol 1 with beta = 1 into the same buffer (power epilogue
+ *   p = alpha * fma(re, re, im * im), then fma(beta, old, p); cutlass_gemm_api.cu:425-437
+ *   with its contraction fixed), then time_integrate_quantize_kernel
+ *   (lib/voltage_pipeline.cu:283-318): acc = 0; acc += P[t] for nt samples in order.
+ * mode 1, the FP6 path (power_sum_quantize_kernel, lib/voltage_pipeline.cu:327-370):
+ *   power += ra*ra + ia*ia + rb*rb + ib*ib, contracted as
+ *   fma(ib, ib, fma(rb, rb, fma(ia, ia, ra * ra))).
+ * uint8: __float2int_rn(acc * scale) clamped to [0, 255] (nearest-even; NaN -> 0). */
+static uint8_t vq_u8(float v, float scale) {
+    float q = rintf(v * scale);
+    q = fmaxf(q, 0.0f);  /* NaN -> 0 */
+    q = fminf(q, 255.0f);
+    return (uint8_t)(int)q;
+}
+
+void ftdo_voltage_tail(const float* re, const float* im, int64_t batch, int M, int n_beam, int nt, int mode,
+                       float scale, int out_u8, void* out) {
+    const int n_out = M / nt;
+    for (int64_t b = 0; b < batch; ++b)
+        for (int beam = 0; beam < n_beam; ++beam)
+            for (int t = 0; t < n_out; ++t) {
+                float acc = 0.0f;
+                for (int dt = 0; dt < nt; ++dt) {
+                    const int64_t i0 = ((b * 2 + 0) * M + (int64_t)t * nt + dt) * n_beam + beam;
+                    const int64_t i1 = ((b * 2 + 1) * M + (int64_t)t * nt + dt) * n_beam + beam;
+                    if (mode == 0) {
+                        const float p0 = 1.0f * fmaf(re[i0], re[i0], im[i0] * im[i0]);
+                        float p = 1.0f * fmaf(re[i1], re[i1], im[i1] * im[i1]);
+                        p = fmaf(1.0f, p0, p);
+                        acc += p;
+                    } else {
+                        const float ra = re[i0], ia = im[i0], rb = re[i1], ib = im[i1];
+                        acc += fmaf(ib, ib, fmaf(rb, rb, fmaf(ia, ia, ra * ra)));
+                    }
+                }
+                const int64_t o = (b * n_beam + beam) * n_out + t;
+                if (out_u8) ((uint8_t*)out)[o] = vq_u8(acc, scale);
+                else ((float*)out)[o] = acc;
+            }
+}
+
+/* ------------------------------------------------------------------ */
 /* Pillbox UV gridding (PillboxGridScatter, include/kernels/visibility_bf.cuh:259-370;
  * uv_cell, include/kernels/imaging_kernels.cuh:33-45): sequential scatter-add of
  * Nf packed triangles vis [Nf][n_bl][2] into grid [Nf][Ng][Ng][2]; the conjugate goes to

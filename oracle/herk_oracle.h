@@ -116,6 +116,9 @@ void ftdo_herk_ftd_i64(const uint8_t* raw, int n_ant, int n_chan, int n_time, in
 void ftdo_gemm_c64(const double* A, const double* B, double* C, int M, int N, int K, int batch,
                    int threads);
 /* The GEMM epilogue's power detection in float arithmetic. */
+/* Voltage beamformer tail (power, polarization sum, time integration, quantization). */
+void ftdo_voltage_tail(const float* re, const float* im, int64_t batch, int M, int n_beam, int nt, int mode,
+                       float scale, int out_u8, void* out);
 void ftdo_power_f32(const float* re, const float* im, const float* old, int64_t n, float alpha,
                     float beta, float* out);
 

@@ -103,6 +103,9 @@ def lib():
         fp = ctypes.POINTER(ctypes.c_float)
         L.ftdo_power_f32.argtypes = [fp, fp, fp, ctypes.c_int64, ctypes.c_float, ctypes.c_float, fp]
         L.ftdo_power_f32.restype = None
+        L.ftdo_voltage_tail.argtypes = [fp, fp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p]
+        L.ftdo_voltage_tail.restype = None
         L.ftdo_pillbox_grid.argtypes = [fp, fp, dp, fp, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_float]
         L.ftdo_pillbox_grid.restype = None
@@ -371,6 +374,101 @@ def power_f32(re: np.ndarray, im: np.ndarray, alpha: float = 1.0, beta: float = 
     lib().ftdo_power_f32(_ptr(re, fp), _ptr(im, fp), _ptr(o, fp), re.size, alpha, beta, _ptr(out, fp))
     return out
 
+
+
+# ---- voltage beamformer (ggp::VoltagePipeline) ----
+_E4M3_OF_F16 = None
+
+
+def fp16_to_e4m3_values(w16: np.ndarray) -> np.ndarray:
+    """FP16 values -> the E4M3 values prepare_b turns them into (clamp +-448, nearest-even;
+    cast_kernels_common.hpp:30-48), as float64, through a table over all FP16 codes."""
+    global _E4M3_OF_F16
+    if _E4M3_OF_F16 is None:
+        vals = np.arange(65536, dtype=np.uint32).astype(np.uint16).view(np.float16).astype(np.float32)
+        _E4M3_OF_F16 = np.array([fp8e4m3_to_float(f32_to_e4m3_clamped(float(v))) for v in vals])
+    w16 = np.ascontiguousarray(w16, dtype=np.float16)
+    return _E4M3_OF_F16[w16.view(np.uint16)]
+
+
+def voltage_weights(weights: np.ndarray, n_ch: int, n_beam: int, n_ant: int, compute: str) -> np.ndarray:
+    """FP32 [n_ch, n_beam, n_ant, 2] weights as the GEMM sees them (complex128): FP32 ->
+    FP16 nearest-even (deinterleave_fp32_to_fp16_kernel, lib/voltage_pipeline.cu:484-498),
+    then E4M3 (compute 'fp8') or MX E3M2 with 32-sample scales along antennas ('fp6')."""
+    w16 = np.asarray(weights, dtype=np.float32).reshape(n_ch, n_beam, n_ant, 2).astype(np.float16)
+    if compute == "fp8":
+        q = fp16_to_e4m3_values(w16)
+        return q[..., 0] + 1j * q[..., 1]
+    codes, scales = mx_quantize(w16.astype(np.float32).reshape(n_ch * n_beam, n_ant, 2), MX_FP6_E3M2)
+    return mx_dequantize(codes, scales, MX_FP6_E3M2, n_ant).reshape(n_ch, n_beam, n_ant)
+
+
+def voltage_beams(payloads, W: np.ndarray, n_ant: int, n_beam: int, n_ch: int, n_time: int,
+                  ch_fuse: int = 1):
+    """Beams of the fused GEMM: re, im float32 [batch, 2, M_total, n_beam] with batch b =
+    n_ch / ch_fuse using weight set b (the first n_ch / ch_fuse sets are prepared,
+    lib/voltage_pipeline.cu:770-776), M_total = n_time * ch_fuse * n_payloads and sample
+    m = (payload * ch_fuse + c) * n_time + t of channel b * ch_fuse + c
+    (qc_fused_transpose_fp8_kernel, lib/voltage_pipeline.cu:176-236).  The products of
+    the rounded weights and the 4-bit voltages are summed in float64; the caller keeps
+    them exact (then every FP32 summation order gives the same value)."""
+    batch = n_ch // ch_fuse
+    V = np.stack([decode_qc_array(np.asarray(p, dtype=np.uint8)).reshape(n_ch, 2, n_ant, n_time)
+                  for p in payloads])                                  # [np, ch, pol, ant, t]
+    V = V.reshape(len(payloads), batch, ch_fuse, 2, n_ant, n_time)
+    V = V.transpose(1, 3, 0, 2, 5, 4).reshape(batch, 2, -1, n_ant)      # [b, pol, m, ant]
+    Wb = W[:batch]                                                      # [b, beam, ant]
+    C = np.einsum("bpma,bna->bpmn", V, Wb)
+    re, im = C.real, C.imag
+    # exactness: every product is a multiple of g (the finest power of two among the
+    # weight parts, the voltages being integers), so every partial sum in any order is
+    # an FP32 value while its magnitude stays below 2^24 g; bound it by sum |v| |w|
+    parts = np.abs(np.concatenate([Wb.real.ravel(), Wb.imag.ravel()]))
+    parts = parts[parts > 0]
+    if parts.size:
+        ints = np.ldexp(parts, 80).astype(object)
+        g = min(int(x) & -int(x) for x in set(ints.tolist()))
+        g = float(np.ldexp(float(g), -80))
+        bound = (np.abs(V.real) + np.abs(V.imag)) @ \
+            (np.abs(Wb.real) + np.abs(Wb.imag)).transpose(0, 2, 1)[:, None]
+        assert bound.max() < 2.0 ** 24 * g, "FP32 partial sums could round: not an exact case"
+    return re.astype(np.float32), im.astype(np.float32)
+
+
+def voltage_tail(re: np.ndarray, im: np.ndarray, nt: int, mode: int, scale: float, out_u8: bool):
+    """ftdo_voltage_tail: [batch, 2, M, n_beam] beams -> [batch, n_beam, M / nt] (mode 0 =
+    FP8 path: power GEMMs with beta accumulation + time integration; mode 1 = FP6 path:
+    power_sum)."""
+    re = np.ascontiguousarray(re, dtype=np.float32)
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    batch, _, M, n_beam = re.shape
+    out = np.zeros((batch, n_beam, M // nt), dtype=np.uint8 if out_u8 else np.float32)
+    fp = ctypes.c_float
+    lib().ftdo_voltage_tail(_ptr(re, fp), _ptr(im, fp), batch, M, n_beam, nt, mode, scale, int(out_u8),
+                            out.ctypes.data)
+    return out
+
+
+def voltage_beamform(payloads, weights: np.ndarray, n_ant: int, n_beam: int, n_ch: int, n_time: int,
+                     n_tps: int, scale: float = 1.0, ch_fuse: int = 1, compute: str = "fp8",
+                     out_u8: bool = True):
+    """The pipeline's output for QC payloads (each [n_ch, 2, n_ant, n_time] bytes):
+    [n_ch / ch_fuse, n_beam, M_total / n_tps] uint8 (beamform / beamform_batched) or FP32
+    (compute_filterbank)."""
+    W = voltage_weights(weights, n_ch, n_beam, n_ant, compute)
+    re, im = voltage_beams(payloads, W, n_ant, n_beam, n_ch, n_time, ch_fuse)
+    return voltage_tail(re, im, n_tps, 0 if compute == "fp8" else 1, scale, out_u8)
+
+
+def voltage_fp64(payload, weights: np.ndarray, n_ant: int, n_beam: int, n_ch: int, n_time: int, n_tps: int):
+    """cpu_beamformer (tests/voltage_pipeline_test.cpp:43-83): FP64 power with the
+    unrounded weights, [n_ch, n_beam, n_time / n_tps]."""
+    w = np.asarray(weights, dtype=np.float64).reshape(n_ch, n_beam, n_ant, 2)
+    Wc = w[..., 0] + 1j * w[..., 1]
+    V = decode_qc_array(np.asarray(payload, dtype=np.uint8)).reshape(n_ch, 2, n_ant, n_time)
+    beams = np.einsum("cpat,cna->cpnt", V, Wc)                            # [ch, pol, beam, t]
+    power = (beams.real ** 2 + beams.imag ** 2).sum(axis=1)               # std::norm; [ch, beam, t]
+    return power.reshape(n_ch, n_beam, n_time // n_tps, n_tps).sum(axis=-1)
 
 
 # ---- visibility consumers ----

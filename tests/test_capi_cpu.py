@@ -48,7 +48,7 @@ def test_headers_compile():
     inc = os.path.join(ROOT, "include")
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-x", "c",
                     os.path.join(inc, "ftdx.h")], check=True)
-    for h in ("cutlass_gemm_api.h", "ggp_cutlass_interface.h", "ggp_xengine.h"):
+    for h in ("cutlass_gemm_api.h", "ggp_cutlass_interface.h", "ggp_xengine.h", "voltage_pipeline.h"):
         subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-x", "c++",
                         "-I", inc, "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-"],
                        input=f'#include "{h}"\n', text=True, check=True)
