@@ -267,6 +267,65 @@ def beamformer_study(torch, ftdx, dev, M: int, N: int, K: int, B: int, seed: int
             "pct_fp8_peak": round(100.0 * tf / FP8_DENSE_PEAK_TFLOPS, 2)}
 
 
+def voltage_study(torch, ftdx, dev, n_ant: int, n_beam: int, n_ch: int, n_time: int, n_tps: int, seed: int,
+                  reps: int) -> dict:
+    """Voltage beamformer end to end (ggp::VoltagePipeline::beamform): QC payload -> uint8
+    Stokes-I filterbank.  The pipeline (rows pass + one GEMM whose epilogue detects,
+    sums the pols, integrates and quantizes) against the reference's call sequence on
+    this library: transpose + pol split, gemm_prepared_power_int4 for pol 0 (beta 0) and
+    pol 1 (beta 1) into an FP32 power buffer, then integration + quantization.
+    4M-equivalent FLOP 8 * (2 n_time) * n_beam * n_ant per channel."""
+    qc = synth_qc(torch, n_ch * 2 * n_ant * n_time, seed, dev)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    w = torch.rand((n_ch, n_beam, n_ant, 2), generator=g, device=dev) * 2 - 1
+    vp = ftdx.VoltagePipeline(n_antennae=n_ant, n_beams=n_beam, n_channels=n_ch, n_time=n_time,
+                              n_time_power_sum=n_tps, strategy_tune_verbosity=0)
+    vp.set_weights(w)
+    out = torch.empty((n_ch, n_beam, n_time // n_tps), dtype=torch.uint8, device=dev)
+    scale = 1e-5
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_strategy_tune_verbosity(0)
+    w16 = w.to(torch.float16)
+    ctx.prepare_b(w16[..., 0].contiguous(), w16[..., 1].contiguous(), n_beam, n_ant, n_ch)
+    del w16, w
+    P = torch.empty((n_ch, n_time, n_beam), dtype=torch.float32, device=dev)
+    ref_out = torch.empty_like(out)
+    v = qc.view(n_ch, 2, n_ant, n_time)
+    rows = [torch.empty((n_ch, n_time, n_ant), dtype=torch.uint8, device=dev) for _ in range(2)]
+
+    def reference_sequence():
+        for p in range(2):
+            rows[p].copy_(v[:, p].transpose(1, 2))
+            ctx.gemm_prepared_power_int4(rows[p], P, n_time, n_beam, n_ant, n_ch, beta=float(p))
+        acc = P.view(n_ch, n_time // n_tps, n_tps, n_beam).sum(dim=2)
+        ref_out.copy_(torch.round(acc.transpose(1, 2) * scale).clamp_(0, 255).to(torch.uint8))
+
+    def pipeline():
+        vp.beamform(out, qc, scale)
+
+    res = {}
+    for name, fn in (("reference_sequence_ms", reference_sequence), ("pipeline_ms", pipeline)):
+        fn()
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = round(e0.elapsed_time(e1) / reps, 4)
+    flop = 8.0 * 2 * n_time * n_beam * n_ant * n_ch
+    diff = (out.int() - ref_out.int()).abs()
+    return {"n_antennae": n_ant, "n_beams": n_beam, "n_channels": n_ch, "n_time": n_time,
+            "n_time_power_sum": n_tps, "seed": seed, **res,
+            "speedup": round(res["reference_sequence_ms"] / res["pipeline_ms"], 3),
+            "tflops_4m_equiv": round(flop / (res["pipeline_ms"] * 1e-3) / 1e12, 2),
+            "pct_fp8_peak": round(100.0 * flop / (res["pipeline_ms"] * 1e-3) / 1e12 / FP8_DENSE_PEAK_TFLOPS, 2),
+            "max_abs_diff_vs_reference_sequence": int(diff.max()),
+            "mean_output": round(float(out.float().mean()), 2)}
+
+
 def pol_reduce_study(torch, ftdx, dev, N: int, K: int, n_ch: int, n_pol: int, n_ti: int, seed: int,
                      reps: int) -> dict:
     """HERK + polarization sum: one fused pass (ftdx_herk_pol_reduce) against the HERK of
@@ -429,6 +488,7 @@ def main() -> int:
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-beamformer", action="store_true")
     ap.add_argument("--no-polsum", action="store_true")
+    ap.add_argument("--no-voltage", action="store_true")
     ap.add_argument("--no-bf6", action="store_true")
     ap.add_argument("--batched-problems", type=int, default=512)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -563,6 +623,15 @@ def main() -> int:
         log(f"bench: beamformer {bf['ms']:.3f} ms, {bf['tflops_4m_equiv']:.0f} TFLOP/s 4M-equiv")
         torch.cuda.empty_cache()
 
+    volt = None
+    if not args.no_voltage:
+        log("bench: voltage pipeline 1664 ant x 4000 beams x 128 ch, 256 times, n_tps 4 ...")
+        volt = voltage_study(torch, ftdx, dev, 1664, 4000, 128, 256, 4, 11, 10)
+        log(f"bench: voltage pipeline {volt['pipeline_ms']:.3f} ms vs reference sequence "
+            f"{volt['reference_sequence_ms']:.3f} ms ({volt['speedup']:.2f}x), "
+            f"{volt['tflops_4m_equiv']:.0f} TFLOP/s 4M-equiv")
+        torch.cuda.empty_cache()
+
     polsum = None
     if not args.no_polsum:
         log("bench: HERK + polarization sum, 2048 inputs x 16 ch x 2 pol x 2 ti, K=1024 ...")
@@ -623,6 +692,7 @@ def main() -> int:
         "streaming_host_payloads": stream_res,
         "beamformer_gemm_power": bf,
         "herk_pol_sum_fusion": polsum,
+        "voltage_pipeline": volt,
         "e3m2_operands": bf6,
         "cpu_baseline": cpu,
     }

@@ -20,9 +20,9 @@ namespace ftdx {
 namespace {
 
 constexpr int VT = 256;
-// LDS bytes per antenna row of a 64-sample tile: 17 dwords, so the gathers of four
+// LDS bytes per antenna row of the 128-sample tile: 33 dwords, so the gathers of four
 // 16-antenna segments by one wave land on disjoint banks
-constexpr int VLP = 68;
+constexpr int VLP = 132;
 
 unsigned grid_of(int64_t n, int64_t cap = 8192) {
     int64_t g = (n + VT - 1) / VT;
@@ -48,25 +48,28 @@ __device__ __forceinline__ const uint8_t* sample_ptr(const VoltageLayout& L, int
     return payload_base(L, p) + ((chan * 2 + pol) * L.n_ant + ant) * (int64_t)L.n_time + t;
 }
 
-// One workgroup: batch b, samples 64c .. 64c + 63 of both polarizations, antennas
-// 64at .. 64at + 63.  Loads run along time (the input's fast axis) into an LDS tile
-// [pol][ant][sample]; stores run along antennas (the operand rows' fast axis).
-// MODE 0: pre-cast FP8 rows, beam order; MODE 1: QC bytes [b][pol][M_total][n_ant].
+// One workgroup: batch b, samples 128q .. 128q + 127 (two 64-sample row blocks, so
+// every 128-byte line of an antenna's time series is read by one workgroup) of both
+// polarizations, antennas 64at .. 64at + 63.  Loads run along time (the input's fast
+// axis) into an LDS tile [pol][ant][sample]; stores run along antennas (the operand
+// rows' fast axis).  MODE 0: pre-cast FP8 rows in beam order (row block c = samples
+// 64c .. 64c + 63, pol 0 then pol 1); MODE 1: QC bytes [b][pol][M_total][n_ant].
 template <int MODE, bool VEC>
 __global__ void __launch_bounds__(VT) voltage_rows_kernel(VoltageLayout L, uint8_t* __restrict__ rows,
                                                          int chunks, int ant_tiles, int Kp) {
     __shared__ __attribute__((aligned(16))) uint8_t tile[2 * 64 * VLP];
     const int M_total = L.n_time * L.ch_fuse * L.n_payloads;
+    const int pairs = (chunks + 1) / 2;
     int w = blockIdx.x;
     const int at = w % ant_tiles;
     w /= ant_tiles;
-    const int c = w % chunks;
-    const int b = w / chunks;
+    const int q = w % pairs;
+    const int b = w / pairs;
     const int tid = threadIdx.x;
-    for (int it = tid; it < 512; it += VT) {
-        const int seg = it & 3, al = (it >> 2) & 63, pol = it >> 8;
+    for (int it = tid; it < 1024; it += VT) {
+        const int seg = it & 7, al = (it >> 3) & 63, pol = it >> 9;
         const int ant = at * 64 + al;
-        const int m0 = c * 64 + seg * 16;
+        const int m0 = q * 128 + seg * 16;
         uint32_t x[4] = {0u, 0u, 0u, 0u};
         if (ant < L.n_ant && m0 < M_total) {
             if constexpr (VEC) {  // the 16 samples are one aligned run of one channel
@@ -82,16 +85,18 @@ __global__ void __launch_bounds__(VT) voltage_rows_kernel(VoltageLayout L, uint8
         d[0] = x[0]; d[1] = x[1]; d[2] = x[2]; d[3] = x[3];
     }
     __syncthreads();
-    for (int it = tid; it < 512; it += VT) {
-        const int seg = it & 3, mm = (it >> 2) & 63, pol = it >> 8;
+    for (int it = tid; it < 1024; it += VT) {
+        const int seg = it & 3, mm = (it >> 2) & 127, pol = it >> 9;
         const int a0 = at * 64 + seg * 16;
+        const int c = q * 2 + (mm >> 6);
+        if (c >= chunks) continue;
         uint32_t x[4] = {0u, 0u, 0u, 0u};
         const uint8_t* src = tile + (pol * 64 + seg * 16) * VLP + mm;
 #pragma unroll
         for (int e = 0; e < 16; ++e) x[e >> 2] |= (uint32_t)src[e * VLP] << (8 * (e & 3));
         if constexpr (MODE == 0) {
             if (a0 >= Kp) continue;
-            const int64_t row = (int64_t)b * chunks * 128 + c * 128 + pol * 64 + mm;
+            const int64_t row = (int64_t)b * chunks * 128 + c * 128 + pol * 64 + (mm & 63);
             uint8_t* dst = rows + row * (2 * Kp) + 2 * a0;
             uint4 lo, hi;
             qc4_to_fp8_planar(x[0], lo.x, lo.y);
@@ -101,7 +106,7 @@ __global__ void __launch_bounds__(VT) voltage_rows_kernel(VoltageLayout L, uint8
             *reinterpret_cast<uint4*>(dst) = lo;
             *reinterpret_cast<uint4*>(dst + 16) = hi;
         } else {
-            const int m = c * 64 + mm;
+            const int m = q * 128 + mm;
             if (m >= M_total || a0 >= L.n_ant) continue;
             uint8_t* dst = rows + (((int64_t)b * 2 + pol) * M_total + m) * L.n_ant + a0;
             if (VEC && a0 + 16 <= L.n_ant && (L.n_ant & 15) == 0) {
@@ -232,7 +237,7 @@ hipError_t launch_rows(const VoltageLayout& L, uint8_t* rows, hipStream_t s) {
     const int Kp = (L.n_ant + 15) / 16 * 16;
     const int chunks = (int)((M_total + 63) / 64);
     const int ant_tiles = (Kp + 63) / 64;
-    const int64_t blocks = (int64_t)batch * chunks * ant_tiles;
+    const int64_t blocks = (int64_t)batch * ((chunks + 1) / 2) * ant_tiles;
     if (blocks > 0x7FFFFFFF || M_total > 0x7FFFFFFF) return hipErrorInvalidValue;
     if (layout_vec(L))
         hipLaunchKernelGGL((voltage_rows_kernel<MODE, true>), dim3((unsigned)blocks), dim3(VT), 0, s, L, rows,
