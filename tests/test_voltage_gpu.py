@@ -4,8 +4,13 @@ beams that every summation order gives exactly), the channel-fused / multi-paylo
 layout of beamform_batched, compute()'s corner turn and quantization, graph replay,
 the FP6 path, the reference's own acceptance test against its FP64 CPU beamformer
 (tests/voltage_pipeline_test.cpp:135-246, lifecycle :270-300) and its error paths."""
+import os
+import subprocess
+
 import numpy as np
 import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -209,3 +214,14 @@ def test_lifecycle_sizes_and_errors(dev):
     fp6.set_weights(torch.zeros((2, 32, 64, 2), device=dev))
     with pytest.raises(ftdx.UnsupportedError, match="FP8"):
         fp6.beamform_batched(out2, [qc, qc])
+
+
+def test_cpp_drop_in_voltage_pipeline(dev):
+    """C++ ggp::VoltagePipeline (include/voltage_pipeline.h) through libftdx.so: the
+    reference test's sizes (128 antennas, 64 beams, 4 channels, 128 times)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "test_voltage_api")
+    assert os.path.exists(exe), "tests/cpp/test_voltage_api not built"
+    r = subprocess.run([exe, "128", "64", "4", "128", "42"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASSED" in r.stdout
