@@ -68,14 +68,10 @@ struct ftdx_context {
     // (FP8 compute: pre-cast rows; FP6 / FP4 compute: MX codes in *_b, scales in *_bs)
     uint8_t* prep_b = nullptr;
     size_t prep_b_cap = 0;
-    uint8_t* prep_bs = nullptr;
-    size_t prep_bs_cap = 0;
     int prep_N = -1, prep_K = -1, prep_batch = -1;
     int prep_fmt = -1;  // -1: FP8 rows, else the MxFormat of the codes
     uint8_t* gemm_b = nullptr;
     size_t gemm_b_cap = 0;
-    uint8_t* gemm_bs = nullptr;
-    size_t gemm_bs_cap = 0;
 };
 
 namespace ftdx {
@@ -87,6 +83,8 @@ thread_local ftdx_context* g_default_ctx = nullptr;
 // Default launch configuration when nothing is tuned or set.
 constexpr int kDefaultCfg = TILE_128x128_K64;
 constexpr int kDefaultRemap = 1;
+// MX HERK (block-scaled operand rows) default tile
+constexpr int kMxDefaultCfg = TILE_256x128_K64;
 
 // DirectTileConfig (reference names, cutlass_gemm_api.h:113-119) -> gfx950 tile
 // configuration: the reference enum picks K chunk x pipeline depth; here it
@@ -384,12 +382,10 @@ int ftdx_create(ftdx_context** out) {
 int ftdx_destroy(ftdx_context* ctx) {
     if (!ctx) return FTDX_OK;
     ftdx_end_herk(ctx);
-    if (ctx->prep_b || ctx->gemm_b || ctx->prep_bs || ctx->gemm_bs) {
+    if (ctx->prep_b || ctx->gemm_b) {
         (void)hipDeviceSynchronize();
         if (ctx->prep_b) (void)hipFree(ctx->prep_b);
         if (ctx->gemm_b) (void)hipFree(ctx->gemm_b);
-        if (ctx->prep_bs) (void)hipFree(ctx->prep_bs);
-        if (ctx->gemm_bs) (void)hipFree(ctx->gemm_bs);
     }
     delete ctx;
     return FTDX_OK;
@@ -642,32 +638,38 @@ int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int ba
     if (dev != ctx->device)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: context belongs to another device (or none)");
 
+    // Quantize into MX operand rows (codes and scale per 32-sample part, the tile's
+    // 64-bytes-per-32-samples geometry), then the HERK tile kernel with scaled MFMAs.
     const int fmt = ftdx::mx_format(compute);
     const int nblkp = ftdx::mx_padded_blocks(K);
     const int64_t rows = (int64_t)N * batch_count;
-    const size_t code_bytes = (size_t)rows * nblkp * 2 * ftdx::mx_code_bytes(fmt);
-    const size_t scale_bytes = (size_t)rows * nblkp * 2;
-    int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap, code_bytes, "ftdx_herk_mx: scratch");
-    if (rc != FTDX_OK) return rc;
-    rc = ftdx::ensure_scratch(ctx->mx_scales, ctx->mx_scales_cap, scale_bytes, "ftdx_herk_mx: scratch");
+    const size_t row_bytes = (size_t)nblkp * 64;
+    if ((int64_t)N * (int64_t)row_bytes > 0xFFFFFFFFll)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: N * K too large");
+    int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap, (size_t)rows * row_bytes + 16,
+                                  "ftdx_herk_mx: scratch");
     if (rc != FTDX_OK) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (nblkp > 0) {
-        e = ftdx::launch_mx_quantize(A, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16,
-                                     fmt, ctx->mx_codes, ctx->mx_scales, rows, K, s);
+        e = ftdx::launch_mx_quantize_rows(A, nullptr, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16,
+                                          fmt, false, ctx->mx_codes, rows, K, s);
         if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: quantize launch");
     }
-    ftdx::MxHerkArgs a{};
-    a.codes = ctx->mx_codes;
-    a.scales = ctx->mx_scales;
+    ftdx::HerkPlainArgs a{};
+    a.A = ctx->mx_codes;
     a.C = C;
     a.N = N;
+    a.K = nblkp * 32;
     a.batch = batch_count;
-    a.nblkp = nblkp;
     a.alpha = alpha;
     a.beta = beta;
+    a.a_bstride = (int64_t)N * row_bytes;
     a.c_bstride = (int64_t)N * (N + 1);
-    e = ftdx::launch_herk_mx(a, fmt, output == FTDX_OUTPUT_FP16, s);
+    a.precast = fmt == ftdx::MX_FP6_E3M2 ? 3 : 4;
+    a.lda = (int64_t)row_bytes;
+    const int cfg = ctx->forced_cfg >= 0 ? ftdx::tile_base(ctx->forced_cfg) : ftdx::kMxDefaultCfg;
+    const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
+    e = ftdx::launch_herk_int4_plain(a, cfg, output == FTDX_OUTPUT_FP16, remap, s);
     if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: kernel launch");
     return FTDX_OK;
 }
@@ -1004,9 +1006,9 @@ int ftdx_pillbox_grid(const void* vis_tri, int vis_output, const float* baseline
 // ---- complex GEMM with prepared B ----
 namespace {
 
-// FP8: conj(B) as pre-cast rows (fmt = -1).  FP6 / FP4: MX codes and scales of conj(B)
-// (fmt = the MxFormat), the block-scaled path of cutlass_gemm_api.h:40-47.
-int prepare_rows(ftdx_context* ctx, uint8_t*& buf, size_t& cap, uint8_t*& sbuf, size_t& scap,
+// FP8: conj(B) as pre-cast rows (fmt = -1).  FP6 / FP4: MX operand rows of conj(B), codes
+// and scales (fmt = the MxFormat), the block-scaled path of cutlass_gemm_api.h:40-47.
+int prepare_rows(ftdx_context* ctx, uint8_t*& buf, size_t& cap,
                  const void* B_re, const void* B_im, int N, int K, int batch, int compute, void* stream,
                  const char* what, int& fmt) {
     const std::string w(what);
@@ -1020,11 +1022,10 @@ int prepare_rows(ftdx_context* ctx, uint8_t*& buf, size_t& cap, uint8_t*& sbuf, 
         fmt = ftdx::mx_format(compute);
         const int nblkp = ftdx::mx_padded_blocks(K);
         const int64_t rows = (int64_t)N * batch;
-        int rc = ftdx::ensure_scratch(buf, cap, (size_t)rows * nblkp * 2 * ftdx::mx_code_bytes(fmt) + 16, what);
-        if (rc == FTDX_OK) rc = ftdx::ensure_scratch(sbuf, scap, (size_t)rows * nblkp * 2 + 16, what);
+        int rc = ftdx::ensure_scratch(buf, cap, (size_t)rows * nblkp * 64 + 16, what);
         if (rc != FTDX_OK) return rc;
-        hipError_t e = ftdx::launch_mx_quantize_ex(B_re, B_im, ftdx::MX_IN_FP16_PLANAR, fmt, true, buf, sbuf,
-                                                   rows, K, static_cast<hipStream_t>(stream));
+        hipError_t e = ftdx::launch_mx_quantize_rows(B_re, B_im, ftdx::MX_IN_FP16_PLANAR, fmt, true, buf, rows, K,
+                                                     static_cast<hipStream_t>(stream));
         return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
     }
     fmt = -1;
@@ -1053,9 +1054,9 @@ int check_prepared(ftdx_context* ctx, int N, int K, int batch, const char* what,
 
 // Block-scaled complex GEMM: quantize A (FP16 planar or INT4) into the MX scratch, then
 // C = alpha A B^T + beta C against the MX codes of conj(B).
-int gemm_mx_run(ftdx_context* ctx, const uint8_t* bcodes, const uint8_t* bscales, int fmt, bool int4,
-                const void* a0, const void* a1, void* c0, void* c1, int M, int N, int K, int batch,
-                bool out_fp16, float alpha, float beta, void* stream, const char* what) {
+int gemm_mx_run(ftdx_context* ctx, const uint8_t* brows, int fmt, bool int4, const void* a0, const void* a1,
+                void* c0, void* c1, int M, int N, int K, int batch, bool out_fp16, float alpha, float beta,
+                void* stream, const char* what) {
     const std::string w(what);
     if (M < 0) return fail(FTDX_ERR_INVALID_ARG, w + ": M must be >= 0");
     if (M == 0 || N == 0 || batch == 0) return FTDX_OK;
@@ -1065,31 +1066,31 @@ int gemm_mx_run(ftdx_context* ctx, const uint8_t* bcodes, const uint8_t* bscales
         return fail(FTDX_ERR_INVALID_ARG, w + ": sizes too large");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int nblkp = ftdx::mx_padded_blocks(K);
-    const int64_t rows = (int64_t)M * batch;
-    int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap,
-                                  (size_t)rows * nblkp * 2 * ftdx::mx_code_bytes(fmt) + 16, what);
-    if (rc == FTDX_OK)
-        rc = ftdx::ensure_scratch(ctx->mx_scales, ctx->mx_scales_cap, (size_t)rows * nblkp * 2 + 16, what);
+    const int64_t rows = (int64_t)M * batch, pitch = (int64_t)nblkp * 64;
+    int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap, (size_t)(rows * pitch) + 16, what);
     if (rc != FTDX_OK) return rc;
-    hipError_t e = ftdx::launch_mx_quantize_ex(a0, a1, int4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16_PLANAR,
-                                               fmt, false, ctx->mx_codes, ctx->mx_scales, rows, K, s);
+    hipError_t e = ftdx::launch_mx_quantize_rows(a0, a1, int4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16_PLANAR, fmt,
+                                                 false, ctx->mx_codes, rows, K, s);
     if (e != hipSuccess) return hip_fail(e, what);
-    ftdx::MxGemmArgs g;
-    g.acodes = ctx->mx_codes;
-    g.ascales = ctx->mx_scales;
-    g.bcodes = bcodes;
-    g.bscales = bscales;
-    g.Cre = c0;
-    g.Cim = c1;
+    ftdx::GemmArgs g;
+    g.A = ctx->mx_codes;
+    g.B = brows;
+    g.C = c0;
+    g.C2 = c1;
     g.M = M;
     g.N = N;
+    g.K = nblkp * 32;
     g.batch = batch;
-    g.nblkp = nblkp;
     g.alpha = alpha;
     g.beta = beta;
+    g.lda = g.ldb = pitch;
+    g.a_bstride = (int64_t)M * pitch;
+    g.b_bstride = (int64_t)N * pitch;
     g.ldc = N;
     g.c_bstride = (int64_t)M * N;
-    e = ftdx::launch_gemm_mx(g, fmt, out_fp16, s);
+    g.operand = fmt == ftdx::MX_FP6_E3M2 ? 3 : 4;
+    const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
+    e = ftdx::launch_gemm_fp8(g, ftdx::TILE_128x128_K64, ftdx::OUT_GEMM_PLANAR, out_fp16, remap, s);
     return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
 }
 
@@ -1099,7 +1100,7 @@ int ftdx_prepare_b(ftdx_context* ctx, const void* B_re, const void* B_im, int N,
                    int compute, void* stream) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_prepare_b: NULL context");
     int fmt = -1;
-    int rc = prepare_rows(ctx, ctx->prep_b, ctx->prep_b_cap, ctx->prep_bs, ctx->prep_bs_cap, B_re, B_im, N,
+    int rc = prepare_rows(ctx, ctx->prep_b, ctx->prep_b_cap, B_re, B_im, N,
                           K, batch_count, compute, stream, "ftdx_prepare_b", fmt);
     if (rc != FTDX_OK) {
         ctx->prep_N = ctx->prep_K = ctx->prep_batch = -1;
@@ -1123,7 +1124,7 @@ int ftdx_gemm_prepared(ftdx_context* ctx, const void* A_re, const void* A_im, vo
     if (input != FTDX_INPUT_FP16 && input != FTDX_INPUT_INT4)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm_prepared: input must be FP16 or INT4");
     if (ctx->prep_fmt >= 0)
-        return gemm_mx_run(ctx, ctx->prep_b, ctx->prep_bs, ctx->prep_fmt, input == FTDX_INPUT_INT4, A_re, A_im,
+        return gemm_mx_run(ctx, ctx->prep_b, ctx->prep_fmt, input == FTDX_INPUT_INT4, A_re, A_im,
                            C_re, C_im, M, N, K, batch_count, output == FTDX_OUTPUT_FP16, alpha, beta, stream,
                            "ftdx_gemm_prepared");
     return ftdx::gemm_run(ctx, ctx->prep_b, input == FTDX_INPUT_INT4 ? ftdx::GEMM_A_INT4 : ftdx::GEMM_A_FP16_PLANAR,
@@ -1187,11 +1188,11 @@ int ftdx_gemm(ftdx_context* ctx, const void* A_re, const void* A_im, const void*
     if (input != FTDX_INPUT_FP16 && input != FTDX_INPUT_INT4)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_gemm: input must be FP16 or INT4");
     int fmt = -1;
-    int rc = prepare_rows(ctx, ctx->gemm_b, ctx->gemm_b_cap, ctx->gemm_bs, ctx->gemm_bs_cap, B_re, B_im, N, K,
+    int rc = prepare_rows(ctx, ctx->gemm_b, ctx->gemm_b_cap, B_re, B_im, N, K,
                           batch_count, compute, stream, "ftdx_gemm", fmt);
     if (rc != FTDX_OK) return rc;
     if (fmt >= 0)
-        return gemm_mx_run(ctx, ctx->gemm_b, ctx->gemm_bs, fmt, input == FTDX_INPUT_INT4, A_re, A_im, C_re, C_im,
+        return gemm_mx_run(ctx, ctx->gemm_b, fmt, input == FTDX_INPUT_INT4, A_re, A_im, C_re, C_im,
                            M, N, K, batch_count, output == FTDX_OUTPUT_FP16, alpha, beta, stream, "ftdx_gemm");
     return ftdx::gemm_run(ctx, ctx->gemm_b, input == FTDX_INPUT_INT4 ? ftdx::GEMM_A_INT4 : ftdx::GEMM_A_FP16_PLANAR,
                           A_re, A_im, C_re, C_im, M, N, K, batch_count, ftdx::OUT_GEMM_PLANAR,

@@ -57,11 +57,44 @@ __device__ __forceinline__ v16f mfma_32x32x64_fp8(v8i a, v8i b, v16f c) {
 // 192 bits of the 8-VGPR operand.  E3M2 holds every integer in [-8, 8] exactly, so integer
 // QC products sum exactly in FP32 either way; the 6-bit format runs the MFMA at twice
 // the FP8 rate.
-enum OperandFormat { OPF_FP8 = 0, OPF_BF6 = 3 };
+// OPF_MX6 / OPF_MX4: MX block-scaled E3M2 / E2M1 operand rows.  Per 32-sample block a row
+// holds two 32-byte parts, Re then Im: the part's 32 codes (24 / 16 bytes), its E8M0
+// scale byte right after them, zeros to 32 bytes.  A lane's operand is one part, so its
+// scale is dword opf_scale_dword of the operand itself.
+enum OperandFormat { OPF_FP8 = 0, OPF_BF6 = 3, OPF_MX6 = 16 + 3, OPF_MX4 = 16 + 4 };
+
+constexpr bool opf_mx(int opf) { return opf >= 16; }
+constexpr int opf_scale_dword(int opf) { return opf == OPF_MX6 ? 6 : 4; }
 
 template <int OPF>
 __device__ __forceinline__ v16f mfma_32x32x64(v8i a, v8i b, v16f c) {
     return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, OPF, OPF, 0, 0, 0, 0);
+}
+
+// Scaled form for the MX operand rows: each lane's 32 values times its own scale.
+template <int OPF>
+__device__ __forceinline__ v16f mfma_32x32x64_mx(v8i a, v8i b, v16f c) {
+    constexpr int code = OPF & 15;
+    constexpr int sd = opf_scale_dword(OPF);
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, code, code, 0, a[sd], 0, b[sd]);
+}
+
+// The partner operand of the Im product, read from the other part of the same block:
+// lanes 0-31 (Re lanes) take the Im part as is, lanes 32-63 take the Re part negated
+// (sign bit of every code: bit 5 of each 6-bit E3M2 code, bit 3 of each E2M1 nibble).
+template <int OPF>
+__device__ __forceinline__ v8i mx_partner(v8i x, bool negate) {
+    if constexpr (OPF == OPF_MX6) {
+        const uint32_t m0 = negate ? 0x20820820u : 0u, m1 = negate ? 0x08208208u : 0u,
+                       m2 = negate ? 0x82082082u : 0u;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) x[d] ^= (int)(d % 3 == 0 ? m0 : d % 3 == 1 ? m1 : m2);
+    } else {
+        const uint32_t m = negate ? 0x88888888u : 0u;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) x[d] ^= (int)m;
+    }
+    return x;
 }
 
 // lower-triangle tile index t -> (bi, bj), bj <= bi, t = bi(bi+1)/2 + bj

@@ -62,7 +62,8 @@ struct HerkPlainArgs {
     int64_t a_bstride;  // bytes of A per batch (N*K)
     int64_t c_bstride;  // scalars of C per batch (N*(N+1))
     int precast = 0;    // 1: A is pre-cast FP8 (dword-planar) rows, 2: pre-cast E3M2 rows
-                        // (32 bytes per 16 samples); row pitch lda bytes
+                        // (32 bytes per 16 samples), 3 / 4: MX E3M2 / E2M1 rows (64 bytes
+                        // per 32-sample block, scales inline); row pitch lda bytes
     int64_t lda = 0;
     // Sum over K segments (QC input only): output batch b = ob * seg_group + ib sums
     // the HERKs of input batches (ob * nseg + s) * seg_group + ib, s < nseg, as one
@@ -99,6 +100,9 @@ struct GemmArgs {
     // OUT_GEMM_BEAM: samples per output (divides 64), valid samples per batch (rows
     // of pol 0); alpha is the quantization scale.
     int beam_nt = 0, beam_samples = 0;
+    // operand rows: 0 pre-cast FP8, 3 / 4 MX E3M2 / E2M1 rows (K = padded samples,
+    // OUT_GEMM_PLANAR only)
+    int operand = 0;
 };
 
 // fmt = OUT_GEMM_PLANAR (float or, out_fp16, half planes) or OUT_GEMM_POWER (float).

@@ -124,6 +124,24 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
     const bool vec = (a.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15u) == 0);
     dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    if (a.precast == 3 || a.precast == 4) {
+        if (out_fp16) {
+            if (a.precast == 3)
+                hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_MX6>), grid, block, 0,
+                                   stream, a, tiles, xcd_remap);
+            else
+                hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_MX4>), grid, block, 0,
+                                   stream, a, tiles, xcd_remap);
+        } else {
+            if (a.precast == 3)
+                hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_MX6>), grid, block, 0,
+                                   stream, a, tiles, xcd_remap);
+            else
+                hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_MX4>), grid, block, 0,
+                                   stream, a, tiles, xcd_remap);
+        }
+        return hipGetLastError();
+    }
     if (a.precast == 2) {
         if (out_fp16)
             hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_BF6>), grid, block, 0,
@@ -164,7 +182,7 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
 // tile grid in column-major tile order within a batch (consecutive workgroups share
 // the column block of B, the large operand in the beamformer shapes, while the few
 // row blocks of A stay in L2), XCD-aware order when asked.
-template <class Cfg, typename OutT, int FMT>
+template <class Cfg, typename OutT, int FMT, int MODE = LOAD_FP8>
 __global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
 gemm_fp8_kernel(GemmArgs p, int tiles_m, int tiles, int xcd_remap) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
@@ -191,7 +209,7 @@ gemm_fp8_kernel(GemmArgs p, int tiles_m, int tiles, int xcd_remap) {
     j.nseg = 1;
     j.beam_nt = p.beam_nt;
     j.beam_samples = p.beam_samples;
-    TileRunner<Cfg>::template run<OutT, LOAD_FP8, FMT>(j, lds);
+    TileRunner<Cfg>::template run<OutT, MODE, FMT>(j, lds);
 }
 
 template <class Cfg>
@@ -201,6 +219,26 @@ hipError_t launch_gemm_cfg(const GemmArgs& a, int fmt, bool out_fp16, int xcd_re
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
     dim3 grid((unsigned)blocks), block(Cfg::THREADS);
     const int tm = (int)tiles_m, tiles = (int)(tiles_m * tiles_n);
+    if (a.operand != 0) {  // MX operand rows: planar output on the 128 x 128 tile
+        if constexpr (Cfg::TM == 128 && Cfg::TN == 128 && Cfg::KC == 64 && Cfg::DEPTH == 1) {
+            if (fmt != OUT_GEMM_PLANAR) return hipErrorInvalidValue;
+            if (a.operand == 3 && out_fp16)
+                hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, _Float16, OUT_GEMM_PLANAR, LOAD_MX6>), grid, block, 0,
+                                   stream, a, tm, tiles, xcd_remap);
+            else if (a.operand == 3)
+                hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_PLANAR, LOAD_MX6>), grid, block, 0,
+                                   stream, a, tm, tiles, xcd_remap);
+            else if (out_fp16)
+                hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, _Float16, OUT_GEMM_PLANAR, LOAD_MX4>), grid, block, 0,
+                                   stream, a, tm, tiles, xcd_remap);
+            else
+                hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_PLANAR, LOAD_MX4>), grid, block, 0,
+                                   stream, a, tm, tiles, xcd_remap);
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
     if (fmt == OUT_GEMM_POWER)
         hipLaunchKernelGGL((gemm_fp8_kernel<Cfg, float, OUT_GEMM_POWER>), grid, block, 0, stream, a, tm,
                            tiles, xcd_remap);

@@ -1,4 +1,4 @@
-// herk_mx.h -- host-side interface of the MX block-scaled HERK kernels (internal).
+// herk_mx.h -- host-side interface of the MX block-scaled quantizer (internal).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -22,31 +22,10 @@ hipError_t launch_mx_quantize(const void* X, int input, int fmt, uint8_t* codes,
 hipError_t launch_mx_quantize_ex(const void* X, const void* X2, int input, int fmt, bool conj,
                                  uint8_t* codes, uint8_t* scales, int64_t rows, int K, hipStream_t s);
 
-// C = alpha A B^T + beta C on MX codes of A ([batch][M] rows) and of conj(B) ([batch][N]
-// rows), both quantized with the same K; planar outputs Cre / Cim [batch][M][ldc].
-struct MxGemmArgs {
-    const uint8_t* acodes;
-    const uint8_t* ascales;
-    const uint8_t* bcodes;
-    const uint8_t* bscales;
-    void* Cre;
-    void* Cim;
-    int M, N, batch, nblkp;
-    float alpha, beta;
-    int64_t ldc, c_bstride;
-};
-
-hipError_t launch_gemm_mx(const MxGemmArgs& a, int fmt, bool out_fp16, hipStream_t s);
-
-struct MxHerkArgs {
-    const uint8_t* codes;   // [batch][N][nblkp][2][PB]
-    const uint8_t* scales;  // [batch][N][nblkp][2]
-    void* C;
-    int N, batch, nblkp;
-    float alpha, beta;
-    int64_t c_bstride;      // scalars of C per batch
-};
-
-hipError_t launch_herk_mx(const MxHerkArgs& a, int fmt, bool out_fp16, hipStream_t s);
+// The operand-row layout of the HERK / GEMM tile (LOAD_MX6 / LOAD_MX4): rows_out [rows]
+// [nblkp][2][32] bytes, each 32-byte part = the 32 codes, the E8M0 scale byte, zeros; row
+// pitch nblkp * 64 bytes (the tile's pre-cast geometry for nblkp * 32 samples).
+hipError_t launch_mx_quantize_rows(const void* X, const void* X2, int input, int fmt, bool conj, uint8_t* rows_out,
+                                   int64_t rows, int K, hipStream_t s);
 
 }  // namespace ftdx

@@ -72,7 +72,9 @@ __device__ __forceinline__ void wait_vmcnt() {
 // 8 bytes per 4 samples in the dword-planar order), copied to LDS as they are.
 // LOAD_BF6: pre-cast E3M2 rows with the same geometry (one 32-byte slot per 16
 // samples: 12 bytes of Re codes, 12 of Im codes, 8 of padding), MFMAs in E3M2.
-enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1, LOAD_FP8 = 2, LOAD_BF6 = 3 };
+// LOAD_MX6 / LOAD_MX4: MX block-scaled operand rows (OPF_MX6 / OPF_MX4 in herk_common.h),
+// the same 64 bytes per 32 samples, MFMAs scaled per lane.
+enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1, LOAD_FP8 = 2, LOAD_BF6 = 3, LOAD_MX6 = 4, LOAD_MX4 = 5 };
 
 struct TileJob {
     const uint8_t* A;  // this problem's operand rows
@@ -567,6 +569,23 @@ struct TileRunner {
 #pragma unroll
             for (int b = 0; b < Cfg::FN; ++b)
                 bu[b] = lds_read32(st + b_off + b * 32 * Cfg::PITCH + ks * 64);
+            if constexpr (opf_mx(OPF)) {
+                static_assert(FMT == OUT_LOWER_PACKED || gemm_layout(FMT), "MX rows: row-major layouts");
+                // the Im partner is the block's other part, 32 bytes away
+                const bool hi = (threadIdx.x & 32) != 0;
+                const int dv = hi ? -32 : 32;
+#pragma unroll
+                for (int a = 0; a < Cfg::FM; ++a) {
+                    const v8i av =
+                        mx_partner<OPF>(lds_read32(st + a_off + dv + a * 32 * Cfg::PITCH + ks * 64), hi);
+#pragma unroll
+                    for (int b = 0; b < Cfg::FN; ++b) {
+                        acc_re[a][b] = mfma_32x32x64_mx<OPF>(au[a], bu[b], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64_mx<OPF>(av, bu[b], acc_im[a][b]);
+                    }
+                }
+                continue;
+            }
 #pragma unroll
             for (int a = 0; a < Cfg::FM; ++a) {
                 const v8i av = conj_op<OPF>(au[a]);
@@ -763,6 +782,27 @@ struct TileRunner {
                 bu[b][0] = (int)x.x; bu[b][1] = (int)x.y; bu[b][2] = (int)x.z; bu[b][3] = (int)x.w;
                 bu[b][4] = (int)y.x; bu[b][5] = (int)y.y; bu[b][6] = (int)y.z; bu[b][7] = (int)y.w;
             }
+            if constexpr (opf_mx(OPF)) {
+                static_assert(FMT == OUT_LOWER_PACKED || gemm_layout(FMT), "MX rows: row-major layouts");
+                // the block's other part: logical chunk ^ 2, so byte offset ^ 32 under the swizzle
+                const bool hi = half != 0;
+#pragma unroll
+                for (int a = 0; a < Cfg::FM; ++a) {
+                    const uint8_t* r = st + (a_row + a * 32) * DROW;
+                    const uint4 x = *reinterpret_cast<const uint4*>(r + (p0 ^ 32));
+                    const uint4 y = *reinterpret_cast<const uint4*>(r + (p1 ^ 32));
+                    v8i v;
+                    v[0] = (int)x.x; v[1] = (int)x.y; v[2] = (int)x.z; v[3] = (int)x.w;
+                    v[4] = (int)y.x; v[5] = (int)y.y; v[6] = (int)y.z; v[7] = (int)y.w;
+                    const v8i av = mx_partner<OPF>(v, hi);
+#pragma unroll
+                    for (int b = 0; b < Cfg::FN; ++b) {
+                        acc_re[a][b] = mfma_32x32x64_mx<OPF>(au[a], bu[b], acc_re[a][b]);
+                        acc_im[a][b] = mfma_32x32x64_mx<OPF>(av, bu[b], acc_im[a][b]);
+                    }
+                }
+                continue;
+            }
 #pragma unroll
             for (int a = 0; a < Cfg::FM; ++a) {
                 const v8i av = conj_op<OPF>(au[a]);
@@ -812,9 +852,14 @@ struct TileRunner {
     template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false>
     static __device__ __forceinline__ void run(const TileJob& j, uint8_t* __restrict__ lds) {
         constexpr bool GEMM = gemm_layout(FMT);
-        static_assert(!GEMM || MODE == LOAD_FP8, "GEMM layouts read pre-cast FP8 rows");
-        constexpr bool PRE = MODE == LOAD_FP8 || MODE == LOAD_BF6;  // pre-cast operand rows
-        constexpr int OPF = MODE == LOAD_BF6 ? OPF_BF6 : OPF_FP8;
+        static_assert(!GEMM || MODE == LOAD_FP8 || MODE == LOAD_MX6 || MODE == LOAD_MX4,
+                      "GEMM layouts read pre-cast rows");
+        constexpr bool PRE = MODE == LOAD_FP8 || MODE == LOAD_BF6 || MODE == LOAD_MX6 ||
+                             MODE == LOAD_MX4;  // pre-cast operand rows
+        constexpr int OPF = MODE == LOAD_BF6   ? OPF_BF6
+                            : MODE == LOAD_MX6 ? OPF_MX6
+                            : MODE == LOAD_MX4 ? OPF_MX4
+                                               : OPF_FP8;
         const int tid = threadIdx.x;
         // wave index in an SGPR: the per-wave conditions below are then scalar branches
         const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
