@@ -19,8 +19,11 @@
 //                 Im C_ij = sum (im_i re_j - re_i im_j)     A = [im | -re], B = [re | im]
 //              each lane's 32 values share one scale, which the MFMA applies.
 // Element conversion: nearest-even with subnormals, saturating at the format
-// maximum (OCP Microscaling Formats v1.0); codes per value are packed LSB
-// first (FP4: nibble i at bits 4i; FP6: bits 6i..6i+5 of the lane's dwords).
+// maximum (OCP Microscaling Formats v1.0), by the gfx950 converters
+// (v_cvt_scalef32_pk_fp4_f32, v_cvt_scalef32_2xpk16_bf6_f32) on the values times the
+// block's inverse scale; checked against the CPU restatement's codes for every case the
+// tests cover.  Codes per value are packed LSB first (FP4: nibble i at bits 4i; FP6:
+// bits 6i..6i+5 of the lane's dwords).
 #include <hip/hip_runtime.h>
 
 #include "herk_common.h"
@@ -28,6 +31,8 @@
 #include "herk_tile.h"
 
 namespace ftdx {
+
+typedef int v6i __attribute__((ext_vector_type(6)));
 
 namespace {
 
@@ -38,7 +43,6 @@ template <>
 struct MxFmt<MX_FP4_E2M1> {
     static constexpr int PB = 16;        // bytes per 32 codes
     static constexpr int DW = 4;         // dwords per lane operand
-    static constexpr int MBITS = 1, BIAS = 1, SIGN_SHIFT = 3, MAXCODE = 7;
     static constexpr float MAXV = 6.0f;
     static __device__ __forceinline__ uint32_t neg(int d) { (void)d; return 0x88888888u; }
 };
@@ -47,43 +51,12 @@ template <>
 struct MxFmt<MX_FP6_E3M2> {
     static constexpr int PB = 24;
     static constexpr int DW = 6;
-    static constexpr int MBITS = 2, BIAS = 3, SIGN_SHIFT = 5, MAXCODE = 31;
     static constexpr float MAXV = 28.0f;
     // sign bits at 6i+5 of the 192-bit lane operand, per dword
     static __device__ __forceinline__ uint32_t neg(int d) {
         return (d % 3 == 0) ? 0x20820820u : (d % 3 == 1) ? 0x08208208u : 0x82082082u;
     }
 };
-
-// Value -> code, nearest-even with subnormals, saturating (same steps as the
-// CPU restatement, so codes are bit-identical).
-template <int FMT>
-__device__ __forceinline__ uint32_t mx_encode(float x) {
-    using F = MxFmt<FMT>;
-    const uint32_t sign = __builtin_signbitf(x) ? 1u : 0u;
-    const float a = __builtin_fabsf(x);
-    uint32_t code;
-    if (a >= F::MAXV) {
-        code = F::MAXCODE;
-    } else {
-        int e;
-        __builtin_frexpf(a, &e);
-        e -= 1;
-        const int emin = 1 - F::BIAS;
-        if (a == 0.0f || e < emin) {
-            code = (uint32_t)__builtin_rintf(__builtin_ldexpf(a, F::MBITS - emin));
-        } else {
-            uint32_t q = (uint32_t)__builtin_rintf(__builtin_ldexpf(a, F::MBITS - e));
-            if (q == (2u << F::MBITS)) {
-                q >>= 1;
-                ++e;
-            }
-            code = ((uint32_t)(e + F::BIAS) << F::MBITS) | (q - (1u << F::MBITS));
-        }
-        if (code > (uint32_t)F::MAXCODE) code = F::MAXCODE;
-    }
-    return (sign << F::SIGN_SHIFT) | code;
-}
 
 template <int FMT>
 __device__ __forceinline__ uint32_t mx_scale_exponent(float amax) {
@@ -119,11 +92,12 @@ __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict
                                                           uint8_t* __restrict__ scales,
                                                           int64_t rows, int K, int nblkp, int conj, int vec) {
     using F = MxFmt<FMT>;
+    const int64_t total = rows * nblkp;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= rows * nblkp) return;
+    float v[2][32];
+    if (t >= total) return;
     const int64_t row = t / nblkp;
     const int blk = (int)(t - row * nblkp);
-    float v[2][32];
     if (vec && blk * 32 + 32 <= K) {
         // whole block, 16-byte aligned rows: wide loads of the block's contiguous bytes
         if constexpr (IN == MX_IN_INT4) {
@@ -205,17 +179,28 @@ __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict
         for (int i = 0; i < 32; ++i) amax = __builtin_fmaxf(amax, __builtin_fabsf(v[part][i]));
         const uint32_t se = mx_scale_exponent<FMT>(amax);
         const float inv = __builtin_ldexpf(1.0f, 127 - (int)se);
+        // v * inv is exact (a power of two within range); the gfx950 converters then
+        // round to nearest even with subnormals and saturate, the OCP MX element rule
         uint32_t w[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+        if constexpr (FMT == MX_FP4_E2M1) {
 #pragma unroll
-        for (int i = 0; i < 32; ++i) {
-            const uint32_t c = mx_encode<FMT>(v[part][i] * inv);
-            if (FMT == MX_FP4_E2M1) {
-                w[i >> 3] |= c << (4 * (i & 7));
-            } else {
-                const int bit = 6 * i, d = bit >> 5, o = bit & 31;
-                w[d] |= c << o;
-                if (o > 26) w[d + 1] |= c >> (32 - o);
+            for (int i = 0; i < 32; i += 2) {
+                const uint32_t b = __builtin_amdgcn_cvt_scalef32_pk_fp4_f32(0u, v[part][i] * inv, v[part][i + 1] * inv,
+                                                                            1.0f, 0) & 0xFFu;
+                w[i >> 3] |= b << (4 * (i & 7));
             }
+        } else {
+            // the 2 x 16 form interleaves its sources: code 2i from the first, 2i + 1 from
+            // the second, code k at bits 6k .. 6k + 5
+            v16f ev, od;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                ev[i] = v[part][2 * i] * inv;
+                od[i] = v[part][2 * i + 1] * inv;
+            }
+            const v6i c = __builtin_amdgcn_cvt_scalef32_2xpk16_bf6_f32(ev, od, 1.0f);
+#pragma unroll
+            for (int d = 0; d < 6; ++d) w[d] = (uint32_t)c[d];
         }
         if constexpr (ROWS) {
             uint32_t o[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
