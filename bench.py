@@ -326,6 +326,55 @@ def voltage_study(torch, ftdx, dev, n_ant: int, n_beam: int, n_ch: int, n_time: 
             "mean_output": round(float(out.float().mean()), 2)}
 
 
+def stage_study(torch, ftdx, dev, N: int, K: int, B: int, seed: int, reps: int, cfg: int, remap: int) -> dict:
+    """Stand-alone unpack / pack stages (fused into the HERK on its main path) against the
+    HBM peak: QC -> FP8 pairs over the headline input, QC -> FTD byte conversion, the
+    N x N -> packed-triangle packs (lower with beta, GGP upper) over 8 channels, and the
+    unfused (ForceBaseline: pre-cast rows, then the tile) HERK against the fused one, both on
+    the headline's tile configuration.  Bytes are algorithmic: the packs read only the
+    triangle of the N x N input."""
+    qc = synth_qc(torch, N * K * B, seed, dev)
+    n = qc.numel()
+    full = torch.randn((8, N, N, 2), device=dev)
+    old = torch.randn((8, N * (N + 1) // 2, 2), device=dev)
+    res = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    P = N * (N + 1) // 2
+    cases = [
+        ("promote_qc_to_fp8", lambda: ftdx.promote_qc_to_fp8(qc), 3 * n),
+        ("convert_qc_to_ftd", lambda: ftdx.convert_qc_to_ftd(qc), 2 * n),
+        ("pack_triangle_beta", lambda: ftdx.pack_triangle(full, N, 8, old=old, beta=0.5), 8 * 3 * P * 8),
+        ("tri_from_herm", lambda: ftdx.tri_from_herm(full, N, 8), 8 * 2 * P * 8),
+    ]
+    for name, fn, nbytes in cases:
+        ms = timed(fn)
+        res[name] = {"ms": round(ms, 4), "bytes": int(nbytes), "gb_per_s": round(nbytes / ms / 1e6, 1),
+                     "frac_hbm_peak": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3)}
+    del full, old
+    torch.cuda.empty_cache()
+    C = torch.empty((B, P, 2), dtype=torch.float32, device=dev)
+    for mode, key in ((ftdx.HerkMode.ForceDirect, "herk_fused_ms"), (ftdx.HerkMode.ForceBaseline, "herk_unfused_ms")):
+        ctx = ftdx.CutlassComplexGemm()
+        ctx.set_strategy_tune_verbosity(0)
+        ctx.set_herk_mode(mode)
+        ctx.set_launch_config(cfg, remap)
+        res[key] = round(timed(lambda: ctx.herk(qc, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                                                ftdx.OutputPrecision.FP32, tune=False)), 4)
+    res["inputs"], res["K"], res["channels"], res["seed"], res["tile_config"] = N, K, B, seed, cfg
+    return res
+
+
 def pol_reduce_study(torch, ftdx, dev, N: int, K: int, n_ch: int, n_pol: int, n_ti: int, seed: int,
                      reps: int) -> dict:
     """HERK + polarization sum: one fused pass (ftdx_herk_pol_reduce) against the HERK of
@@ -489,6 +538,7 @@ def main() -> int:
     ap.add_argument("--no-beamformer", action="store_true")
     ap.add_argument("--no-polsum", action="store_true")
     ap.add_argument("--no-voltage", action="store_true")
+    ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--no-bf6", action="store_true")
     ap.add_argument("--batched-problems", type=int, default=512)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -623,6 +673,14 @@ def main() -> int:
         log(f"bench: beamformer {bf['ms']:.3f} ms, {bf['tflops_4m_equiv']:.0f} TFLOP/s 4M-equiv")
         torch.cuda.empty_cache()
 
+    stages = None
+    if not args.no_stages:
+        log("bench: stand-alone unpack / pack stages vs HBM peak ...")
+        stages = stage_study(torch, ftdx, dev, N, K, B, args.seed, 5, cfg, remap)
+        log("bench: " + ", ".join(f"{k} {v['gb_per_s']:.0f} GB/s" for k, v in stages.items() if isinstance(v, dict))
+            + f"; HERK fused {stages['herk_fused_ms']:.3f} ms vs unfused {stages['herk_unfused_ms']:.3f} ms")
+        torch.cuda.empty_cache()
+
     volt = None
     if not args.no_voltage:
         log("bench: voltage pipeline 1664 ant x 4000 beams x 128 ch, 256 times, n_tps 4 ...")
@@ -693,6 +751,7 @@ def main() -> int:
         "beamformer_gemm_power": bf,
         "herk_pol_sum_fusion": polsum,
         "voltage_pipeline": volt,
+        "stages_vs_hbm_peak": stages,
         "e3m2_operands": bf6,
         "cpu_baseline": cpu,
     }
