@@ -37,11 +37,9 @@ struct ftdx_context {
     bool tune_loaded = false;
     ftdx::TuneCache tune;
     int init_N = 0, init_K = 0, init_batch = 0;
-    // MX path scratch (quantized codes and scales), grown on demand
+    // MX path scratch (operand rows: codes with their scales inline), grown on demand
     uint8_t* mx_codes = nullptr;
     size_t mx_codes_cap = 0;
-    uint8_t* mx_scales = nullptr;
-    size_t mx_scales_cap = 0;
     // calibration scratch: gradient partials, loss partials, dense Hermitian matrix
     uint8_t* cal_part = nullptr;
     size_t cal_part_cap = 0;
@@ -502,11 +500,10 @@ int ftdx_end_herk(ftdx_context* ctx) {
     ctx->init_N = ctx->init_K = ctx->init_batch = 0;
     bool any_graph = false;
     for (auto& gph : ctx->graphs) any_graph = any_graph || gph.used;
-    if (ctx->mx_codes || ctx->mx_scales || ctx->cal_part || ctx->cal_herm || ctx->precast ||
+    if (ctx->mx_codes || ctx->cal_part || ctx->cal_herm || ctx->precast ||
         any_graph) {
         (void)hipDeviceSynchronize();
         if (ctx->mx_codes) (void)hipFree(ctx->mx_codes);
-        if (ctx->mx_scales) (void)hipFree(ctx->mx_scales);
         if (ctx->cal_part) (void)hipFree(ctx->cal_part);
         if (ctx->cal_herm) (void)hipFree(ctx->cal_herm);
         if (ctx->precast) (void)hipFree(ctx->precast);
@@ -519,8 +516,8 @@ int ftdx_end_herk(ftdx_context* ctx) {
         (void)hipStreamDestroy(ctx->cap_stream);
         ctx->cap_stream = nullptr;
     }
-    ctx->mx_codes = ctx->mx_scales = nullptr;
-    ctx->mx_codes_cap = ctx->mx_scales_cap = 0;
+    ctx->mx_codes = nullptr;
+    ctx->mx_codes_cap = 0;
     ctx->cal_part = ctx->cal_herm = nullptr;
     ctx->cal_part_cap = ctx->cal_herm_cap = 0;
     ctx->precast = nullptr;
