@@ -101,6 +101,53 @@ int CutlassComplexGemm::gemm_prepared(const void* A_re, const void* A_im, void* 
                  "gemm_prepared");
 }
 
+int CutlassComplexGemm::gemm_prepared_fused(const void* A_re, const void* A_im, void* C_re, void* C_im, int M,
+                                            int N, int K, int batch_count, int fuse_factor, OutputPrecision output,
+                                            float alpha, float beta, hipStream_t stream) {
+    // every failure throws in the reference (cutlass_gemm_api.cu:982-995)
+    check_void(ftdx_gemm_prepared_fused(ctx_, A_re, A_im, C_re, C_im, M, N, K, batch_count, fuse_factor,
+                                        static_cast<int>(output), alpha, beta, stream),
+               "gemm_prepared_fused");
+    return 0;
+}
+
+int CutlassComplexGemm::debug_fp6_real_gemm(int M, int N, int K, hipStream_t stream) {
+    const int rc = ftdx_debug_fp6_real_gemm(ctx_, M, N, K, stream);
+    return rc == FTDX_OK ? 0 : (rc == FTDX_ERR_INVALID_ARG ? 1 : -1);
+}
+
+void CutlassComplexGemm::set_gemm_tune_verbosity(int level) {
+    check_void(ftdx_set_gemm_setting(ctx_, FTDX_GEMM_SETTING_TUNE_VERBOSITY, level < 0 ? 0 : level),
+               "set_gemm_tune_verbosity");
+}
+
+void CutlassComplexGemm::set_gemm_tune_cache_path(const std::string& path) {
+    check_void(ftdx_set_gemm_tune_cache_path(ctx_, path.c_str()), "set_gemm_tune_cache_path");
+}
+
+void CutlassComplexGemm::set_tile_config(TileConfig config) {
+    check_void(ftdx_set_gemm_setting(ctx_, FTDX_GEMM_SETTING_TILE_CONFIG, static_cast<int>(config)),
+               "set_tile_config");
+}
+
+void CutlassComplexGemm::set_power_mode(PowerMode mode) {
+    check_void(ftdx_set_gemm_setting(ctx_, FTDX_GEMM_SETTING_POWER_MODE, static_cast<int>(mode)), "set_power_mode");
+}
+
+void CutlassComplexGemm::set_gemm_mode(GemmMode mode) {
+    check_void(ftdx_set_gemm_setting(ctx_, FTDX_GEMM_SETTING_GEMM_MODE, static_cast<int>(mode)), "set_gemm_mode");
+}
+
+GemmMode CutlassComplexGemm::get_gemm_mode() const {
+    int v = 0;
+    check_void(ftdx_get_gemm_setting(ctx_, FTDX_GEMM_SETTING_GEMM_MODE, &v), "get_gemm_mode");
+    return static_cast<GemmMode>(v);
+}
+
+void CutlassComplexGemm::set_auto_tune(bool enable) {
+    check_void(ftdx_set_gemm_setting(ctx_, FTDX_GEMM_SETTING_AUTO_TUNE, enable ? 1 : 0), "set_auto_tune");
+}
+
 int CutlassComplexGemm::gemm_prepared_direct(const void* A_re, const void* A_im, float* C_re, float* C_im,
                                              int M, int N, int K, int batch_count, float alpha,
                                              float beta, hipStream_t stream) {

@@ -68,6 +68,9 @@ struct ftdx_context {
     size_t prep_b_cap = 0;
     int prep_N = -1, prep_K = -1, prep_batch = -1;
     int prep_fmt = -1;  // -1: FP8 rows, else the MxFormat of the codes
+    // the reference's GEMM dispatch settings, kept for its getters (one kernel family here)
+    int gemm_settings[5] = {0, 0, 0, 1, 1};
+    std::string gemm_tune_cache_path;
     uint8_t* gemm_b = nullptr;
     size_t gemm_b_cap = 0;
 };
@@ -1173,6 +1176,159 @@ int ftdx_gemm_prepared_power_int4(ftdx_context* ctx, const void* A_int4, void* C
     return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_INT4, A_int4, nullptr, C_power, nullptr, M, N, K,
                           batch_count, ftdx::OUT_GEMM_POWER, false, alpha, beta, stream,
                           "ftdx_gemm_prepared_power_int4");
+}
+
+namespace {
+
+// fuse factor 0: the reference's choice (cutlass_gemm_api.cu:951-972): M already >= 256
+// stays unfused; else the largest power of two up to 32 that divides the batch with
+// M_fused <= 1024, then, if M_fused < 128, the smallest divisor reaching 128 (up to 128).
+int auto_fuse(int M, int batch) {
+    if (M <= 0 || batch <= 0 || M >= 256) return 1;
+    int ff = 1;
+    for (int f = 2; f <= 32 && M * f <= 1024; f *= 2)
+        if (batch % f == 0) ff = f;
+    if (M * ff < 128)
+        for (int f = ff + 1; f <= 128; ++f)
+            if (batch % f == 0 && M * f >= 128) return f;
+    return ff;
+}
+
+}  // namespace
+
+int ftdx_gemm_prepared_fused(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_re, void* C_im,
+                             int M, int N, int K, int batch_count, int fuse_factor, int output, float alpha,
+                             float beta, void* stream) {
+    const char* W = "ftdx_gemm_prepared_fused";
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": NULL context");
+    if (output != FTDX_OUTPUT_FP32)
+        return fail(FTDX_ERR_UNSUPPORTED, "gemm_prepared_fused: only FP32 output is currently supported");
+    if (M < 0 || batch_count < 0) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": negative size");
+    const int ff = fuse_factor <= 0 ? auto_fuse(M, batch_count) : fuse_factor;
+    if (batch_count > 0 && batch_count % ff != 0)
+        return fail(FTDX_ERR_INVALID_ARG, "gemm_prepared_fused: fuse_factor must divide batch_count evenly");
+    if (ff == 1)
+        return ftdx_gemm_prepared(ctx, A_re, A_im, C_re, C_im, M, N, K, batch_count, output, alpha, beta, stream, 0,
+                                  FTDX_INPUT_FP16);
+    if (!ctx->prep_b || ctx->prep_N != N || ctx->prep_K != K)
+        return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": prepare_b() must have been called with matching N, K");
+    if (ctx->prep_fmt >= 0)
+        return fail(FTDX_ERR_UNSUPPORTED, std::string(W) + ": prepare_b() must have been called with FP8 compute");
+    const int eb = batch_count / ff;
+    if (eb > ctx->prep_batch)
+        return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": more fused batches than prepared weight sets");
+    if ((int64_t)M * ff > 0x7FFFFFFF) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": sizes too large");
+    return ftdx::gemm_run(ctx, ctx->prep_b, ftdx::GEMM_A_FP16_PLANAR, A_re, A_im, C_re, C_im, M * ff, N, K, eb,
+                          ftdx::OUT_GEMM_PLANAR, false, alpha, beta, stream, W);
+}
+
+int ftdx_set_gemm_setting(ftdx_context* ctx, int setting, int value) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_gemm_setting: NULL context");
+    static const int limits[5] = {8, 2, 2, 1, 1 << 30};
+    if (setting < 0 || setting > 4) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_gemm_setting: unknown setting");
+    if (value < 0 || value > limits[setting]) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_gemm_setting: bad value");
+    ctx->gemm_settings[setting] = value;
+    return FTDX_OK;
+}
+
+int ftdx_get_gemm_setting(const ftdx_context* ctx, int setting, int* value) {
+    if (!ctx || !value) return fail(FTDX_ERR_INVALID_ARG, "ftdx_get_gemm_setting: NULL argument");
+    if (setting < 0 || setting > 4) return fail(FTDX_ERR_INVALID_ARG, "ftdx_get_gemm_setting: unknown setting");
+    *value = ctx->gemm_settings[setting];
+    return FTDX_OK;
+}
+
+int ftdx_set_gemm_tune_cache_path(ftdx_context* ctx, const char* path) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_gemm_tune_cache_path: NULL context");
+    ctx->gemm_tune_cache_path = path ? path : "";
+    return FTDX_OK;
+}
+
+int ftdx_debug_fp6_real_gemm(ftdx_context* ctx, int M, int N, int K, void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_debug_fp6_real_gemm: NULL context");
+    if (M <= 0 || N <= 0 || K <= 0 || (int64_t)M * K > (1 << 26) || (int64_t)N * K > (1 << 26) ||
+        (int64_t)M * N > (1 << 26))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_debug_fp6_real_gemm: sizes must be positive and modest");
+    std::fprintf(stderr, "\n=== debug_fp6_real_gemm M=%d N=%d K=%d ===\n", M, N, K);
+    const size_t sA = (size_t)M * K, sB = (size_t)N * K, sC = (size_t)M * N;
+    // integers in [-7, 7] from a fixed linear congruential stream
+    std::vector<uint16_t> hA(sA), hB(sB), hz(std::max(sA, sB), 0);
+    std::vector<double> ref(sC, 0.0);
+    std::vector<int> a(sA), b(sB);
+    uint64_t st = 42;
+    auto next = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (int)((st >> 33) % 15) - 7;
+    };
+    auto f16 = [](int v) {  // exact FP16 bits of a small integer
+        if (v == 0) return (uint16_t)0;
+        const uint16_t sgn = v < 0 ? 0x8000 : 0;
+        int m = v < 0 ? -v : v, e = 0;
+        while ((m >> (e + 1)) != 0) ++e;
+        return (uint16_t)(sgn | ((e + 15) << 10) | ((m << (10 - e)) & 0x3FF));
+    };
+    for (size_t i = 0; i < sA; ++i) hA[i] = f16(a[i] = next());
+    for (size_t i = 0; i < sB; ++i) hB[i] = f16(b[i] = next());
+    for (int m = 0; m < M; ++m)
+        for (int n = 0; n < N; ++n) {
+            double acc = 0.0;
+            for (int k = 0; k < K; ++k) acc += (double)a[(size_t)m * K + k] * b[(size_t)n * K + k];
+            ref[(size_t)m * N + n] = acc;
+        }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void *dA = nullptr, *dB = nullptr, *dz = nullptr, *dC = nullptr, *dCi = nullptr;
+    hipError_t e = hipMalloc(&dA, sA * 2);
+    if (e == hipSuccess) e = hipMalloc(&dB, sB * 2);
+    if (e == hipSuccess) e = hipMalloc(&dz, hz.size() * 2);
+    if (e == hipSuccess) e = hipMalloc(&dC, sC * 4);
+    if (e == hipSuccess) e = hipMalloc(&dCi, sC * 4);
+    if (e == hipSuccess) e = hipMemcpy(dA, hA.data(), sA * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB, hB.data(), sB * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dz, hz.data(), hz.size() * 2, hipMemcpyHostToDevice);
+    int result = e == hipSuccess ? 0 : -1;
+    if (result == 0) {
+        for (int pass = 0; pass < 2 && result == 0; ++pass) {
+            const int out = pass == 0 ? FTDX_OUTPUT_FP16 : FTDX_OUTPUT_FP32;
+            int rc = ftdx_gemm(ctx, dA, dz, dB, dz, dC, dCi, M, N, K, 1, FTDX_COMPUTE_FP6, out, 1.0f, 0.0f, s, 0,
+                               FTDX_INPUT_FP16);
+            if (rc == FTDX_OK) e = hipStreamSynchronize(s);
+            if (rc != FTDX_OK || e != hipSuccess) {
+                result = -1;
+                break;
+            }
+            double max_ref = 0.0, max_err = 0.0;
+            if (pass == 0) {
+                std::vector<uint16_t> h(sC);
+                if (hipMemcpy(h.data(), dC, sC * 2, hipMemcpyDeviceToHost) != hipSuccess) { result = -1; break; }
+                for (size_t i = 0; i < sC; ++i) {
+                    _Float16 v;
+                    std::memcpy(&v, &h[i], 2);
+                    max_ref = std::max(max_ref, std::fabs(ref[i]));
+                    max_err = std::max(max_err, std::fabs((double)(float)v - ref[i]));
+                }
+            } else {
+                std::vector<float> h(sC);
+                if (hipMemcpy(h.data(), dC, sC * 4, hipMemcpyDeviceToHost) != hipSuccess) { result = -1; break; }
+                for (size_t i = 0; i < sC; ++i) {
+                    max_ref = std::max(max_ref, std::fabs(ref[i]));
+                    max_err = std::max(max_err, std::fabs((double)h[i] - ref[i]));
+                }
+            }
+            // FP16 output rounds sums beyond 2048 to its 11-bit precision
+            const double tol = pass == 0 ? max_ref * 1e-3 : 0.0;
+            const bool ok = max_err <= tol;
+            std::fprintf(stderr, "  %s output: max_ref=%.1f max_err=%.2f  %s\n", pass == 0 ? "FP16" : "FP32",
+                         max_ref, max_err, ok ? "PASS" : "FAIL");
+            if (!ok) result = 1;
+        }
+    }
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dz);
+    (void)hipFree(dC);
+    (void)hipFree(dCi);
+    if (result < 0) return hip_fail(e == hipSuccess ? hipErrorUnknown : e, "ftdx_debug_fp6_real_gemm");
+    return result == 0 ? FTDX_OK : fail(FTDX_ERR_INVALID_ARG, "debug_fp6_real_gemm: results differ");
 }
 
 int ftdx_gemm(ftdx_context* ctx, const void* A_re, const void* A_im, const void* B_re, const void* B_im,

@@ -160,6 +160,11 @@ _SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                 ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "ftdx_corner_turn": (_I, [_VP, _VP, _I, _I, _I, _VP]),
+    "ftdx_gemm_prepared_fused": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _F, _F, _VP]),
+    "ftdx_debug_fp6_real_gemm": (_I, [_VP, _I, _I, _I, _VP]),
+    "ftdx_set_gemm_setting": (_I, [_VP, _I, _I]),
+    "ftdx_get_gemm_setting": (_I, [_VP, _I, ctypes.POINTER(_I)]),
+    "ftdx_set_gemm_tune_cache_path": (_I, [_VP, _S]),
 }
 
 PREC_FP32 = 1
@@ -175,6 +180,13 @@ TILE_256x128_K64_P2 = 5
 TILE_CONFIGS = (TILE_128x128_K64, TILE_128x128_K128, TILE_64x64_K64, TILE_256x128_K64,
                 TILE_128x128_K64_P2, TILE_256x128_K64_P2)
 TILE_BF6 = 16  # flag: the tile with pre-cast E3M2 operands (ftdx_herk, INT4 input)
+
+# ftdx_set_gemm_setting / ftdx_get_gemm_setting (the reference's GEMM dispatch setters)
+GEMM_SETTING_TILE_CONFIG = 0
+GEMM_SETTING_POWER_MODE = 1
+GEMM_SETTING_GEMM_MODE = 2
+GEMM_SETTING_AUTO_TUNE = 3
+GEMM_SETTING_TUNE_VERBOSITY = 4
 
 
 class HerkProblem(ctypes.Structure):
@@ -363,6 +375,27 @@ class CutlassComplexGemm:
         return self._gemm_rc(lib().ftdx_gemm_prepared_power_fp8(
             self._h, _ptr(A_fp8), _ptr(C_power), M, N, K, batch_count, alpha, beta,
             _stream_ptr(stream)), "gemm_prepared_power_fp8")
+
+    def gemm_prepared_fused(self, A_re, A_im, C_re, C_im, M, N, K, batch_count, fuse_factor=0,
+                            output: OutputPrecision = OutputPrecision.FP32, alpha=1.0, beta=0.0,
+                            stream=None) -> int:
+        # every failure throws in the reference (cutlass_gemm_api.cu:982-995)
+        _check(lib().ftdx_gemm_prepared_fused(
+            self._h, _ptr(A_re), _ptr(A_im), _ptr(C_re), _ptr(C_im), M, N, K, batch_count, fuse_factor,
+            int(output), alpha, beta, _stream_ptr(stream)), "gemm_prepared_fused")
+        return 0
+
+    def debug_fp6_real_gemm(self, M: int, N: int, K: int, stream=None) -> int:
+        rc = lib().ftdx_debug_fp6_real_gemm(self._h, M, N, K, _stream_ptr(stream))
+        return 0 if rc == FTDX_OK else (1 if rc == FTDX_ERR_INVALID_ARG else -1)
+
+    def set_gemm_setting(self, setting: int, value: int) -> None:
+        _check(lib().ftdx_set_gemm_setting(self._h, setting, int(value)), "set_gemm_setting")
+
+    def get_gemm_setting(self, setting: int) -> int:
+        v = ctypes.c_int()
+        _check(lib().ftdx_get_gemm_setting(self._h, setting, ctypes.byref(v)), "get_gemm_setting")
+        return v.value
 
     def gemm_prepared_power_int4(self, A_int4, C_power, M, N, K, batch_count, alpha=1.0,
                                  beta=0.0, stream=None) -> int:

@@ -9,8 +9,8 @@
  * The class is a thin wrapper over the C ABI in ftdx.h (libftdx.so); it is
  * implemented in libftdx_api.so and contains no device code.
  *
- * Only the HERK path is provided; the reference's planar GEMM / power-GEMM
- * methods (cutlass_gemm_api.h:215-470) are outside this library.
+ * The HERK methods, the complex GEMM with prepared B (including power detection
+ * and channel fusion) and the reference's tuning / dispatch setters are provided.
  *
  * Error behaviour (as the reference): herk() returns 0 on success and non-zero
  * on invalid arguments; unsupported precision combinations and HIP failures
@@ -60,6 +60,15 @@ enum class HerkMode {
 /// values select among the tile/pipeline configurations of the HERK kernel.
 enum class DirectTileConfig { Auto, K32_B3, K64_B2, K64_B3, K128_B2 };
 
+/// The reference's CUTLASS-baseline tile, power and GEMM dispatch choices
+/// (cutlass_gemm_api.h:76-136).  This library has one GEMM tile kernel: the values are
+/// kept and returned by get_gemm_mode(), results and kernels do not depend on them.
+enum class TileConfig {
+    Auto, HerkOptimal, GemmOptimal, WideN, Cluster1x2, Cluster2x2, SmallTile, Pingpong, PingpongSmall
+};
+enum class PowerMode { Auto, ForceDirect, Force4M };
+enum class GemmMode { Auto, ForceDirect, Force4M };
+
 class CutlassComplexGemm {
 public:
     CutlassComplexGemm();
@@ -90,14 +99,16 @@ public:
     // FP16 planar operands are passed as `const __half*` by reference callers; any
     // pointer converts to the `const void*` parameters here.
 
-    /// One call, B converted each time (gemm, :215-225).  FP8 compute only.
+    /// One call, B converted each time (gemm, :215-225).  FP8, or FP6 / FP4 on the
+    /// MX block-scaled path.
     int gemm(const void* A_re, const void* A_im, const void* B_re, const void* B_im, void* C_re,
              void* C_im, int M, int N, int K, int batch_count,
              ComputePrecision compute = ComputePrecision::FP8,
              OutputPrecision output = OutputPrecision::FP16, float alpha = 1.0f, float beta = 0.0f,
              hipStream_t stream = nullptr, bool tune = true,
              InputPrecision input = InputPrecision::FP16);
-    /// Converts B once (prepare_b, :251-255); throws for FP6 / FP4.
+    /// Converts B once (prepare_b, :251-255): FP8 rows, or MX codes for FP6 / FP4 (the
+    /// direct and power variants then throw: they need FP8-prepared B).
     void prepare_b(const void* B_re, const void* B_im, int N, int K, int batch_count,
                    ComputePrecision compute, hipStream_t stream = nullptr);
     int gemm_prepared(const void* A_re, const void* A_im, void* C_re, void* C_im, int M, int N,
@@ -119,6 +130,22 @@ public:
     int gemm_prepared_direct_int4(const void* A_int4, float* C_re, float* C_im, int M, int N, int K,
                                   int batch_count, float alpha = 1.0f, float beta = 0.0f,
                                   hipStream_t stream = nullptr);
+
+    /// fuse_factor batch elements per product along M (gemm_prepared_fused, :440-470);
+    /// FP32 output only, FP8-prepared B.
+    int gemm_prepared_fused(const void* A_re, const void* A_im, void* C_re, void* C_im, int M, int N, int K,
+                            int batch_count, int fuse_factor = 0, OutputPrecision output = OutputPrecision::FP32,
+                            float alpha = 1.0f, float beta = 0.0f, hipStream_t stream = nullptr);
+    /// Real FP6 GEMM of small integers against the host product (debug_fp6_real_gemm, :477-490).
+    int debug_fp6_real_gemm(int M, int N, int K, hipStream_t stream = nullptr);
+
+    void set_gemm_tune_verbosity(int level);
+    void set_gemm_tune_cache_path(const std::string& path);
+    void set_tile_config(TileConfig config);
+    void set_power_mode(PowerMode mode);
+    void set_gemm_mode(GemmMode mode);
+    GemmMode get_gemm_mode() const;
+    void set_auto_tune(bool enable);
 
     static void print_build_info();
 

@@ -388,6 +388,34 @@ int ftdx_gemm_prepared_power_fp8(ftdx_context* ctx, const void* A_fp8_interleave
 /* gemm_prepared_power_int4 (cutlass_gemm_api.h:398-401). */
 int ftdx_gemm_prepared_power_int4(ftdx_context* ctx, const void* A_int4, void* C_power, int M, int N,
                                   int K, int batch_count, float alpha, float beta, void* stream);
+/* gemm_prepared_fused (cutlass_gemm_api.h:440-470; cutlass_gemm_api.cu:951-1007): fuse_factor
+ * consecutive batch elements run as one product of M * fuse_factor rows (A is [batch][M][K],
+ * so that is the same memory), effective batch e using prepared weight set e, which equals
+ * gemm_prepared when the prepared sets are one B repeated.  fuse_factor 0 picks the
+ * reference's rule (M_fused in [128, 1024] where the batch allows); 1 is gemm_prepared.
+ * FP16 planar A, FP32 output only (FTDX_ERR_UNSUPPORTED otherwise), FP8-prepared B. */
+int ftdx_gemm_prepared_fused(ftdx_context* ctx, const void* A_re, const void* A_im, void* C_re, void* C_im,
+                             int M, int N, int K, int batch_count, int fuse_factor, int output, float alpha,
+                             float beta, void* stream);
+/* debug_fp6_real_gemm (cutlass_gemm_api.h:477-490): a real M x N x K product of integers in
+ * [-7, 7] through the FP6 complex GEMM (Im = 0), FP16 and FP32 outputs, compared with the
+ * host product (exact; E3M2 holds these integers after block scaling); prints to stderr,
+ * returns 0 when both match. */
+int ftdx_debug_fp6_real_gemm(ftdx_context* ctx, int M, int N, int K, void* stream);
+/* GEMM dispatch settings of the reference (set_tile_config, set_power_mode, set_gemm_mode /
+ * get_gemm_mode, set_auto_tune, set_gemm_tune_verbosity, set_gemm_tune_cache_path;
+ * cutlass_gemm_api.h:540-720).  They choose among the reference's CUTLASS and direct-PTX
+ * GEMM kernels; this library has one GEMM tile kernel, so the values are kept (and
+ * returned by the getter) without changing results or kernels.  Enum values as the
+ * reference's TileConfig (0-8), PowerMode (0-2) and GemmMode (0-2). */
+int ftdx_set_gemm_setting(ftdx_context* ctx, int setting, int value);
+int ftdx_get_gemm_setting(const ftdx_context* ctx, int setting, int* value);
+int ftdx_set_gemm_tune_cache_path(ftdx_context* ctx, const char* path);
+#define FTDX_GEMM_SETTING_TILE_CONFIG 0
+#define FTDX_GEMM_SETTING_POWER_MODE 1
+#define FTDX_GEMM_SETTING_GEMM_MODE 2
+#define FTDX_GEMM_SETTING_AUTO_TUNE 3
+#define FTDX_GEMM_SETTING_TUNE_VERBOSITY 4
 /* gemm (cutlass_gemm_api.h:215-225): one call with B converted each time (does not
  * touch the prepared B). */
 int ftdx_gemm(ftdx_context* ctx, const void* A_re, const void* A_im, const void* B_re, const void* B_im,

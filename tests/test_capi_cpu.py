@@ -175,3 +175,28 @@ def test_gemm_validation_without_device():
                                            None) == ftdx.FTDX_ERR_INVALID_ARG
     assert L.ftdx_gemm(None, None, None, None, None, None, None, 8, 8, 8, 1, 0, 1, 1.0, 0.0, None,
                        0, 0) == ftdx.FTDX_ERR_INVALID_ARG
+
+
+def test_gemm_dispatch_settings_round_trip():
+    """The reference's set_gemm_mode / get_gemm_mode, set_power_mode, set_tile_config,
+    set_auto_tune (cutlass_gemm_api.h:656-720) keep their values; out-of-range values and
+    unknown settings are rejected."""
+    from ftd_amd import ftdx
+    ctx = ftdx.CutlassComplexGemm()  # settings only: no device needed
+    assert ctx.get_gemm_setting(ftdx.GEMM_SETTING_GEMM_MODE) == 0
+    ctx.set_gemm_setting(ftdx.GEMM_SETTING_GEMM_MODE, 2)
+    assert ctx.get_gemm_setting(ftdx.GEMM_SETTING_GEMM_MODE) == 2
+    ctx.set_gemm_setting(ftdx.GEMM_SETTING_TILE_CONFIG, 8)
+    ctx.set_gemm_setting(ftdx.GEMM_SETTING_AUTO_TUNE, 0)
+    assert ctx.get_gemm_setting(ftdx.GEMM_SETTING_AUTO_TUNE) == 0
+    for setting, value in ((ftdx.GEMM_SETTING_GEMM_MODE, 3), (ftdx.GEMM_SETTING_TILE_CONFIG, 9),
+                           (ftdx.GEMM_SETTING_POWER_MODE, -1), (7, 0)):
+        try:
+            ctx.set_gemm_setting(setting, value)
+        except ftdx.FtdxError:
+            continue
+        raise AssertionError(f"setting {setting}={value} accepted")
+    L = ftdx.lib()
+    assert L.ftdx_gemm_prepared_fused(None, None, None, None, None, 8, 8, 8, 2, 0, 1, 1.0, 0.0,
+                                      None) == ftdx.FTDX_ERR_INVALID_ARG
+    assert L.ftdx_debug_fp6_real_gemm(None, 8, 8, 8, None) == ftdx.FTDX_ERR_INVALID_ARG

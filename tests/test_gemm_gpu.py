@@ -284,3 +284,55 @@ def test_mx_gemm_int4_a_and_one_shot(oracle, dev):
     assert g.gemm_prepared(dA, None, again[0], again[1], M, N, K, B, input=ftdx.InputPrecision.INT4) == 0
     torch.cuda.synchronize()
     assert torch.equal(again[0], out[0]) and torch.equal(again[1], out[1])
+
+
+@pytest.mark.parametrize("fuse", [0, 2, 4])
+def test_gemm_prepared_fused_matches_per_batch(oracle, dev, fuse):
+    """gemm_prepared_fused (cutlass_gemm_api.h:440-470): with the prepared weight sets one B
+    repeated, any fuse factor gives gemm_prepared's bytes; with distinct sets, fused batch e
+    uses set e (gemm_complex_fp8.hpp:418-472)."""
+    import torch
+    from ftd_amd import ftdx
+    M, N, K, B = 48, 96, 160, 8
+    rng = np.random.default_rng(30 + fuse)
+    a_re, a_im = _int_planes(rng, (B, M, K))
+    b_re, b_im = _int_planes(rng, (1, N, K))
+    dA_re, dA_im = _dev(dev, a_re, a_im)
+    g = ftdx.CutlassComplexGemm()
+    rep_re, rep_im = _dev(dev, np.repeat(b_re, B, 0), np.repeat(b_im, B, 0))
+    g.prepare_b(rep_re, rep_im, N, K, B)
+    C1 = [torch.empty((B, M, N), dtype=torch.float32, device=dev) for _ in range(4)]
+    assert g.gemm_prepared(dA_re, dA_im, C1[0], C1[1], M, N, K, B) == 0
+    assert g.gemm_prepared_fused(dA_re, dA_im, C1[2], C1[3], M, N, K, B, fuse_factor=fuse) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(C1[0], C1[2]) and torch.equal(C1[1], C1[3])
+    ref = oracle.gemm_c64(a_re.astype(np.float64) + 1j * a_im, np.repeat(b_re, B, 0) + 1j * np.repeat(b_im, B, 0))
+    assert np.array_equal(C1[2].cpu().numpy(), ref.real.astype(np.float32))
+    # distinct weight sets: fused batch e = b // ff multiplies by set e
+    if fuse:
+        sets_re, sets_im = _int_planes(rng, (B, N, K))
+        s_re, s_im = _dev(dev, sets_re, sets_im)
+        g.prepare_b(s_re, s_im, N, K, B)
+        Cr, Ci = torch.empty((B, M, N), device=dev), torch.empty((B, M, N), device=dev)
+        assert g.gemm_prepared_fused(dA_re, dA_im, Cr, Ci, M, N, K, B, fuse_factor=fuse) == 0
+        torch.cuda.synchronize()
+        Bsel = (sets_re.astype(np.float64) + 1j * sets_im)[np.arange(B) // fuse]
+        ref = oracle.gemm_c64(a_re.astype(np.float64) + 1j * a_im, Bsel)
+        assert np.array_equal(Cr.cpu().numpy(), ref.real.astype(np.float32))
+        assert np.array_equal(Ci.cpu().numpy(), ref.imag.astype(np.float32))
+
+
+def test_gemm_prepared_fused_errors_and_fp6_diagnostic(dev):
+    import torch
+    from ftd_amd import ftdx
+    M, N, K, B = 32, 64, 64, 6
+    g = ftdx.CutlassComplexGemm()
+    z = torch.zeros((B, N, K), dtype=torch.float16, device=dev)
+    g.prepare_b(z, z, N, K, B)
+    A = torch.zeros((B, M, K), dtype=torch.float16, device=dev)
+    C = torch.empty((B, M, N), dtype=torch.float32, device=dev)
+    with pytest.raises(ftdx.FtdxError, match="divide"):
+        g.gemm_prepared_fused(A, A, C, C, M, N, K, B, fuse_factor=4)
+    with pytest.raises(ftdx.UnsupportedError, match="FP32"):
+        g.gemm_prepared_fused(A, A, C, C, M, N, K, B, fuse_factor=2, output=ftdx.OutputPrecision.FP16)
+    assert g.debug_fp6_real_gemm(64, 96, 160) == 0
