@@ -1331,6 +1331,74 @@ int ftdx_debug_fp6_real_gemm(ftdx_context* ctx, int M, int N, int K, void* strea
     return result == 0 ? FTDX_OK : fail(FTDX_ERR_INVALID_ARG, "debug_fp6_real_gemm: results differ");
 }
 
+namespace {
+
+// Scratch owned by one BLAS-glue call (the reference allocates and frees per call).
+struct CallScratch {
+    std::vector<void*> p;
+    ~CallScratch() {
+        for (void* q : p) (void)hipFree(q);
+    }
+    void* get(size_t bytes, hipError_t& e) {
+        void* q = nullptr;
+        if (e == hipSuccess) e = hipMalloc(&q, bytes > 0 ? bytes : 1);
+        if (e == hipSuccess) p.push_back(q);
+        return q;
+    }
+};
+
+}  // namespace
+
+int ftdx_blas_herk_fp32(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch_count, int tune) {
+    const char* W = "ftdx_blas_herk_fp32";
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": NULL context");
+    if (N < 0 || K < 0 || batch_count < 0) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": negative size");
+    if ((int64_t)N * batch_count == 0) return FTDX_OK;
+    if (!C || (!A && K > 0)) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": NULL pointer");
+    const int64_t nA = (int64_t)N * K * 2 * batch_count, nC = (int64_t)N * (N + 1) * batch_count;
+    CallScratch sc;
+    hipError_t e = hipSuccess;
+    uint16_t* a16 = static_cast<uint16_t*>(sc.get((size_t)nA * 2, e));
+    uint16_t* c16 = static_cast<uint16_t*>(sc.get((size_t)nC * 2, e));
+    if (e == hipSuccess) e = ftdx::launch_narrow_fp32(static_cast<const float*>(A), a16, nA, nullptr);
+    if (e != hipSuccess) return hip_fail(e, W);
+    int rc = ftdx_herk(ctx, a16, c16, N, K, batch_count, FTDX_INPUT_FP16, FTDX_COMPUTE_FP8, FTDX_OUTPUT_FP16,
+                       FTDX_FORMAT_PACKED_TRIANGLE, 1.0f, 0.0f, nullptr, tune);
+    if (rc != FTDX_OK) return rc;
+    e = ftdx::launch_widen_fp16(c16, static_cast<float*>(C), nC, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, W);
+}
+
+int ftdx_blas_gemm_fp32(ftdx_context* ctx, const void* A, const void* B, void* C, int M, int N, int K,
+                        int batch_count, int conjugate_b, int tune) {
+    const char* W = "ftdx_blas_gemm_fp32";
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": NULL context");
+    if (M < 0 || N < 0 || K < 0 || batch_count < 0) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": negative size");
+    if ((int64_t)M * N * batch_count == 0) return FTDX_OK;
+    if (!C || (K > 0 && (!A || !B))) return fail(FTDX_ERR_INVALID_ARG, std::string(W) + ": NULL pointer");
+    const int64_t nA = (int64_t)M * K * batch_count, nB = (int64_t)N * K * batch_count,
+                  nC = (int64_t)M * N * batch_count;
+    CallScratch sc;
+    hipError_t e = hipSuccess;
+    uint16_t* are = static_cast<uint16_t*>(sc.get((size_t)nA * 2, e));
+    uint16_t* aim = static_cast<uint16_t*>(sc.get((size_t)nA * 2, e));
+    uint16_t* bre = static_cast<uint16_t*>(sc.get((size_t)nB * 2, e));
+    uint16_t* bim = static_cast<uint16_t*>(sc.get((size_t)nB * 2, e));
+    float* cre = static_cast<float*>(sc.get((size_t)nC * 4, e));
+    float* cim = static_cast<float*>(sc.get((size_t)nC * 4, e));
+    if (e == hipSuccess) e = ftdx::launch_deinterleave_narrow(static_cast<const float*>(A), are, aim, nA, false, nullptr);
+    if (e == hipSuccess)
+        e = ftdx::launch_deinterleave_narrow(static_cast<const float*>(B), bre, bim, nB, conjugate_b == 0, nullptr);
+    if (e != hipSuccess) return hip_fail(e, W);
+    int rc = ftdx_gemm(ctx, are, aim, bre, bim, cre, cim, M, N, K, batch_count, FTDX_COMPUTE_FP8, FTDX_OUTPUT_FP32,
+                       1.0f, 0.0f, nullptr, tune, FTDX_INPUT_FP16);
+    if (rc != FTDX_OK) return rc;
+    e = ftdx::launch_interleave_fp32(cre, cim, static_cast<float*>(C), nC, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, W);
+}
+
 int ftdx_gemm(ftdx_context* ctx, const void* A_re, const void* A_im, const void* B_re, const void* B_im,
               void* C_re, void* C_im, int M, int N, int K, int batch_count, int compute, int output,
               float alpha, float beta, void* stream, int tune, int input) {

@@ -45,4 +45,13 @@ int herkBatchedCutlassQC_FP16(const void* qc_data, void* tri_output, int N, int 
                       1.0f, 0.0f, stream, g_herk_tune_enabled);
 }
 
+int herkBatchedCutlassFP16(const void* fp16_data, void* tri_output, int N, int K, int batch, hipStream_t stream) {
+    return api().herk(fp16_data, tri_output, N, K, batch, InputPrecision::FP16, ComputePrecision::FP8,
+                      OutputPrecision::FP16, OutputFormat::PackedTriangle, 1.0f, 0.0f, stream, g_herk_tune_enabled);
+}
+
+ftdx_context* cutlassContext() { return api().context(); }
+
+bool cutlassHerkTune() { return g_herk_tune_enabled; }
+
 }  // namespace ggp

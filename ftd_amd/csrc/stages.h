@@ -43,4 +43,14 @@ hipError_t launch_promote_fp8_interleaved_rows(const uint8_t* in, uint8_t* out, 
 hipError_t launch_interleave_tri(const uint16_t* re, const uint16_t* im, void* out, int64_t n,
                                  bool fp64, hipStream_t s);
 
+// FP32 -> FP16 nearest-even and FP16 -> FP32, n scalars (the BLAS glue's narrow / widen,
+// lib/cutlass_interface.cu:63-96).
+hipError_t launch_narrow_fp32(const float* in, uint16_t* out, int64_t n, hipStream_t s);
+hipError_t launch_widen_fp16(const uint16_t* in, float* out, int64_t n, hipStream_t s);
+// FP32 [re, im] -> FP16 Re / Im planes, Im negated when asked (lib/cutlass_interface.cu:98-113);
+// FP32 Re / Im planes -> [re, im] (:116-130).  n complex values.
+hipError_t launch_deinterleave_narrow(const float* in, uint16_t* re, uint16_t* im, int64_t n, bool negate_im,
+                                      hipStream_t s);
+hipError_t launch_interleave_fp32(const float* re, const float* im, float* out, int64_t n, hipStream_t s);
+
 }  // namespace ftdx

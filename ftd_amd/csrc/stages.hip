@@ -585,6 +585,72 @@ hipError_t launch_convert_qc_to_ftd(const uint8_t* in, uint8_t* out, int64_t n, 
     return hipGetLastError();
 }
 
+namespace {
+
+// FP32 -> FP16 (nearest even) and back; FP32 [re, im] -> FP16 planar (optionally -im);
+// FP32 planar -> [re, im].  Grid-stride element loops.
+__global__ void __launch_bounds__(ST_THREADS) narrow_fp32_kernel(const float* __restrict__ in,
+                                                                _Float16* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * ST_THREADS)
+        out[i] = (_Float16)in[i];
+}
+
+__global__ void __launch_bounds__(ST_THREADS) widen_fp16_kernel(const _Float16* __restrict__ in,
+                                                               float* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * ST_THREADS)
+        out[i] = (float)in[i];
+}
+
+__global__ void __launch_bounds__(ST_THREADS) deinterleave_narrow_kernel(const float2* __restrict__ in,
+                                                                        _Float16* __restrict__ re,
+                                                                        _Float16* __restrict__ im, int64_t n,
+                                                                        int negate_im) {
+    for (int64_t i = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * ST_THREADS) {
+        const float2 v = in[i];
+        re[i] = (_Float16)v.x;
+        im[i] = (_Float16)(negate_im ? -v.y : v.y);
+    }
+}
+
+__global__ void __launch_bounds__(ST_THREADS) interleave_fp32_kernel(const float* __restrict__ re,
+                                                                    const float* __restrict__ im,
+                                                                    float2* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * ST_THREADS)
+        out[i] = make_float2(re[i], im[i]);
+}
+
+}  // namespace
+
+hipError_t launch_narrow_fp32(const float* in, uint16_t* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(narrow_fp32_kernel, dim3(grid_for(n)), dim3(ST_THREADS), 0, s, in,
+                       reinterpret_cast<_Float16*>(out), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen_fp16(const uint16_t* in, float* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(widen_fp16_kernel, dim3(grid_for(n)), dim3(ST_THREADS), 0, s,
+                       reinterpret_cast<const _Float16*>(in), out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_deinterleave_narrow(const float* in, uint16_t* re, uint16_t* im, int64_t n, bool negate_im,
+                                      hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(deinterleave_narrow_kernel, dim3(grid_for(n)), dim3(ST_THREADS), 0, s,
+                       reinterpret_cast<const float2*>(in), reinterpret_cast<_Float16*>(re),
+                       reinterpret_cast<_Float16*>(im), n, negate_im ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_interleave_fp32(const float* re, const float* im, float* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(interleave_fp32_kernel, dim3(grid_for(n)), dim3(ST_THREADS), 0, s, re, im,
+                       reinterpret_cast<float2*>(out), n);
+    return hipGetLastError();
+}
+
 int precast_row_samples(int K) { return (K + 15) / 16 * 16; }
 
 hipError_t launch_promote_qc_fp8_interleaved(const uint8_t* in, uint8_t* out, int64_t n,

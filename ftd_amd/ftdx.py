@@ -165,6 +165,8 @@ _SIGNATURES = {
     "ftdx_set_gemm_setting": (_I, [_VP, _I, _I]),
     "ftdx_get_gemm_setting": (_I, [_VP, _I, ctypes.POINTER(_I)]),
     "ftdx_set_gemm_tune_cache_path": (_I, [_VP, _S]),
+    "ftdx_blas_herk_fp32": (_I, [_VP, _VP, _VP, _I, _I, _I, _I]),
+    "ftdx_blas_gemm_fp32": (_I, [_VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I]),
 }
 
 PREC_FP32 = 1

@@ -422,6 +422,19 @@ int ftdx_gemm(ftdx_context* ctx, const void* A_re, const void* A_im, const void*
               void* C_re, void* C_im, int M, int N, int K, int batch_count, int compute, int output,
               float alpha, float beta, void* stream, int tune, int input);
 
+/* ---- the BLAS glue of the GGP library (quda namespace, lib/cutlass_interface.cu:316-459) ----
+ * Synchronous, as the reference (it ends with a device synchronize); the scratch is
+ * allocated per call.  Pointers are device pointers; FP32 interleaved complex data. */
+/* stridedBatchGEMMCutlass's HERK case (A == B, op(A) = A^H, M == N; :316-369): A [batch][N][K]
+ * narrowed to FP16 (nearest even), HERK with FP8 compute and FP16 packed output, widened into
+ * C, the FP32 packed lower triangle [batch][N(N+1)/2][2]. */
+int ftdx_blas_herk_fp32(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch_count, int tune);
+/* gemmBatchedCutlass (:371-459): A [batch][M][K], B [batch][N][K] narrowed to FP16 planar, B's
+ * imaginary part negated when conjugate_b == 0 (the reference's flag sense), FP8 GEMM
+ * C = A B^T with FP32 planar output, interleaved into C [batch][M][N]. */
+int ftdx_blas_gemm_fp32(ftdx_context* ctx, const void* A, const void* B, void* C, int M, int N, int K,
+                        int batch_count, int conjugate_b, int tune);
+
 /* ---- streaming correlation of host payloads ----
  * VisibilityPipeline::correlate / correlate_pipelined (lib/visibility_pipeline.cu:253-300,
  * 484-545): each payload is QC bytes [batch][N][K] in host memory (K = n_time /

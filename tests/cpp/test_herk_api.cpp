@@ -228,6 +228,48 @@ int main(int argc, char** argv) {
         HIP_OK(hipFree(dp));
     }
 
+    {   // quda::stridedBatchGEMMCutlass, HERK case (lib/cutlass_interface.cu:316-369), with a
+        // parameter struct of the reference's field names and enum values
+        struct BlasParam {
+            int trans_a = 2, trans_b = 0, m, n, k, batch_count, data_type = 2;
+        } bp;
+        const int hN = 40, hK = 12, hB = 2;  // |C| <= 98 * 12: the FP16 triangle stays exact
+        bp.m = bp.n = hN;
+        bp.k = hK;
+        bp.batch_count = hB;
+        std::vector<uint8_t> qc((size_t)hN * hK * hB);
+        ftdo_gen_qc_uniform(seed + 7, (int64_t)qc.size(), qc.data());
+        std::vector<float> x(qc.size() * 2);
+        for (size_t i = 0; i < qc.size(); ++i) {
+            int r, m;
+            ftdo_decode_qc(qc[i], &r, &m);
+            x[2 * i] = (float)r;
+            x[2 * i + 1] = (float)m;
+        }
+        std::vector<float> want((size_t)hN * (hN + 1) * hB), got(want.size());
+        ftdo_herk_f32(qc.data(), want.data(), hN, hK, hB, 1.0f, 0.0f, 0);
+        float *dx, *dtri;
+        HIP_OK(hipMalloc(&dx, x.size() * 4));
+        HIP_OK(hipMalloc(&dtri, got.size() * 4));
+        HIP_OK(hipMemcpy(dx, x.data(), x.size() * 4, hipMemcpyHostToDevice));
+        const long long flops = quda::stridedBatchGEMMCutlass(dx, dx, dtri, bp, 2);
+        HIP_OK(hipMemcpy(got.data(), dtri, got.size() * 4, hipMemcpyDeviceToHost));
+        bool ok = flops == 8LL * hN * hN * hK * hB;
+        for (size_t i = 0; i < got.size(); ++i) ok = ok && std::memcmp(&got[i], &want[i], 4) == 0;
+        bool threw = false;
+        try {
+            bp.trans_a = 0;  // not the HERK pattern: the reference aborts
+            (void)quda::stridedBatchGEMMCutlass(dx, dx, dtri, bp, 2);
+        } catch (const std::runtime_error&) {
+            threw = true;
+        }
+        ok = ok && threw;
+        std::printf("  %-40s %s\n", "quda::stridedBatchGEMMCutlass (HERK)", ok ? "PASS" : "FAIL");
+        failures += ok ? 0 : 1;
+        HIP_OK(hipFree(dx));
+        HIP_OK(hipFree(dtri));
+    }
+
     HIP_OK(hipFree(dA));
     HIP_OK(hipFree(dC));
     HIP_OK(hipFree(dC16));

@@ -336,3 +336,38 @@ def test_gemm_prepared_fused_errors_and_fp6_diagnostic(dev):
     with pytest.raises(ftdx.UnsupportedError, match="FP32"):
         g.gemm_prepared_fused(A, A, C, C, M, N, K, B, fuse_factor=2, output=ftdx.OutputPrecision.FP16)
     assert g.debug_fp6_real_gemm(64, 96, 160) == 0
+
+
+def test_blas_glue_herk_and_gemm_fp32(oracle, dev):
+    """quda::stridedBatchGEMMCutlass's HERK case and gemmBatchedCutlass
+    (lib/cutlass_interface.cu:316-459) through ftdx_blas_*: FP32 interleaved integers, exact
+    through FP16 and FP8 (|C| <= 2048 keeps the FP16 triangle exact)."""
+    import ctypes
+    import torch
+    from ftd_amd import ftdx
+    L = ftdx.lib()
+    g = ftdx.CutlassComplexGemm()
+    rng = np.random.default_rng(44)
+    N, K, B = 48, 16, 3
+    X = rng.integers(-7, 8, (B, N, K, 2)).astype(np.float32)
+    dX = torch.from_numpy(X).to(dev)
+    C = torch.zeros((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    assert L.ftdx_blas_herk_fp32(g._h, dX.data_ptr(), C.data_ptr(), N, K, B, 0) == 0
+    Xc = X[..., 0].astype(np.float64) + 1j * X[..., 1]
+    full = np.einsum("bik,bjk->bij", Xc, Xc.conj())
+    il, jl = np.tril_indices(N)
+    ref = np.stack([full[:, il, jl].real, full[:, il, jl].imag], axis=-1)
+    ref[:, il == jl, 1] = 0
+    assert np.array_equal(C.cpu().numpy(), ref.astype(np.float32))
+    M, N2, K2 = 40, 56, 96
+    A = rng.integers(-7, 8, (B, M, K2, 2)).astype(np.float32)
+    Bm = rng.integers(-7, 8, (B, N2, K2, 2)).astype(np.float32)
+    dA, dB = torch.from_numpy(A).to(dev), torch.from_numpy(Bm).to(dev)
+    Ac = A[..., 0].astype(np.float64) + 1j * A[..., 1]
+    Bc = Bm[..., 0].astype(np.float64) + 1j * Bm[..., 1]
+    for conj_b in (0, 1):
+        Cg = torch.zeros((B, M, N2, 2), dtype=torch.float32, device=dev)
+        assert L.ftdx_blas_gemm_fp32(g._h, dA.data_ptr(), dB.data_ptr(), Cg.data_ptr(), M, N2, K2, B, conj_b, 0) == 0
+        want = oracle.gemm_c64(Ac, Bc if conj_b else Bc.conj())  # Im negated unless conjugate_b
+        assert np.array_equal(Cg.cpu().numpy(), np.stack([want.real, want.imag], -1).astype(np.float32)), conj_b
+    del ctypes
