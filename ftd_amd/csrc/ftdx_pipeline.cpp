@@ -84,8 +84,13 @@ void host_copy(void* dst, const void* src, size_t n) {
                                  std::min(per, n - off));
     };
     std::vector<std::thread> workers;
-    workers.reserve(want - 1);
-    for (size_t t = 1; t < want; ++t) workers.emplace_back(piece, t);
+    size_t t = 1;
+    try {
+        workers.reserve(want - 1);
+        for (; t < want; ++t) workers.emplace_back(piece, t);
+    } catch (...) {  // no thread available: copy the remaining pieces here
+        for (; t < want; ++t) piece(t);
+    }
     piece(0);
     for (auto& w : workers) w.join();
 }
