@@ -644,8 +644,6 @@ int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int ba
     const int nblkp = ftdx::mx_padded_blocks(K);
     const int64_t rows = (int64_t)N * batch_count;
     const size_t row_bytes = (size_t)nblkp * 64;
-    if ((int64_t)N * (int64_t)row_bytes > 0xFFFFFFFFll)
-        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: N * K too large");
     int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap, (size_t)rows * row_bytes + 16,
                                   "ftdx_herk_mx: scratch");
     if (rc != FTDX_OK) return rc;
