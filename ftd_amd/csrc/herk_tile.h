@@ -755,6 +755,25 @@ struct TileRunner {
                 (void __attribute__((address_space(3)))*)(st + (q * NWAVES + wave) * 1024), 16, 0, 0);
     }
 
+    // One 32-byte part of an MX block on the swizzled layout (first 16 bytes at p0, rest at
+    // p1): only the dwords the scaled MFMA reads -- the codes (4 for E2M1, 6 for E3M2) and
+    // the scale (dword 4 / 6) -- and not the zero padding behind them.
+    template <int OPF>
+    static __device__ __forceinline__ v8i lds_mx_part(const uint8_t* r, int p0, int p1) {
+        const uint4 x = *reinterpret_cast<const uint4*>(r + p0);
+        v8i v;
+        v[0] = (int)x.x; v[1] = (int)x.y; v[2] = (int)x.z; v[3] = (int)x.w;
+        v[5] = 0; v[6] = 0; v[7] = 0;
+        if constexpr (OPF == OPF_MX6) {
+            const uint2 y = *reinterpret_cast<const uint2*>(r + p1);
+            v[4] = (int)y.x; v[5] = (int)y.y;
+            v[6] = (int)*reinterpret_cast<const uint32_t*>(r + p1 + 8);
+        } else {
+            v[4] = (int)*reinterpret_cast<const uint32_t*>(r + p1);
+        }
+        return v;
+    }
+
     // One stage of MFMAs on the swizzled layout.
     template <int FMT, int OPF>
     static __device__ __forceinline__ void mfma_stage_dma(const uint8_t* st, int a_row, int b_row,
@@ -766,21 +785,28 @@ struct TileRunner {
         for (int ks = 0; ks < Cfg::KC / 32; ++ks) {
             const int p0 = ((ks * 4 + half) ^ sw) * 16, p1 = p0 ^ 16;
             v8i au[Cfg::FM], bu[Cfg::FN];
+            if constexpr (opf_mx(OPF)) {
 #pragma unroll
-            for (int a = 0; a < Cfg::FM; ++a) {
-                const uint8_t* r = st + (a_row + a * 32) * DROW;
-                const uint4 x = *reinterpret_cast<const uint4*>(r + p0);
-                const uint4 y = *reinterpret_cast<const uint4*>(r + p1);
-                au[a][0] = (int)x.x; au[a][1] = (int)x.y; au[a][2] = (int)x.z; au[a][3] = (int)x.w;
-                au[a][4] = (int)y.x; au[a][5] = (int)y.y; au[a][6] = (int)y.z; au[a][7] = (int)y.w;
-            }
+                for (int a = 0; a < Cfg::FM; ++a) au[a] = lds_mx_part<OPF>(st + (a_row + a * 32) * DROW, p0, p1);
 #pragma unroll
-            for (int b = 0; b < Cfg::FN; ++b) {
-                const uint8_t* r = st + (b_row + b * 32) * DROW;
-                const uint4 x = *reinterpret_cast<const uint4*>(r + p0);
-                const uint4 y = *reinterpret_cast<const uint4*>(r + p1);
-                bu[b][0] = (int)x.x; bu[b][1] = (int)x.y; bu[b][2] = (int)x.z; bu[b][3] = (int)x.w;
-                bu[b][4] = (int)y.x; bu[b][5] = (int)y.y; bu[b][6] = (int)y.z; bu[b][7] = (int)y.w;
+                for (int b = 0; b < Cfg::FN; ++b) bu[b] = lds_mx_part<OPF>(st + (b_row + b * 32) * DROW, p0, p1);
+            } else {
+#pragma unroll
+                for (int a = 0; a < Cfg::FM; ++a) {
+                    const uint8_t* r = st + (a_row + a * 32) * DROW;
+                    const uint4 x = *reinterpret_cast<const uint4*>(r + p0);
+                    const uint4 y = *reinterpret_cast<const uint4*>(r + p1);
+                    au[a][0] = (int)x.x; au[a][1] = (int)x.y; au[a][2] = (int)x.z; au[a][3] = (int)x.w;
+                    au[a][4] = (int)y.x; au[a][5] = (int)y.y; au[a][6] = (int)y.z; au[a][7] = (int)y.w;
+                }
+#pragma unroll
+                for (int b = 0; b < Cfg::FN; ++b) {
+                    const uint8_t* r = st + (b_row + b * 32) * DROW;
+                    const uint4 x = *reinterpret_cast<const uint4*>(r + p0);
+                    const uint4 y = *reinterpret_cast<const uint4*>(r + p1);
+                    bu[b][0] = (int)x.x; bu[b][1] = (int)x.y; bu[b][2] = (int)x.z; bu[b][3] = (int)x.w;
+                    bu[b][4] = (int)y.x; bu[b][5] = (int)y.y; bu[b][6] = (int)y.z; bu[b][7] = (int)y.w;
+                }
             }
             if constexpr (opf_mx(OPF)) {
                 static_assert(FMT == OUT_LOWER_PACKED || gemm_layout(FMT), "MX rows: row-major layouts");
@@ -788,13 +814,8 @@ struct TileRunner {
                 const bool hi = half != 0;
 #pragma unroll
                 for (int a = 0; a < Cfg::FM; ++a) {
-                    const uint8_t* r = st + (a_row + a * 32) * DROW;
-                    const uint4 x = *reinterpret_cast<const uint4*>(r + (p0 ^ 32));
-                    const uint4 y = *reinterpret_cast<const uint4*>(r + (p1 ^ 32));
-                    v8i v;
-                    v[0] = (int)x.x; v[1] = (int)x.y; v[2] = (int)x.z; v[3] = (int)x.w;
-                    v[4] = (int)y.x; v[5] = (int)y.y; v[6] = (int)y.z; v[7] = (int)y.w;
-                    const v8i av = mx_partner<OPF>(v, hi);
+                    const v8i av = mx_partner<OPF>(
+                        lds_mx_part<OPF>(st + (a_row + a * 32) * DROW, p0 ^ 32, p1 ^ 32), hi);
 #pragma unroll
                     for (int b = 0; b < Cfg::FN; ++b) {
                         acc_re[a][b] = mfma_32x32x64_mx<OPF>(au[a], bu[b], acc_re[a][b]);

@@ -63,10 +63,10 @@ def test_promote_qc_to_fp32_and_convert_to_ftd(oracle, dev):
     assert np.array_equal(q2.cpu().numpy(), oracle.convert_qc_to_ftd(qc[:1001]))
 
 
-def test_pack_triangle_and_tri_from_herm(oracle, dev):
+@pytest.mark.parametrize("N,B", [(131, 3), (1100, 2)])  # 1100: rows longer than one 4 x 256 pass
+def test_pack_triangle_and_tri_from_herm(oracle, dev, N, B):
     import torch
     from ftd_amd import ftdx
-    N, B = 131, 3
     rng = np.random.default_rng(1)
     full = rng.standard_normal((B, N, N, 2)).astype(np.float32)
     old = rng.standard_normal((B, N * (N + 1) // 2, 2)).astype(np.float32)
