@@ -98,6 +98,28 @@ def test_mx_herk_matches_fp64_of_quantized(oracle, dev, compute, output, N, K, B
     assert np.abs(got - ref).max() <= tol * scale, np.abs(got - ref).max() / scale
 
 
+@pytest.mark.parametrize("compute", ["FP4", "FP6"])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
+def test_mx_herk_every_tile_config(oracle, dev, compute, cfg):
+    """The MX operand rows on each forced tile configuration: the FP64 HERK of the
+    quantized voltages, to the same FP32 tolerance as the default configuration."""
+    import torch
+    from ftd_amd import ftdx
+    N, K, B = 200, 300, 2
+    fmt = 0 if compute == "FP4" else 1
+    X16 = _voltages(77, B * N, K)
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_launch_config(cfg, 1)
+    C = ftdx.herk_mx(ctx, torch.from_numpy(X16).to(dev), N, K, B, ftdx.InputPrecision.FP16,
+                     getattr(ftdx.ComputePrecision, compute), output="fp32")
+    torch.cuda.synchronize()
+    codes, scales = oracle.mx_quantize(X16.astype(np.float32), fmt)
+    ref = oracle.herk_c64(oracle.mx_dequantize(codes, scales, fmt, K).reshape(B, N, K), N, K, B)
+    tol = 2e-6 if compute == "FP4" else 5e-4
+    got = C.cpu().numpy().astype(np.float64)
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
 def test_mx_fp6_int4_bit_exact(oracle, dev):
     """INT4 voltages are exact in FP6 E3M2 (scale 2^-2): the MX HERK is bit-exact."""
     import torch
