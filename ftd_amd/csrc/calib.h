@@ -18,8 +18,10 @@ hipError_t launch_gain_loss_grad(const float2* V, const float2* M, const float2*
                                  double* loss, float2* grad, hipStream_t s);
 
 // dX [batch][N][K] complex = (Gamma + Gamma^H) X with X QC bytes [batch][N][K];
-// H is scratch [batch][N][N] complex.
-hipError_t launch_herk_adjoint(const float2* Gamma, const uint8_t* X, float2* H, float2* dX, int N,
+// scratch holds adjoint_scratch_bytes(N, K, batch) bytes (16-byte aligned): the bf16
+// split planes of H and the transposed bf16 voltages.
+size_t adjoint_scratch_bytes(int N, int K, int batch);
+hipError_t launch_herk_adjoint(const float2* Gamma, const uint8_t* X, void* scratch, float2* dX, int N,
                                int K, int batch, hipStream_t s);
 
 }  // namespace ftdx

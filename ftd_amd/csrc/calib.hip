@@ -15,12 +15,12 @@
 //   gain_tile_kernel    32x32 tiles of the triangle: residual, Gamma, loss and
 //                       per-tile row / column partial sums of the gain gradient;
 //   gain_reduce_kernel  fixed-order reduction of the partials (deterministic);
-//   hermitian_expand_kernel  Gamma (packed) -> H = Gamma + Gamma^H (dense N x N),
-//                       LDS-staged so both the block and its conjugate transpose
-//                       are written with coalesced rows;
-//   adjoint_gemm_kernel dX = H X: FP32 MFMA (v_mfma_f32_32x32x2_f32) complex GEMM,
-//                       QC voltages decoded to FP32 in the load stage; the Re/Im
-//                       partner operand is a lane-half swap with one sign.
+//   hermitian_expand_kernel  Gamma (packed) -> H = Gamma + Gamma^H (dense N x N) as three
+//                       bf16 split planes, LDS-staged so both the block and its conjugate
+//                       transpose are written with coalesced rows;
+//   adjoint_x_rows_kernel  QC voltages -> bf16 (Re, Im) rows per time sample (transposed);
+//   adjoint_gemm_kernel dX = H X: complex GEMM on bf16 MFMA (v_mfma_f32_32x32x16_bf16)
+//                       over the three exact split terms of H; FP32 accumulation.
 #include <hip/hip_runtime.h>
 
 #include "calib.h"
@@ -143,10 +143,25 @@ gain_reduce_kernel(const float2* __restrict__ rowpart, const float2* __restrict_
     grad[(int64_t)b * N + k] = s;
 }
 
-// H = Gamma + Gamma^H from the packed lower triangle; one 32x32 tile pair per block.
+// (a, b) -> three bf16 pairs, term t in s[t] (a in the low half, b in the high half): the
+// top 8 significant bits of each value (truncated), then the next 8 of the remainder, then
+// the last 8, so a = sum of its three terms exactly; each remainder a - trunc(a) is exact.
+__device__ __forceinline__ void split3(float a, float b, uint32_t (&s)[3]) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const uint32_t ha = __float_as_uint(a) & 0xFFFF0000u, hb = __float_as_uint(b) & 0xFFFF0000u;
+        s[t] = (ha >> 16) | hb;
+        a -= __uint_as_float(ha);
+        b -= __uint_as_float(hb);
+    }
+}
+
+// H = Gamma + Gamma^H from the packed lower triangle, written as the adjoint GEMM's row
+// operand: plane t of [3][N][Np] holds term t of the bf16 split of (Re H_ij, -Im H_ij);
+// columns N .. Np-1 are zero.  One 32x32 tile pair per block.
 __global__ void __launch_bounds__(256)
-hermitian_expand_kernel(const float2* __restrict__ Gamma, int N, int tiles,
-                        float2* __restrict__ H) {
+hermitian_expand_kernel(const float2* __restrict__ Gamma, int N, int Np, int tiles,
+                        uint32_t* __restrict__ H3) {
     __shared__ float2 blk[GT][GT + 1];
     const int w = (int)blockIdx.x;
     const int b = w / tiles, t = w - b * tiles;
@@ -154,8 +169,15 @@ hermitian_expand_kernel(const float2* __restrict__ Gamma, int N, int tiles,
     tri_decode(t, I, J);
     const int tid = threadIdx.x, c = tid & 31, r0 = tid >> 5;
     const int64_t P = (int64_t)N * (N + 1) / 2;
+    const int64_t plane = (int64_t)N * Np;
     const float2* Gb = Gamma + (int64_t)b * P;
-    float2* Hb = H + (int64_t)b * N * N;
+    uint32_t* Hb = H3 + (int64_t)b * 3 * plane;
+    auto put = [&](int i, int j, float2 v) {
+        uint32_t sp[3];
+        split3(v.x, -v.y, sp);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Hb[k * plane + (int64_t)i * Np + j] = sp[k];
+    };
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int rr = r0 + 8 * q, i = I * GT + rr, j = J * GT + c;
@@ -167,49 +189,95 @@ hermitian_expand_kernel(const float2* __restrict__ Gamma, int N, int tiles,
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int rr = r0 + 8 * q;
-        // block (I, J): H_ij for i in I-block, j in J-block
+        // block (I, J): H_ij for i in I-block, j in J-block (zero where j >= N)
         {
             const int i = I * GT + rr, j = J * GT + c;
-            if (i < N && j < N) {
+            if (i < N && j < Np) {
                 float2 v;
                 if (i > j) v = blk[rr][c];
                 else if (i < j) v = cconj(blk[c][rr]);       // diagonal block only
                 else v = make_float2(2.f * blk[rr][c].x, 0.f);
-                Hb[(int64_t)i * N + j] = v;
+                put(i, j, v);
             }
         }
         // mirrored block (J, I) for off-diagonal tiles: H_ji = conj(Gamma_ij)
         if (I != J) {
             const int i = J * GT + rr, j = I * GT + c;  // row in J-block, column in I-block
-            if (i < N && j < N) Hb[(int64_t)i * N + j] = cconj(blk[c][rr]);
+            if (i < N && j < Np) put(i, j, cconj(blk[c][rr]));
         }
     }
 }
-
-// ---- dX = H X with FP32 MFMA ----
-constexpr int AT_M = 128, AT_N = 128, AT_J = 16, AT_THREADS = 256;
-constexpr int AT_HP = AT_J + 1;   // float2 pitch of a staged H row
-constexpr int AT_XP = AT_N + 2;   // float2 pitch of a staged X row
 
 __device__ __forceinline__ float qc_val(uint32_t n) {
     const int m = (int)(n & 7u);
     return (float)((n & 8u) ? -m : m);
 }
 
+// X QC bytes [batch][N][K] -> the adjoint GEMM's column operand XT [batch][K][Np]: bf16
+// pairs (Re, Im) (small integers: the top half of the FP32 pattern, exactly), zero for
+// N <= j < Np.  32 x 32 tiles through LDS; grid (ceil(K/32), ceil(Np/32), batch).
+__global__ void __launch_bounds__(256)
+adjoint_x_rows_kernel(const uint8_t* __restrict__ X, int N, int K, int Np, uint32_t* __restrict__ XT) {
+    __shared__ uint32_t t[GT][GT + 1];
+    const int k0 = (int)blockIdx.x * GT, j0 = (int)blockIdx.y * GT, b = (int)blockIdx.z;
+    const int tid = threadIdx.x, c = tid & 31, r0 = tid >> 5;
+    const uint8_t* Xb = X + (int64_t)b * N * K;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = j0 + r0 + 8 * q, k = k0 + c;
+        uint32_t v = 0u;
+        if (j < N && k < K) {
+            const uint32_t byte = Xb[(int64_t)j * K + k];
+            v = (__float_as_uint(qc_val(byte >> 4)) >> 16) | (__float_as_uint(qc_val(byte & 15u)) & 0xFFFF0000u);
+        }
+        t[r0 + 8 * q][c] = v;
+    }
+    __syncthreads();
+    uint32_t* XTb = XT + (int64_t)b * K * Np;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = k0 + r0 + 8 * q, j = j0 + c;
+        if (k < K && j < Np) XTb[(int64_t)k * Np + j] = t[c][r0 + 8 * q];
+    }
+}
+
+// ---- dX = H X on bf16 MFMA, H split into three bf16 terms ----
+// H3 and XT come from the two passes above.  QC voltages are integers in [-7, 7], exact in
+// bf16, so every product of a split term and a voltage is exact in FP32 and the MFMA
+// accumulates them in FP32: the accuracy of an FP32 GEMM at 3/16 of its MFMA time.
+// Complex values run along K as (re, im) bf16 pairs: the row operand holds (Hr, -Hi), the
+// column operand (Xr, Xi), giving Re; the column pair swapped with one sign, (Xi, -Xr),
+// gives Im.
+constexpr int AT_M = 128, AT_N = 128, AT_J = 16, AT_THREADS = 256;
+constexpr int AT_ROW = AT_J * 4;                // staged row bytes: 16 complex as bf16 pairs
+constexpr int AT_SPLIT = AT_M * AT_ROW;         // one split plane of H rows
+constexpr int AT_STAGE = 3 * AT_SPLIT + AT_N * AT_ROW;
+
+typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
+
+// 16-byte chunk c (4 complex) of staged row r sits at chunk c ^ ((r / 4) mod 4): the 16
+// rows one ds_read_b128 pass serves land in 16 distinct groups of 4 banks.
+__device__ __forceinline__ int at_chunk(int r, int c) { return c ^ ((r >> 2) & 3); }
+
+__device__ __forceinline__ v8bf16 as_bf16x8(uint4 u) {
+    v8bf16 v;
+    __builtin_memcpy(&v, &u, 16);
+    return v;
+}
+
 __global__ void __launch_bounds__(AT_THREADS, 2)
-adjoint_gemm_kernel(const float2* __restrict__ H, const uint8_t* __restrict__ X,
-                    float2* __restrict__ dX, int N, int K, int tiles_m, int tiles_n) {
-    __shared__ float2 hs[2][AT_M][AT_HP];
-    __shared__ float2 xs[2][AT_J][AT_XP];
-    const int w = (int)blockIdx.x;
+adjoint_gemm_kernel(const uint32_t* __restrict__ H3, const uint32_t* __restrict__ XT,
+                    float2* __restrict__ dX, int N, int K, int Np, int tiles_m, int tiles_n) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2 * AT_STAGE];
     const int per_b = tiles_m * tiles_n;
+    // the column tiles of one row block run on one XCD and share its rows in that L2
+    const int w = xcd_logical((int)blockIdx.x, (int)gridDim.x);
     const int b = w / per_b, rem = w - b * per_b;
     const int tm = rem / tiles_n, tn = rem - tm * tiles_n;
     const int i0 = tm * AT_M, k0 = tn * AT_N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const float2* Hb = H + (int64_t)b * N * N;
-    const uint8_t* Xb = X + (int64_t)b * N * K;
+    const int64_t plane = (int64_t)N * Np;
 
     v16f acc_re[2][2], acc_im[2][2];
 #pragma unroll
@@ -222,71 +290,86 @@ adjoint_gemm_kernel(const float2* __restrict__ H, const uint8_t* __restrict__ X,
                 acc_im[x][y][r] = 0.f;
             }
 
-    // per stage: H rows i0..i0+127, cols j0..j0+15 (8 float2 per thread);
-    //            X rows j0..j0+15, cols k0..k0+127 (8 bytes per thread)
-    float2 hreg[8];
-    uint32_t xreg[8];
+    // item e = tid + 256 q (q = 0, 1): row operand row e / 4, column operand column
+    // e / 4 of the tile, 4 complex (16 bytes) at j0 + 4 (e % 4) of each; rows past N and
+    // columns past K load zeros, and j never passes Np.
+    const uint32_t* hsrc[2];
+    const uint32_t* xsrc[2];
+    bool hok[2], xok[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int e = tid + AT_THREADS * q, r = e >> 2, jq = e & 3;
+        hok[q] = i0 + r < N;
+        xok[q] = k0 + r < K;
+        hsrc[q] = H3 + (int64_t)b * 3 * plane + (int64_t)(hok[q] ? i0 + r : 0) * Np + jq * 4;
+        xsrc[q] = XT + ((int64_t)b * K + (xok[q] ? k0 + r : 0)) * Np + jq * 4;
+    }
+    uint4 hreg[2][3], xreg[2];
     auto load = [&](int j0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int e = tid + AT_THREADS * q;  // 0..2047
-            const int r = e / AT_J, cj = e % AT_J;
-            const int i = i0 + r, j = j0 + cj;
-            hreg[q] = (i < N && j < N) ? Hb[(int64_t)i * N + j] : make_float2(0.f, 0.f);
-        }
-        const int r = tid / 16, c8 = (tid % 16) * 8;  // 16 rows x 16 groups of 8 columns
-        const int j = j0 + r;
+        for (int q = 0; q < 2; ++q) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int k = k0 + c8 + e;
-            xreg[e] = (j < N && k < K) ? (uint32_t)Xb[(int64_t)j * K + k] : 0u;
+            for (int t = 0; t < 3; ++t)
+                hreg[q][t] = hok[q] ? *reinterpret_cast<const uint4*>(hsrc[q] + t * plane + j0)
+                                    : make_uint4(0u, 0u, 0u, 0u);
+            xreg[q] = xok[q] ? *reinterpret_cast<const uint4*>(xsrc[q] + j0) : make_uint4(0u, 0u, 0u, 0u);
         }
     };
     auto store = [&](int buf) {
+        uint8_t* st = lds + buf * AT_STAGE;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int e = tid + AT_THREADS * q;
-            hs[buf][e / AT_J][e % AT_J] = hreg[q];
+        for (int q = 0; q < 2; ++q) {
+            const int e = tid + AT_THREADS * q, r = e >> 2, jq = e & 3;
+            const int off = r * AT_ROW + at_chunk(r, jq) * 16;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) *reinterpret_cast<uint4*>(st + t * AT_SPLIT + off) = hreg[q][t];
+            *reinterpret_cast<uint4*>(st + 3 * AT_SPLIT + off) = xreg[q];
         }
-        const int r = tid / 16, c8 = (tid % 16) * 8;
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            xs[buf][r][c8 + e] = make_float2(qc_val(xreg[e] >> 4), qc_val(xreg[e] & 15u));
     };
 
-    const int nst = (N + AT_J - 1) / AT_J;
+    const int nst = Np / AT_J;
     load(0);
     store(0);
     __syncthreads();
-    const int lr = lane & 31, kh = lane >> 5;  // kh: MFMA k index (0: real part, 1: imag part)
+    const int lr = lane & 31, half = lane >> 5;
     for (int s = 0; s < nst; ++s) {
         const bool more = s + 1 < nst;
         if (more) load((s + 1) * AT_J);
-        const int buf = s & 1;
-#pragma unroll 4
-        for (int jj = 0; jj < AT_J; ++jj) {
-            float are[2], aim[2], bv[2];
+        const uint8_t* st = lds + (s & 1) * AT_STAGE;
+        const uint8_t* sb = st + 3 * AT_SPLIT;
 #pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                const float2 h = hs[buf][wm * 64 + x * 32 + lr][jj];
-                // Re: sum Hr Xr - Hi Xi ; Im: sum Hi Xr + Hr Xi
-                are[x] = kh ? -h.y : h.x;
-                aim[x] = kh ? h.x : h.y;
-            }
+        for (int ks = 0; ks < AT_J / 8; ++ks) {
+            // this lane's 8 K values: chunk ks * 2 + half, the same complex indices in
+            // the row and the column operand
+            const int c = ks * 2 + half;
+            v8bf16 bre[2], bim[2];
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
-                const float2 xv = xs[buf][jj][wn * 64 + y * 32 + lr];
-                bv[y] = kh ? xv.y : xv.x;
+                const int kk = wn * 64 + y * 32 + lr;
+                const uint4 u = *reinterpret_cast<const uint4*>(sb + kk * AT_ROW + at_chunk(kk, c) * 16);
+                // (Xr, Xi) -> (Xi, -Xr): swap the halves, negate the new high half
+                const uint4 v = make_uint4(((u.x >> 16) | (u.x << 16)) ^ 0x80000000u,
+                                           ((u.y >> 16) | (u.y << 16)) ^ 0x80000000u,
+                                           ((u.z >> 16) | (u.z << 16)) ^ 0x80000000u,
+                                           ((u.w >> 16) | (u.w << 16)) ^ 0x80000000u);
+                bre[y] = as_bf16x8(u);
+                bim[y] = as_bf16x8(v);
             }
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
+            for (int t = 0; t < 3; ++t)
 #pragma unroll
-                for (int y = 0; y < 2; ++y) {
-                    acc_re[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(are[x], bv[y], acc_re[x][y], 0, 0, 0);
-                    acc_im[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(aim[x], bv[y], acc_im[x][y], 0, 0, 0);
+                for (int x = 0; x < 2; ++x) {
+                    const int r = wm * 64 + x * 32 + lr;
+                    const v8bf16 a = as_bf16x8(*reinterpret_cast<const uint4*>(st + t * AT_SPLIT + r * AT_ROW +
+                                                                              at_chunk(r, c) * 16));
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) {
+                        acc_re[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bre[y], acc_re[x][y], 0, 0, 0);
+                        acc_im[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bim[y], acc_im[x][y], 0, 0, 0);
+                    }
                 }
         }
-        if (more) store(buf ^ 1);
+        if (more) store((s & 1) ^ 1);
         __syncthreads();
     }
 
@@ -331,21 +414,37 @@ hipError_t launch_gain_loss_grad(const float2* V, const float2* M, const float2*
     return hipGetLastError();
 }
 
-hipError_t launch_herk_adjoint(const float2* Gamma, const uint8_t* X, float2* H, float2* dX, int N,
+int adjoint_padded_n(int N) { return (N + AT_J - 1) / AT_J * AT_J; }
+
+size_t adjoint_scratch_bytes(int N, int K, int batch) {
+    const size_t Np = (size_t)adjoint_padded_n(N);
+    return (size_t)batch * Np * (12 * (size_t)N + 4 * (size_t)K);
+}
+
+hipError_t launch_herk_adjoint(const float2* Gamma, const uint8_t* X, void* scratch, float2* dX, int N,
                                int K, int batch, hipStream_t s) {
     if (N <= 0 || batch <= 0) return hipSuccess;
+    const int Np = adjoint_padded_n(N);
+    uint32_t* H3 = static_cast<uint32_t*>(scratch);
+    uint32_t* XT = H3 + (size_t)batch * 3 * N * Np;
     const int tiles = gain_tiles(N);
     int64_t blocks = (int64_t)tiles * batch;
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(hermitian_expand_kernel, dim3((unsigned)blocks), dim3(256), 0, s, Gamma, N,
-                       tiles, H);
+    hipLaunchKernelGGL(hermitian_expand_kernel, dim3((unsigned)blocks), dim3(256), 0, s, Gamma, N, Np,
+                       tiles, H3);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || K <= 0) return e;
+    if (batch > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(adjoint_x_rows_kernel, dim3((unsigned)((K + GT - 1) / GT), (unsigned)((Np + GT - 1) / GT),
+                                                   (unsigned)batch),
+                       dim3(256), 0, s, X, N, K, Np, XT);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
     const int tm = (N + AT_M - 1) / AT_M, tn = (K + AT_N - 1) / AT_N;
     blocks = (int64_t)tm * tn * batch;
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(adjoint_gemm_kernel, dim3((unsigned)blocks), dim3(AT_THREADS), 0, s, H, X, dX,
-                       N, K, tm, tn);
+    hipLaunchKernelGGL(adjoint_gemm_kernel, dim3((unsigned)blocks), dim3(AT_THREADS), 0, s, H3, XT, dX,
+                       N, K, Np, tm, tn);
     return hipGetLastError();
 }
 

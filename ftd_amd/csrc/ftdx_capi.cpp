@@ -578,11 +578,11 @@ int ftdx_herk_adjoint(ftdx_context* ctx, const void* Gamma, const void* X, void*
     if (e != hipSuccess) return hip_fail(e, "ftdx_herk_adjoint: hipGetDevice");
     if (dev != ctx->device)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_adjoint: context belongs to another device (or none)");
-    const size_t hb = (size_t)batch_count * N * N * sizeof(float2);
+    const size_t hb = ftdx::adjoint_scratch_bytes(N, K, batch_count);
     int rc = ftdx::ensure_scratch(ctx->cal_herm, ctx->cal_herm_cap, hb, "ftdx_herk_adjoint: scratch");
     if (rc != FTDX_OK) return rc;
     e = ftdx::launch_herk_adjoint(static_cast<const float2*>(Gamma), static_cast<const uint8_t*>(X),
-                                  reinterpret_cast<float2*>(ctx->cal_herm), static_cast<float2*>(dX),
+                                  ctx->cal_herm, static_cast<float2*>(dX),
                                   N, K, batch_count, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "ftdx_herk_adjoint: launch");
     return FTDX_OK;

@@ -97,6 +97,14 @@ __device__ __forceinline__ v8i mx_partner(v8i x, bool negate) {
     return x;
 }
 
+// logical workgroup id for hardware id w when consecutive ids go round-robin over
+// 8 XCDs: XCD x gets one contiguous range of logical ids.
+__device__ __forceinline__ int xcd_logical(int w, int total) {
+    const int x = w & 7, local = w >> 3;
+    const int q = total >> 3, r = total & 7;
+    return x * q + (x < r ? x : r) + local;
+}
+
 // lower-triangle tile index t -> (bi, bj), bj <= bi, t = bi(bi+1)/2 + bj
 __device__ __forceinline__ void tri_decode(int t, int& bi, int& bj) {
     int r = (int)((__builtin_sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);

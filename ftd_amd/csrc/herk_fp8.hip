@@ -50,14 +50,6 @@ int cfg_value(int cfg, F&& f, int fallback) {
     return v;
 }
 
-// logical workgroup id for hardware id w when consecutive ids go round-robin over
-// 8 XCDs: XCD x gets one contiguous range of logical ids.
-__device__ __forceinline__ int xcd_logical(int w, int total) {
-    const int x = w & 7, local = w >> 3;
-    const int q = total >> 3, r = total & 7;
-    return x * q + (x < r ? x : r) + local;
-}
-
 template <class Cfg, typename OutT, int MODE, int FMT = OUT_LOWER_PACKED>
 __global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
 herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
