@@ -248,16 +248,16 @@ adjoint_x_rows_kernel(const uint8_t* __restrict__ X, int N, int K, int Np, uint3
 // Complex values run along K as (re, im) bf16 pairs: the row operand holds (Hr, -Hi), the
 // column operand (Xr, Xi), giving Re; the column pair swapped with one sign, (Xi, -Xr),
 // gives Im.
-constexpr int AT_M = 128, AT_N = 128, AT_J = 16, AT_THREADS = 256;
-constexpr int AT_ROW = AT_J * 4;                // staged row bytes: 16 complex as bf16 pairs
+constexpr int AT_M = 128, AT_N = 256, AT_J = 32, AT_THREADS = 512;  // 8 waves: 2 x 4 of 64 x 64
+constexpr int AT_ROW = AT_J * 4;                // staged row bytes: 32 complex as bf16 pairs
 constexpr int AT_SPLIT = AT_M * AT_ROW;         // one split plane of H rows
 constexpr int AT_STAGE = 3 * AT_SPLIT + AT_N * AT_ROW;
 
 typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
 
-// 16-byte chunk c (4 complex) of staged row r sits at chunk c ^ ((r / 4) mod 4): the 16
-// rows one ds_read_b128 pass serves land in 16 distinct groups of 4 banks.
-__device__ __forceinline__ int at_chunk(int r, int c) { return c ^ ((r >> 2) & 3); }
+// 16-byte chunk c (4 complex, 8 per row) of staged row r sits at chunk c ^ ((r / 2) mod 8):
+// the 16 rows one ds_read_b128 pass serves land in 16 distinct groups of 4 banks.
+__device__ __forceinline__ int at_chunk(int r, int c) { return c ^ ((r >> 1) & 7); }
 
 __device__ __forceinline__ v8bf16 as_bf16x8(uint4 u) {
     v8bf16 v;
@@ -265,7 +265,7 @@ __device__ __forceinline__ v8bf16 as_bf16x8(uint4 u) {
     return v;
 }
 
-__global__ void __launch_bounds__(AT_THREADS, 2)
+__global__ void __launch_bounds__(AT_THREADS)
 adjoint_gemm_kernel(const uint32_t* __restrict__ H3, const uint32_t* __restrict__ XT,
                     float2* __restrict__ dX, int N, int K, int Np, int tiles_m, int tiles_n) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * AT_STAGE];
@@ -276,7 +276,7 @@ adjoint_gemm_kernel(const uint32_t* __restrict__ H3, const uint32_t* __restrict_
     const int tm = rem / tiles_n, tn = rem - tm * tiles_n;
     const int i0 = tm * AT_M, k0 = tn * AT_N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave >> 2, wn = wave & 3;
     const int64_t plane = (int64_t)N * Np;
 
     v16f acc_re[2][2], acc_im[2][2];
@@ -290,40 +290,51 @@ adjoint_gemm_kernel(const uint32_t* __restrict__ H3, const uint32_t* __restrict_
                 acc_im[x][y][r] = 0.f;
             }
 
-    // item e = tid + 256 q (q = 0, 1): row operand row e / 4, column operand column
-    // e / 4 of the tile, 4 complex (16 bytes) at j0 + 4 (e % 4) of each; rows past N and
-    // columns past K load zeros, and j never passes Np.
-    const uint32_t* hsrc[2];
-    const uint32_t* xsrc[2];
-    bool hok[2], xok[2];
+    // Items e = tid + 512 q: row operand row e / 8 (q = 0, 1), column operand column e / 8
+    // (q = 0 .. 3), each 4 complex (16 bytes) at j0 + 4 (e % 8).  Rows past N and columns
+    // past K load zeros; j never passes Np.
+    constexpr int HQ = AT_M * 8 / AT_THREADS, XQ = AT_N * 8 / AT_THREADS;
+    static_assert(HQ * AT_THREADS == AT_M * 8 && XQ * AT_THREADS == AT_N * 8, "whole items per thread");
+    const uint32_t* hsrc[HQ];
+    const uint32_t* xsrc[XQ];
+    bool hok[HQ], xok[XQ];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int e = tid + AT_THREADS * q, r = e >> 2, jq = e & 3;
+    for (int q = 0; q < HQ; ++q) {
+        const int e = tid + AT_THREADS * q, r = e >> 3, jq = e & 7;
         hok[q] = i0 + r < N;
-        xok[q] = k0 + r < K;
         hsrc[q] = H3 + (int64_t)b * 3 * plane + (int64_t)(hok[q] ? i0 + r : 0) * Np + jq * 4;
+    }
+#pragma unroll
+    for (int q = 0; q < XQ; ++q) {
+        const int e = tid + AT_THREADS * q, r = e >> 3, jq = e & 7;
+        xok[q] = k0 + r < K;
         xsrc[q] = XT + ((int64_t)b * K + (xok[q] ? k0 + r : 0)) * Np + jq * 4;
     }
-    uint4 hreg[2][3], xreg[2];
+    uint4 hreg[HQ][3], xreg[XQ];
     auto load = [&](int j0) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < HQ; ++q)
 #pragma unroll
             for (int t = 0; t < 3; ++t)
                 hreg[q][t] = hok[q] ? *reinterpret_cast<const uint4*>(hsrc[q] + t * plane + j0)
                                     : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int q = 0; q < XQ; ++q)
             xreg[q] = xok[q] ? *reinterpret_cast<const uint4*>(xsrc[q] + j0) : make_uint4(0u, 0u, 0u, 0u);
-        }
     };
     auto store = [&](int buf) {
         uint8_t* st = lds + buf * AT_STAGE;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int e = tid + AT_THREADS * q, r = e >> 2, jq = e & 3;
+        for (int q = 0; q < HQ; ++q) {
+            const int e = tid + AT_THREADS * q, r = e >> 3, jq = e & 7;
             const int off = r * AT_ROW + at_chunk(r, jq) * 16;
 #pragma unroll
             for (int t = 0; t < 3; ++t) *reinterpret_cast<uint4*>(st + t * AT_SPLIT + off) = hreg[q][t];
-            *reinterpret_cast<uint4*>(st + 3 * AT_SPLIT + off) = xreg[q];
+        }
+#pragma unroll
+        for (int q = 0; q < XQ; ++q) {
+            const int e = tid + AT_THREADS * q, r = e >> 3, jq = e & 7;
+            *reinterpret_cast<uint4*>(st + 3 * AT_SPLIT + r * AT_ROW + at_chunk(r, jq) * 16) = xreg[q];
         }
     };
 
