@@ -66,6 +66,32 @@ def test_gain_and_adjoint_vs_fp64_oracle(oracle, dev, N, K, B):
     assert np.abs(_c(dX) - d64g).max() <= 1e-5 * np.abs(d64g).max()
 
 
+@pytest.mark.parametrize("N,B", [(200, 2), (130, 1)])
+def test_adjoint_identity_voltages_exact(dev, N, B):
+    """X = I (K = N): dX = Gamma + Gamma^H exactly.  The adjoint splits each FP32 value
+    into three bf16 terms whose sum is exact, so every bit of a random 24-bit significand
+    must come back; a missing or rounded term fails."""
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(N)
+    P = N * (N + 1) // 2
+    G = rng.standard_normal((B, P, 2)).astype(np.float32) * np.float32(3.7)
+    X = np.zeros((B, N, N), dtype=np.uint8)
+    X[:, np.arange(N), np.arange(N)] = 0x10  # Re = +1, Im = 0 (high nibble Re)
+    ctx = ftdx.CutlassComplexGemm()
+    dX = ftdx.herk_adjoint(ctx, torch.from_numpy(G).to(dev), torch.from_numpy(X.reshape(-1)).to(dev), N, N, B)
+    torch.cuda.synchronize()
+    il, jl = np.tril_indices(N)
+    H = np.zeros((B, N, N, 2), dtype=np.float32)
+    H[:, il, jl] = G
+    H[:, jl, il, 0] = G[..., 0]
+    H[:, jl, il, 1] = -G[..., 1]
+    d = np.arange(N)
+    H[:, d, d, 0] = 2 * H[:, d, d, 0]
+    H[:, d, d, 1] = 0
+    assert np.array_equal(dX.cpu().numpy().reshape(B, N, N, 2), H)
+
+
 def test_gpu_gradients_vs_central_differences(oracle, dev):
     """FP64 central differences of the forward loss at a sample of coordinates."""
     import torch
