@@ -638,16 +638,51 @@ int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int ba
     if (dev != ctx->device)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_mx: context belongs to another device (or none)");
 
+    const int cfg = ctx->forced_cfg >= 0 ? ftdx::tile_base(ctx->forced_cfg) : ftdx::kMxDefaultCfg;
+    const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int fmt = ftdx::mx_format(compute);
+    // E2M1 on a configuration whose tiles are all interior: dense rows (32 bytes per
+    // block, scales apart), twice the samples per stage of the inline-scale rows.
+    const int nblkd = ftdx::mx_dense_blocks(K);
+    if (fmt == ftdx::MX_FP4_E2M1 && nblkd > 0 && ftdx::mx4d_dense_ok(cfg, N, (int64_t)nblkd * 32)) {
+        const int64_t rows = (int64_t)N * batch_count;
+        const size_t code_bytes = (size_t)rows * nblkd * 32, scale_bytes = (size_t)rows * (nblkd / 4) * 8;
+        int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap, code_bytes + scale_bytes + 16,
+                                      "ftdx_herk_mx: scratch");
+        if (rc != FTDX_OK) return rc;
+        uint8_t* scales = ctx->mx_codes + code_bytes;  // code_bytes is a multiple of 128
+        e = ftdx::launch_mx_quantize_dense(A, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16,
+                                           ctx->mx_codes, scales, rows, N, K, s);
+        if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: quantize launch");
+        ftdx::HerkPlainArgs a{};
+        a.A = ctx->mx_codes;
+        a.C = C;
+        a.N = N;
+        a.K = nblkd * 16;  // 2-byte units: 64 per 4-block stage
+        a.batch = batch_count;
+        a.alpha = alpha;
+        a.beta = beta;
+        a.a_bstride = (int64_t)N * nblkd * 32;
+        a.c_bstride = (int64_t)N * (N + 1);
+        a.precast = 5;
+        a.lda = (int64_t)nblkd * 32;
+        a.S = scales;
+        a.s_stage = (int64_t)N * 8;
+        a.s_bstride = (int64_t)(nblkd / 4) * N * 8;
+        e = ftdx::launch_herk_int4_plain(a, cfg, output == FTDX_OUTPUT_FP16, remap, s);
+        if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: kernel launch");
+        return FTDX_OK;
+    }
+
     // Quantize into MX operand rows (codes and scale per 32-sample part, the tile's
     // 64-bytes-per-32-samples geometry), then the HERK tile kernel with scaled MFMAs.
-    const int fmt = ftdx::mx_format(compute);
     const int nblkp = ftdx::mx_padded_blocks(K);
     const int64_t rows = (int64_t)N * batch_count;
     const size_t row_bytes = (size_t)nblkp * 64;
     int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap, (size_t)rows * row_bytes + 16,
                                   "ftdx_herk_mx: scratch");
     if (rc != FTDX_OK) return rc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
     if (nblkp > 0) {
         e = ftdx::launch_mx_quantize_rows(A, nullptr, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16,
                                           fmt, false, ctx->mx_codes, rows, K, s);
@@ -665,8 +700,6 @@ int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int ba
     a.c_bstride = (int64_t)N * (N + 1);
     a.precast = fmt == ftdx::MX_FP6_E3M2 ? 3 : 4;
     a.lda = (int64_t)row_bytes;
-    const int cfg = ctx->forced_cfg >= 0 ? ftdx::tile_base(ctx->forced_cfg) : ftdx::kMxDefaultCfg;
-    const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
     e = ftdx::launch_herk_int4_plain(a, cfg, output == FTDX_OUTPUT_FP16, remap, s);
     if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: kernel launch");
     return FTDX_OK;

@@ -61,7 +61,9 @@ __device__ __forceinline__ v16f mfma_32x32x64_fp8(v8i a, v8i b, v16f c) {
 // holds two 32-byte parts, Re then Im: the part's 32 codes (24 / 16 bytes), its E8M0
 // scale byte right after them, zeros to 32 bytes.  A lane's operand is one part, so its
 // scale is dword opf_scale_dword of the operand itself.
-enum OperandFormat { OPF_FP8 = 0, OPF_BF6 = 3, OPF_MX6 = 16 + 3, OPF_MX4 = 16 + 4 };
+enum OperandFormat { OPF_FP8 = 0, OPF_BF6 = 3, OPF_MX6 = 16 + 3, OPF_MX4 = 16 + 4, OPF_MX4D = 48 + 4 };
+// OPF_MX4D: dense E2M1 rows -- per 32-sample block 16 code bytes of Re then 16 of Im, the
+// E8M0 scales in a separate stage-major array -- on the direct-to-LDS path only.
 
 constexpr bool opf_mx(int opf) { return opf >= 16; }
 constexpr int opf_scale_dword(int opf) { return opf == OPF_MX6 ? 6 : 4; }

@@ -63,8 +63,13 @@ struct HerkPlainArgs {
     int64_t c_bstride;  // scalars of C per batch (N*(N+1))
     int precast = 0;    // 1: A is pre-cast FP8 (dword-planar) rows, 2: pre-cast E3M2 rows
                         // (32 bytes per 16 samples), 3 / 4: MX E3M2 / E2M1 rows (64 bytes
-                        // per 32-sample block, scales inline); row pitch lda bytes
+                        // per 32-sample block, scales inline), 5: dense E2M1 rows (32 bytes
+                        // per block, scales in S; K counts 2-byte units, a multiple of 64,
+                        // and every tile must be interior: mx4d_dense_ok); row pitch lda bytes
     int64_t lda = 0;
+    // precast 5: scales [batch][stage][N][8], the (Re, Im) E8M0 bytes of each stage's 4 blocks
+    const uint8_t* S = nullptr;
+    int64_t s_stage = 0, s_bstride = 0;
     // Sum over K segments (QC input only): output batch b = ob * seg_group + ib sums
     // the HERKs of input batches (ob * nseg + s) * seg_group + ib, s < nseg, as one
     // product over nseg * K samples (polarization sum: nseg = n_pol, seg_group =
@@ -75,6 +80,10 @@ struct HerkPlainArgs {
 
 // Lower-triangle tiles per problem for a configuration.
 int herk_tiles(int cfg, int N);
+
+// Whether dense E2M1 rows (precast 5) run on configuration cfg for N rows of lda bytes:
+// a 64-sample-stage configuration whose tiles are all interior.
+bool mx4d_dense_ok(int cfg, int N, int64_t lda);
 
 // Plain batched launch: grid = tiles x batch, XCD-aware workgroup order when
 // xcd_remap != 0.

@@ -75,6 +75,10 @@ herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
     j.alpha = p.alpha;
     j.beta = p.beta;
     j.lda = p.lda;
+    if constexpr (MODE == LOAD_MX4D) {
+        j.S = p.S + (int64_t)b * p.s_bstride;
+        j.s_stage = p.s_stage;
+    }
     TileRunner<Cfg>::template run<OutT, MODE, FMT>(j, lds);
 }
 
@@ -116,6 +120,19 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
     const bool vec = (a.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15u) == 0);
     dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    if (a.precast == 5) {
+        if constexpr (TileRunner<Cfg>::MX4D_OK && TileRunner<Cfg>::DMA_OK) {
+            if (out_fp16)
+                hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_MX4D>), grid, block, 0, stream,
+                                   a, tiles, xcd_remap);
+            else
+                hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_MX4D>), grid, block, 0, stream, a,
+                                   tiles, xcd_remap);
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
     if (a.precast == 3 || a.precast == 4) {
         if (out_fp16) {
             if (a.precast == 3)
@@ -330,6 +347,19 @@ int tile_config_wg_per_cu(int cfg) {
 }
 
 int herk_tiles(int cfg, int N) { return tri_tiles(N, tile_config_rows(cfg), tile_config_cols(cfg)); }
+
+bool mx4d_dense_ok(int cfg, int N, int64_t lda) {
+    return cfg_value(
+               cfg,
+               [&](auto c) {
+                   using Cfg = decltype(c);
+                   return (TileRunner<Cfg>::MX4D_OK && TileRunner<Cfg>::DMA_OK && N > 0 && N % Cfg::TM == 0 &&
+                           N % Cfg::TN == 0 && lda > 0 && lda % 16 == 0 && (int64_t)N * lda <= 0xFFFFFFFFll)
+                              ? 1
+                              : 0;
+               },
+               0) != 0;
+}
 
 hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, int cfg, bool out_fp16, int xcd_remap,
                                   hipStream_t stream) {

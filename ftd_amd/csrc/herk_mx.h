@@ -28,4 +28,11 @@ hipError_t launch_mx_quantize_ex(const void* X, const void* X2, int input, int f
 hipError_t launch_mx_quantize_rows(const void* X, const void* X2, int input, int fmt, bool conj, uint8_t* rows_out,
                                    int64_t rows, int K, hipStream_t s);
 
+// Dense E2M1 operand rows (LOAD_MX4D): rows_out [rows][nblkp][2][16] code bytes (Re part,
+// Im part), row pitch nblkp * 32 bytes; scales [rows / nrow][nblkp / 4][nrow][8], the
+// (Re, Im) E8M0 bytes of each 4-block stage.  nblkp = mx_dense_blocks(K), a multiple of 4.
+int mx_dense_blocks(int K);
+hipError_t launch_mx_quantize_dense(const void* X, int input, uint8_t* rows_out, uint8_t* scales, int64_t rows,
+                                    int64_t nrow, int K, hipStream_t s);
+
 }  // namespace ftdx

@@ -85,12 +85,16 @@ __device__ __forceinline__ float qc_nibble_value(uint32_t n) {
 // since the encoding is symmetric), i.e. the codes of conj(X).
 // ROWS: the operand-row layout of the HERK tile instead (codes = rows, scales unused):
 // per (row, block) the Re part then the Im part, 32 bytes each: codes, scale byte, zeros.
-template <int FMT, int IN, bool ROWS = false>
+// DENSE (E2M1, with ROWS): codes = rows [rows][nblkp][2][16] (the 16 code bytes of the Re
+// part, then of the Im part), scales = [rows / nrow][nblkp / 4][nrow][8]: per stage of 4
+// blocks, each row's (Re, Im) scale bytes of those blocks.
+template <int FMT, int IN, bool ROWS = false, bool DENSE = false>
 __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict__ X,
                                                           const void* __restrict__ X2,
                                                           uint8_t* __restrict__ codes,
                                                           uint8_t* __restrict__ scales,
-                                                          int64_t rows, int K, int nblkp, int conj, int vec) {
+                                                          int64_t rows, int K, int nblkp, int conj, int vec,
+                                                          int64_t nrow = 0) {
     using F = MxFmt<FMT>;
     const int64_t total = rows * nblkp;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -202,7 +206,15 @@ __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict
 #pragma unroll
             for (int d = 0; d < 6; ++d) w[d] = (uint32_t)c[d];
         }
-        if constexpr (ROWS) {
+        if constexpr (DENSE) {
+            static_assert(FMT == MX_FP4_E2M1 && ROWS, "dense rows are E2M1 operand rows");
+            uint32_t o[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) o[d] = (part == 1 && conj) ? w[d] ^ F::neg(d) : w[d];
+            *reinterpret_cast<uint4*>(codes + ((row * nblkp + blk) * 2 + part) * 16) = make_uint4(o[0], o[1], o[2], o[3]);
+            const int64_t pb = row / nrow, n = row - pb * nrow;
+            scales[((pb * (nblkp / 4) + blk / 4) * nrow + n) * 8 + (blk & 3) * 2 + part] = (uint8_t)se;
+        } else if constexpr (ROWS) {
             uint32_t o[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 #pragma unroll
             for (int d = 0; d < F::DW; ++d) o[d] = (part == 1 && conj) ? w[d] ^ F::neg(d) : w[d];
@@ -274,6 +286,28 @@ hipError_t launch_mx_quantize_rows(const void* X, const void* X2, int input, int
     if (fmt == MX_FP6_E3M2)
         return quantize_fmt<MX_FP6_E3M2, true>(X, X2, input, conj, rows_out, nullptr, rows, K, nblkp, s);
     return hipErrorInvalidValue;
+}
+
+int mx_dense_blocks(int K) { return ((K + 31) / 32 + 3) / 4 * 4; }
+
+hipError_t launch_mx_quantize_dense(const void* X, int input, uint8_t* rows_out, uint8_t* scales, int64_t rows,
+                                    int64_t nrow, int K, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    const int nblkp = mx_dense_blocks(K);
+    if (nblkp == 0 || nrow <= 0 || rows % nrow != 0) return hipErrorInvalidValue;
+    const int64_t total = rows * nblkp;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    const int esz = input == MX_IN_INT4 ? 1 : 4;
+    const int vec = ((int64_t)K * esz % 16 == 0 && (reinterpret_cast<uintptr_t>(X) & 15u) == 0) ? 1 : 0;
+    if (input == MX_IN_INT4)
+        hipLaunchKernelGGL((mx_quantize_kernel<MX_FP4_E2M1, MX_IN_INT4, true, true>), dim3(blocks), dim3(256), 0, s,
+                           X, nullptr, rows_out, scales, rows, K, nblkp, 0, vec, nrow);
+    else if (input == MX_IN_FP16)
+        hipLaunchKernelGGL((mx_quantize_kernel<MX_FP4_E2M1, MX_IN_FP16, true, true>), dim3(blocks), dim3(256), 0, s,
+                           X, nullptr, rows_out, scales, rows, K, nblkp, 0, vec, nrow);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_mx_quantize(const void* X, int input, int fmt, uint8_t* codes, uint8_t* scales,

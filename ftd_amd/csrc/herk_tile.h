@@ -74,7 +74,11 @@ __device__ __forceinline__ void wait_vmcnt() {
 // samples: 12 bytes of Re codes, 12 of Im codes, 8 of padding), MFMAs in E3M2.
 // LOAD_MX6 / LOAD_MX4: MX block-scaled operand rows (OPF_MX6 / OPF_MX4 in herk_common.h),
 // the same 64 bytes per 32 samples, MFMAs scaled per lane.
-enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1, LOAD_FP8 = 2, LOAD_BF6 = 3, LOAD_MX6 = 4, LOAD_MX4 = 5 };
+// LOAD_MX4D: dense E2M1 rows (OPF_MX4D), interior tiles of 64-sample-stage configurations
+// only: each 128-byte stage row holds 4 blocks of 32 samples (16 code bytes of Re, then Im),
+// and TileJob::S holds the scales, [stage][row][8]: (Re, Im) bytes of the stage's 4 blocks.
+enum LoadMode { LOAD_VEC16 = 0, LOAD_BYTES = 1, LOAD_FP8 = 2, LOAD_BF6 = 3, LOAD_MX6 = 4, LOAD_MX4 = 5,
+                LOAD_MX4D = 6 };
 
 struct TileJob {
     const uint8_t* A;  // this problem's operand rows
@@ -97,6 +101,9 @@ struct TileJob {
     int64_t ldc;
     // OUT_GEMM_BEAM: samples per output and valid samples of this problem
     int beam_nt, beam_samples;
+    // LOAD_MX4D: this problem's scales (row 0 of stage 0) and the bytes between stages
+    const uint8_t* S;
+    int64_t s_stage;
 };
 
 // 4 QC bytes (4 samples) -> u dwords: re = fp8(Re0..Re3), im = fp8(Im0..Im3).
@@ -774,6 +781,87 @@ struct TileRunner {
         return v;
     }
 
+    // ---- Dense E2M1 rows: the stage's scales, 8 bytes per row, beside the codes ----
+    static constexpr int SC_ROW = 8;                               // scale bytes per row per stage
+    static constexpr int SC_STAGE = (Cfg::TM + Cfg::TN) * SC_ROW;  // row block, then column block
+    static constexpr int SC_OPS = SC_STAGE / 1024;                 // 1 KB loads per stage
+    static constexpr bool MX4D_OK = Cfg::KC == 64 && (Cfg::TM * SC_ROW) % 1024 == 0 &&
+                                    (Cfg::TN * SC_ROW) % 1024 == 0 && SC_OPS <= NWAVES &&
+                                    2 * DSTAGE + 2 * SC_STAGE <= Cfg::LDS_BYTES;
+
+    // Waves 0 .. SC_OPS-1 each load 1 KB of the stage's scales (rows' 8 bytes are
+    // contiguous per stage in S, so the row block and the column block are two runs).
+    static __device__ __forceinline__ void dma_scales(const TileJob& j, int stage, uint8_t* sc, int wave,
+                                                      int lane) {
+        if (wave < SC_OPS) {
+            const int byte = wave * 1024 + lane * 16;
+            const uint8_t* base = j.S + (int64_t)stage * j.s_stage;
+            const uint8_t* src = byte < Cfg::TM * SC_ROW ? base + (int64_t)j.row0 * SC_ROW + byte
+                                                         : base + (int64_t)j.col0 * SC_ROW + (byte - Cfg::TM * SC_ROW);
+            __builtin_amdgcn_global_load_lds((const void*)src,
+                                             (void __attribute__((address_space(3)))*)(sc + wave * 1024), 16, 0,
+                                             0);
+        }
+    }
+
+    // E8M0 byte `idx` (0 .. 7) of a row's 8 scale bytes, as the scale operand's low byte
+    static __device__ __forceinline__ int scale_byte(uint2 v, int idx) {
+        return (int)(((idx < 4 ? v.x : v.y) >> (8 * (idx & 3))) & 0xFFu);
+    }
+
+    // One stage of dense E2M1 MFMAs: 4 blocks; lanes 0-31 hold the Re part of a block and
+    // lanes 32-63 the Im part, the Im product's partner is the other part (16 bytes away
+    // before the swizzle) with Re negated, as on the MX rows.
+    template <int FMT>
+    static __device__ __forceinline__ void mfma_stage_mx4d(const uint8_t* st, const uint8_t* sc, int a_row,
+                                                           int b_row, int lane, v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                           v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+        static_assert(FMT == OUT_LOWER_PACKED, "dense E2M1 rows: packed HERK output");
+        const int sw = swz(lane & 31), half = lane >> 5;
+        uint2 sa[Cfg::FM], sb[Cfg::FN];
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a) sa[a] = *reinterpret_cast<const uint2*>(sc + (a_row + a * 32) * SC_ROW);
+#pragma unroll
+        for (int b = 0; b < Cfg::FN; ++b) sb[b] = *reinterpret_cast<const uint2*>(sc + (b_row + b * 32) * SC_ROW);
+#pragma unroll
+        for (int blk = 0; blk < 4; ++blk) {
+            const int c = blk * 2 + half;  // this lane's part of the block
+            const int po = (c ^ sw) * 16, pp = ((c ^ 1) ^ sw) * 16;
+            v8i bu[Cfg::FN];
+            int bs[Cfg::FN];
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b) {
+                const uint4 x = *reinterpret_cast<const uint4*>(st + (b_row + b * 32) * DROW + po);
+                bu[b][0] = (int)x.x; bu[b][1] = (int)x.y; bu[b][2] = (int)x.z; bu[b][3] = (int)x.w;
+                bu[b][4] = 0; bu[b][5] = 0; bu[b][6] = 0; bu[b][7] = 0;
+                bs[b] = scale_byte(sb[b], blk * 2 + half);
+            }
+#pragma unroll
+            for (int a = 0; a < Cfg::FM; ++a) {
+                const uint8_t* r = st + (a_row + a * 32) * DROW;
+                const uint4 x = *reinterpret_cast<const uint4*>(r + po);
+                const uint4 y = *reinterpret_cast<const uint4*>(r + pp);
+                v8i au, av;
+                au[0] = (int)x.x; au[1] = (int)x.y; au[2] = (int)x.z; au[3] = (int)x.w;
+                av[0] = (int)y.x; av[1] = (int)y.y; av[2] = (int)y.z; av[3] = (int)y.w;
+#pragma unroll
+                for (int d = 4; d < 8; ++d) {
+                    au[d] = 0;
+                    av[d] = 0;
+                }
+                av = mx_partner<OPF_MX4>(av, half != 0);
+                const int as = scale_byte(sa[a], blk * 2 + half), ps = scale_byte(sa[a], blk * 2 + 1 - half);
+#pragma unroll
+                for (int b = 0; b < Cfg::FN; ++b) {
+                    acc_re[a][b] =
+                        __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(au, bu[b], acc_re[a][b], 4, 4, 0, as, 0, bs[b]);
+                    acc_im[a][b] =
+                        __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bu[b], acc_im[a][b], 4, 4, 0, ps, 0, bs[b]);
+                }
+            }
+        }
+    }
+
     // One stage of MFMAs on the swizzled layout.
     template <int FMT, int OPF>
     static __device__ __forceinline__ void mfma_stage_dma(const uint8_t* st, int a_row, int b_row,
@@ -854,7 +942,10 @@ struct TileRunner {
         const uint8_t* B = GEMM ? j.B : j.A;
         uint32_t off[DOPS];
         dma_offsets<GEMM>(j, lane, wave, off);
+        constexpr bool DENSE = OPF == OPF_MX4D;
+        [[maybe_unused]] uint8_t* const scl = lds + 2 * DSTAGE;  // dense E2M1: two scale buffers after the codes
         dma_stage(j.A, B, lds, wave, off);
+        if constexpr (DENSE) dma_scales(j, 0, scl, wave, lane);
         for (int s = 0; s < nst; ++s) {
             wait_vmcnt<0>();  // this wave's loads of stage s landed
             __syncthreads();
@@ -862,8 +953,14 @@ struct TileRunner {
             if (s + 1 < nst) {
                 const int64_t k1 = (int64_t)(s + 1) * 2 * Cfg::KC;
                 dma_stage(j.A + k1, B + k1, lds + ((s + 1) & 1) * DSTAGE, wave, off);
+                if constexpr (DENSE) dma_scales(j, s + 1, scl + ((s + 1) & 1) * SC_STAGE, wave, lane);
             }
-            if constexpr (ACTIVE) mfma_stage_dma<FMT, OPF>(cur, a_row, b_row, lane, acc_re, acc_im);
+            if constexpr (ACTIVE) {
+                if constexpr (DENSE)
+                    mfma_stage_mx4d<FMT>(cur, scl + (s & 1) * SC_STAGE, a_row, b_row, lane, acc_re, acc_im);
+                else
+                    mfma_stage_dma<FMT, OPF>(cur, a_row, b_row, lane, acc_re, acc_im);
+            }
         }
         __syncthreads();
     }
@@ -875,12 +972,15 @@ struct TileRunner {
         constexpr bool GEMM = gemm_layout(FMT);
         static_assert(!GEMM || MODE == LOAD_FP8 || MODE == LOAD_MX6 || MODE == LOAD_MX4,
                       "GEMM layouts read pre-cast rows");
+        static_assert(MODE != LOAD_MX4D || (MX4D_OK && DMA_OK && FMT == OUT_LOWER_PACKED),
+                      "dense E2M1 rows: direct-load HERK tiles of 64-sample stages");
         constexpr bool PRE = MODE == LOAD_FP8 || MODE == LOAD_BF6 || MODE == LOAD_MX6 ||
-                             MODE == LOAD_MX4;  // pre-cast operand rows
-        constexpr int OPF = MODE == LOAD_BF6   ? OPF_BF6
-                            : MODE == LOAD_MX6 ? OPF_MX6
-                            : MODE == LOAD_MX4 ? OPF_MX4
-                                               : OPF_FP8;
+                             MODE == LOAD_MX4 || MODE == LOAD_MX4D;  // pre-cast operand rows
+        constexpr int OPF = MODE == LOAD_BF6    ? OPF_BF6
+                            : MODE == LOAD_MX6  ? OPF_MX6
+                            : MODE == LOAD_MX4  ? OPF_MX4
+                            : MODE == LOAD_MX4D ? OPF_MX4D
+                                                : OPF_FP8;
         const int tid = threadIdx.x;
         // wave index in an SGPR: the per-wave conditions below are then scalar branches
         const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -953,6 +1053,9 @@ struct TileRunner {
             }
         }
         if (looped) {
+        } else if constexpr (MODE == LOAD_MX4D) {
+            // not reached: the launch takes dense rows only when every tile is an interior
+            // tile on the direct path (the accumulators stay zero otherwise)
         } else if constexpr (DEPTH == 1) {
             uint4 g[NG];
             if (nst > 0) {

@@ -75,7 +75,7 @@ def test_quantizer_bytes_match_oracle(oracle, dev, compute, inp):
 
 @pytest.mark.parametrize("compute", ["FP4", "FP6"])
 @pytest.mark.parametrize("output", ["fp32", "fp16"])
-@pytest.mark.parametrize("N,K,B", [(200, 300, 2), (129, 64, 1), (33, 1000, 1)])
+@pytest.mark.parametrize("N,K,B", [(200, 300, 2), (129, 64, 1), (33, 1000, 1), (256, 300, 2), (512, 1000, 1)])
 def test_mx_herk_matches_fp64_of_quantized(oracle, dev, compute, output, N, K, B):
     import torch
     from ftd_amd import ftdx
@@ -98,14 +98,16 @@ def test_mx_herk_matches_fp64_of_quantized(oracle, dev, compute, output, N, K, B
     assert np.abs(got - ref).max() <= tol * scale, np.abs(got - ref).max() / scale
 
 
+@pytest.mark.parametrize("N", [200, 256])
 @pytest.mark.parametrize("compute", ["FP4", "FP6"])
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
-def test_mx_herk_every_tile_config(oracle, dev, compute, cfg):
+def test_mx_herk_every_tile_config(oracle, dev, compute, cfg, N):
     """The MX operand rows on each forced tile configuration: the FP64 HERK of the
-    quantized voltages, to the same FP32 tolerance as the default configuration."""
+    quantized voltages, to the same FP32 tolerance as the default configuration.  N = 256
+    with FP4 takes the dense E2M1 rows on the 64-sample-stage configurations."""
     import torch
     from ftd_amd import ftdx
-    N, K, B = 200, 300, 2
+    K, B = 300, 2
     fmt = 0 if compute == "FP4" else 1
     X16 = _voltages(77, B * N, K)
     ctx = ftdx.CutlassComplexGemm()
