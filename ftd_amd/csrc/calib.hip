@@ -215,29 +215,45 @@ __device__ __forceinline__ float qc_val(uint32_t n) {
 
 // X QC bytes [batch][N][K] -> the adjoint GEMM's column operand XT [batch][K][Np]: bf16
 // pairs (Re, Im) (small integers: the top half of the FP32 pattern, exactly), zero for
-// N <= j < Np.  32 x 32 tiles through LDS; grid (ceil(K/32), ceil(Np/32), batch).
+// N <= j < Np.  Tiles of 32 rows j x 128 columns k through LDS, each thread reading 4
+// consecutive bytes of a row (one dword when X rows are 4-byte aligned); grid
+// (ceil(K/128), ceil(Np/32), batch).
+constexpr int XR_K = 128;
+
+__device__ __forceinline__ uint32_t qc_bf16_pair(uint32_t byte) {
+    return (__float_as_uint(qc_val(byte >> 4)) >> 16) | (__float_as_uint(qc_val(byte & 15u)) & 0xFFFF0000u);
+}
+
 __global__ void __launch_bounds__(256)
-adjoint_x_rows_kernel(const uint8_t* __restrict__ X, int N, int K, int Np, uint32_t* __restrict__ XT) {
-    __shared__ uint32_t t[GT][GT + 1];
-    const int k0 = (int)blockIdx.x * GT, j0 = (int)blockIdx.y * GT, b = (int)blockIdx.z;
-    const int tid = threadIdx.x, c = tid & 31, r0 = tid >> 5;
+adjoint_x_rows_kernel(const uint8_t* __restrict__ X, int N, int K, int Np, int vec, uint32_t* __restrict__ XT) {
+    __shared__ uint32_t t[GT][XR_K + 1];
+    const int k0 = (int)blockIdx.x * XR_K, j0 = (int)blockIdx.y * GT, b = (int)blockIdx.z;
+    const int tid = threadIdx.x, c4 = (tid & 31) * 4, r0 = tid >> 5;
     const uint8_t* Xb = X + (int64_t)b * N * K;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int j = j0 + r0 + 8 * q, k = k0 + c;
-        uint32_t v = 0u;
-        if (j < N && k < K) {
-            const uint32_t byte = Xb[(int64_t)j * K + k];
-            v = (__float_as_uint(qc_val(byte >> 4)) >> 16) | (__float_as_uint(qc_val(byte & 15u)) & 0xFFFF0000u);
+        const int jr = r0 + 8 * q, j = j0 + jr, k = k0 + c4;
+        uint32_t w = 0u;
+        if (j < N) {
+            const uint8_t* src = Xb + (int64_t)j * K + k;
+            if (vec && k + 4 <= K) {
+                w = *reinterpret_cast<const uint32_t*>(src);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (k + e < K) w |= (uint32_t)src[e] << (8 * e);
+            }
         }
-        t[r0 + 8 * q][c] = v;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) t[jr][c4 + e] = j < N ? qc_bf16_pair((w >> (8 * e)) & 0xFFu) : 0u;
     }
     __syncthreads();
     uint32_t* XTb = XT + (int64_t)b * K * Np;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int k = k0 + r0 + 8 * q, j = j0 + c;
-        if (k < K && j < Np) XTb[(int64_t)k * Np + j] = t[c][r0 + 8 * q];
+    const int jc = tid & 31;
+#pragma unroll 4
+    for (int q = 0; q < XR_K / 8; ++q) {
+        const int kr = r0 + 8 * q, k = k0 + kr, j = j0 + jc;
+        if (k < K && j < Np) XTb[(int64_t)k * Np + j] = t[jc][kr];
     }
 }
 
@@ -446,9 +462,10 @@ hipError_t launch_herk_adjoint(const float2* Gamma, const uint8_t* X, void* scra
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || K <= 0) return e;
     if (batch > 65535) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(adjoint_x_rows_kernel, dim3((unsigned)((K + GT - 1) / GT), (unsigned)((Np + GT - 1) / GT),
-                                                   (unsigned)batch),
-                       dim3(256), 0, s, X, N, K, Np, XT);
+    const int vec = (K % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 3u) == 0) ? 1 : 0;
+    hipLaunchKernelGGL(adjoint_x_rows_kernel,
+                       dim3((unsigned)((K + XR_K - 1) / XR_K), (unsigned)((Np + GT - 1) / GT), (unsigned)batch),
+                       dim3(256), 0, s, X, N, K, Np, vec, XT);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int tm = (N + AT_M - 1) / AT_M, tn = (K + AT_N - 1) / AT_N;
