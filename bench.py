@@ -303,8 +303,20 @@ def voltage_study(torch, ftdx, dev, n_ant: int, n_beam: int, n_ch: int, n_time: 
     def pipeline():
         vp.beamform(out, qc, scale)
 
+    # FP6 mode (the reference's lower-precision option): QC rows as E3M2 operands against the
+    # MX E3M2 weights, one GEMM per payload, then the power / pol-sum / integration pass
+    vp6 = ftdx.VoltagePipeline(n_antennae=n_ant, n_beams=n_beam, n_channels=n_ch, n_time=n_time,
+                               n_time_power_sum=n_tps, strategy_tune_verbosity=0,
+                               compute_mode=ftdx.VOLTAGE_FP6)
+    vp6.set_weights((torch.rand((n_ch, n_beam, n_ant, 2), generator=g, device=dev) * 2 - 1))
+    out6 = torch.empty_like(out)
+
+    def pipeline_fp6():
+        vp6.beamform(out6, qc, scale)
+
     res = {}
-    for name, fn in (("reference_sequence_ms", reference_sequence), ("pipeline_ms", pipeline)):
+    for name, fn in (("reference_sequence_ms", reference_sequence), ("pipeline_ms", pipeline),
+                     ("pipeline_fp6_ms", pipeline_fp6)):
         fn()
         fn()
         torch.cuda.synchronize()
@@ -322,6 +334,7 @@ def voltage_study(torch, ftdx, dev, n_ant: int, n_beam: int, n_ch: int, n_time: 
             "speedup": round(res["reference_sequence_ms"] / res["pipeline_ms"], 3),
             "tflops_4m_equiv": round(flop / (res["pipeline_ms"] * 1e-3) / 1e12, 2),
             "pct_fp8_peak": round(100.0 * flop / (res["pipeline_ms"] * 1e-3) / 1e12 / FP8_DENSE_PEAK_TFLOPS, 2),
+            "fp6_tflops_4m_equiv": round(flop / (res["pipeline_fp6_ms"] * 1e-3) / 1e12, 2),
             "max_abs_diff_vs_reference_sequence": int(diff.max()),
             "mean_output": round(float(out.float().mean()), 2)}
 
@@ -687,7 +700,7 @@ def main() -> int:
         volt = voltage_study(torch, ftdx, dev, 1664, 4000, 128, 256, 4, 11, 10)
         log(f"bench: voltage pipeline {volt['pipeline_ms']:.3f} ms vs reference sequence "
             f"{volt['reference_sequence_ms']:.3f} ms ({volt['speedup']:.2f}x), "
-            f"{volt['tflops_4m_equiv']:.0f} TFLOP/s 4M-equiv")
+            f"{volt['tflops_4m_equiv']:.0f} TFLOP/s 4M-equiv; FP6 mode {volt['pipeline_fp6_ms']:.3f} ms")
         torch.cuda.empty_cache()
 
     polsum = None
