@@ -206,8 +206,10 @@ int ftdx_gain_loss_grad(ftdx_context* ctx, const void* V, const void* M, const v
 
 /* HERK adjoint: for M = X X^H (packed) and an upstream gradient Gamma (packed,
  * the convention above), dX = (Gamma + Gamma^H) X, FP32 [batch][N][K][2].
- * X is QC bytes [batch][N][K] (input = FTDX_INPUT_INT4).  FP32 MFMA complex GEMM;
- * the context owns the dense Hermitian scratch (batch * N * N * 8 bytes). */
+ * X is QC bytes [batch][N][K] (input = FTDX_INPUT_INT4).  Complex GEMM on bf16
+ * MFMA over the exact three-term bf16 split of Gamma + Gamma^H, FP32 accumulation
+ * (FP32-GEMM accuracy).  The context owns the scratch: the split planes and the
+ * transposed voltages, batch * Np * (12 N + 4 K) bytes with Np = N rounded up to 32. */
 int ftdx_herk_adjoint(ftdx_context* ctx, const void* Gamma, const void* X, void* dX, int N, int K,
                       int batch_count, int input, void* stream);
 
