@@ -38,28 +38,46 @@ def log(msg: str) -> None:
 
 
 def synth_qc(torch, n: int, seed: int, device) -> "torch.Tensor":
-    """QC bytes with Re, Im i.i.d. uniform integers in [-7, 7] (sign-magnitude nibbles)."""
-    g = torch.Generator(device=device).manual_seed(seed)
-    v = torch.randint(-7, 8, (2, n), generator=g, device=device, dtype=torch.int16)
-    nib = torch.where(v < 0, 8 + (-v), v).to(torch.uint8)
-    return (nib[0] << 4) | nib[1]
+    """QC bytes with Re, Im i.i.d. uniform integers in [-7, 7] (sign-magnitude nibbles);
+    the same generator the full-size parity tests use, so a seed names one input."""
+    from ftd_amd import selfcheck
+    return selfcheck.qc_uniform(n, seed, device)
+
+
+# Tile geometry per tile configuration id: (TM, TN, WAVES_M, WAVES_N, KC)
+TILE_GEOMETRY = {0: (128, 128, 2, 2, 64), 1: (128, 128, 2, 2, 128), 2: (64, 64, 1, 1, 64),
+                 3: (256, 128, 4, 2, 64), 4: (128, 128, 2, 2, 64), 5: (256, 128, 4, 2, 64)}
+
+
+def executed_mfma_flop(N: int, K: int, batch: int, cfg: int) -> float:
+    """MFMA FLOP the plain launch executes: every wave of a lower-triangle tile that
+    holds an element i >= j runs its full WM x WN block over K rounded up to the stage
+    (8 real FLOP per complex MAC, counted the way the MFMA counts them)."""
+    TM, TN, WVM, WVN, KC = TILE_GEOMETRY[cfg & 15]
+    WM, WN = TM // WVM, TN // WVN
+    nbM, nbN, r = -(-N // TM), -(-N // TN), TM // TN
+    waves = 0
+    for I in range(nbM):
+        for J in range(min(nbN, r * (I + 1))):
+            for wm in range(WVM):
+                for wn in range(WVN):
+                    w_r, w_c = I * TM + wm * WM, J * TN + wn * WN
+                    if w_r < N and w_r + WM - 1 >= w_c:
+                        waves += 1
+    return 8.0 * WM * WN * (-(-K // KC) * KC) * waves * batch
 
 
 def batched_vs_plain(torch, ftdx, dev, count: int, seed: int, reps: int) -> dict:
     """Batched uneven HERK: `count` problems, N ~ U{256..4096 (powers of 2)},
     K ~ U{256, 320, ..., 4096}; one grouped persistent launch (hipGraph replay,
-    autotuned tile configuration) vs one plain launch per problem on one stream.
-    Output bytes of the two must be identical."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    Ns = rng.choice([256, 512, 1024, 2048, 4096], size=count).tolist()
-    Ks = (rng.integers(4, 65, size=count) * 64).tolist()
-    a_off, c_off, oa, oc = [], [], 0, 0
-    for N, K in zip(Ns, Ks):
-        a_off.append(oa)
-        c_off.append(oc)
-        oa += (N * K + 255) // 256 * 256
-        oc += (N * (N + 1) + 63) // 64 * 64
+    autotuned tile configuration) against a loop of one plain launch per problem on
+    one stream, the loop timed three ways: the library default tile, the grouped
+    plan's tile, and each shape's own autotuned tile (the reference's per-call
+    tune=true).  Every output is compared byte for byte with the grouped one, and the
+    grouped output is checked exactly (weighted checksum of every problem, exact rows
+    of the largest problems)."""
+    from ftd_amd import selfcheck
+    Ns, Ks, a_off, c_off, oa, oc = selfcheck.uneven_problem_list(count, seed)
     A = synth_qc(torch, oa, seed, dev)
     Cp = torch.empty(oc, dtype=torch.float32, device=dev)
     Cg = torch.empty(oc, dtype=torch.float32, device=dev)
@@ -72,38 +90,73 @@ def batched_vs_plain(torch, ftdx, dev, count: int, seed: int, reps: int) -> dict
     t_plan = time.perf_counter()
     plan = ftdx.GroupedPlan(ctx, pg, tile_config=ftdx.TILE_AUTO, use_graph=True)
     t_plan = time.perf_counter() - t_plan
+    info = plan.info()
     stream = torch.cuda.current_stream()
-    ftdx.herk_each(ctx, pp, stream=stream, arr=arr)
     plan.run(stream)
     torch.cuda.synchronize()
-    identical = bool(torch.equal(Cp.view(torch.int32), Cg.view(torch.int32)))
-    tp, tg = [], []
+    big = [p for p, N in enumerate(Ns) if N == max(Ns)][:3]
+    parity = selfcheck.check_problem_list(
+        torch, A, Cg, Ns, Ks, a_off, c_off,
+        rows_of=[(p, [0, 1, 255, 256, Ns[p] // 2, Ns[p] - 1]) for p in big])
+
+    # plain-loop contexts: default tile (no tuning), the grouped plan's tile, per-shape tuned
+    loops = {}
+    c_def = ftdx.CutlassComplexGemm()
+    c_def.set_strategy_tune_verbosity(0)
+    loops["default_tile"] = c_def
+    c_grp = ftdx.CutlassComplexGemm()
+    c_grp.set_strategy_tune_verbosity(0)
+    c_grp.set_launch_config(info["tile_config"], -1)
+    loops["grouped_tile"] = c_grp
+    c_tun = ftdx.CutlassComplexGemm()
+    c_tun.set_strategy_tune_verbosity(0)
+    t_tune = time.perf_counter()
+    for (a, c, N, K) in {(N, K): q for q in pp for N, K in [(q[2], q[3])]}.values():
+        c_tun.herk(a, c, N, K, 1, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                   ftdx.OutputPrecision.FP32, stream=stream, tune=True)
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t_tune
+    loops["per_shape_tuned"] = c_tun
+    identical = {}
+    for name, c in loops.items():
+        Cp.fill_(-1.0)
+        ftdx.herk_each(c, pp, stream=stream, arr=arr)
+        torch.cuda.synchronize()
+        identical[name] = bool(torch.equal(Cp.view(torch.int32), Cg.view(torch.int32)))
+
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    tl = {name: [] for name in loops}
+    tg = []
     for _ in range(reps):
-        e0.record(stream)
-        ftdx.herk_each(ctx, pp, stream=stream, arr=arr)
-        e1.record(stream)
-        e1.synchronize()
-        tp.append(e0.elapsed_time(e1))
-        e0.record(stream)
-        plan.run(stream)
-        e1.record(stream)
-        e1.synchronize()
-        tg.append(e0.elapsed_time(e1))
-    tp.sort()
-    tg.sort()
-    ms_p, ms_g = tp[len(tp) // 2], tg[len(tg) // 2]
-    info = plan.info()
+        for name, c in loops.items():
+            tl[name].append(timed(lambda: ftdx.herk_each(c, pp, stream=stream, arr=arr)))
+        tg.append(timed(lambda: plan.run(stream)))
+    med = lambda v: sorted(v)[len(v) // 2]  # noqa: E731
+    ms_g = med(tg)
+    ms_l = {name: med(v) for name, v in tl.items()}
     plan.close()
+    for c in loops.values():
+        c.close()
     return {
         "problems": count, "seed": seed, "flop_4m_equiv": flop,
-        "grouped_ms": round(ms_g, 4), "plain_loop_ms": round(ms_p, 4),
+        "grouped_ms": round(ms_g, 4),
         "grouped_tflops": round(flop / (ms_g * 1e-3) / 1e12, 2),
-        "plain_loop_tflops": round(flop / (ms_p * 1e-3) / 1e12, 2),
-        "speedup": round(ms_p / ms_g, 3), "identical_output": identical,
+        "plain_loop_ms": {k: round(v, 4) for k, v in ms_l.items()},
+        "plain_loop_tflops": {k: round(flop / (v * 1e-3) / 1e12, 2) for k, v in ms_l.items()},
+        "speedup": {k: round(v / ms_g, 3) for k, v in ms_l.items()},
+        "identical_output": identical,
+        "grouped_parity": parity,
         "tile_config": info["tile_config"], "workgroups": info["workgroups"],
         "items": info["items"], "schedule_imbalance": round(info["imbalance"], 4),
-        "plan_s": round(t_plan, 3), "graph": True,
+        "plan_s": round(t_plan, 3), "per_shape_tune_s": round(t_tune, 2), "graph": True,
     }
 
 
@@ -550,7 +603,8 @@ def main() -> int:
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-beamformer", action="store_true")
     ap.add_argument("--no-polsum", action="store_true")
-    ap.add_argument("--no-voltage", action="store_true")
+    ap.add_argument("--voltage", action="store_true",
+                    help="also time the voltage beamformer pipeline (a caller outside the correlator path)")
     ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--no-bf6", action="store_true")
     ap.add_argument("--batched-problems", type=int, default=512)
@@ -613,6 +667,21 @@ def main() -> int:
     log(f"bench: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms "
         f"(median {per[len(per) // 2]:.3f}), {value:.1f} TFLOP/s 4M-equiv")
 
+    # the output of the last timed step, checked exactly (outside the timed region):
+    # weighted checksums of every channel, exact rows of three channels
+    from ftd_amd import selfcheck
+    t_chk = time.perf_counter()
+    rows = sorted({r for r in (0, 1, 31, 32, 127, 128, 129, 255, 256, 1000, N // 2, N - 2, N - 1)
+                   if 0 <= r < N})
+    parity = selfcheck.check_herk(torch, A, C, N, K, B, rows=rows,
+                                  row_channels=sorted({0, B // 2, B - 1}))
+    parity["seconds"] = round(time.perf_counter() - t_chk, 2)
+    parity["tile_config"] = cfg
+    log(f"bench: parity of the timed output: ok={parity['ok']} "
+        f"({parity['weighted_checksums_equal']}/{B} channel checksums, "
+        f"{parity['exact_rows_checked']} exact rows)")
+    exec_flop = executed_mfma_flop(N, K, B, cfg)
+
     bf6 = None
     if not args.no_bf6:
         # the same tile fed by E3M2 operand rows (one extra pass over the input)
@@ -660,9 +729,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         log(f"bench: batched uneven HERK, {args.batched_problems} problems ...")
         batched = batched_vs_plain(torch, ftdx, dev, args.batched_problems, 3, 5)
-        log(f"bench: grouped {batched['grouped_ms']:.3f} ms vs plain loop "
-            f"{batched['plain_loop_ms']:.3f} ms -> {batched['speedup']:.2f}x, "
-            f"identical={batched['identical_output']}")
+        log(f"bench: grouped {batched['grouped_ms']:.3f} ms vs plain loop {batched['plain_loop_ms']} "
+            f"-> {batched['speedup']}, identical={batched['identical_output']}, "
+            f"parity ok={batched['grouped_parity']['ok']}")
 
     mx = None
     if not args.no_mx:
@@ -695,7 +764,7 @@ def main() -> int:
         torch.cuda.empty_cache()
 
     volt = None
-    if not args.no_voltage:
+    if args.voltage:
         log("bench: voltage pipeline 1664 ant x 4000 beams x 128 ch, 256 times, n_tps 4 ...")
         volt = voltage_study(torch, ftdx, dev, 1664, 4000, 128, 256, 4, 11, 10)
         log(f"bench: voltage pipeline {volt['pipeline_ms']:.3f} ms vs reference sequence "
@@ -744,15 +813,26 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "pct_fp8_peak": round(100.0 * value / FP8_DENSE_PEAK_TFLOPS, 2),
+        "parity_checked": bool(parity["ok"]),
+        "parity": parity,
         "roofline": {
             "bound": "mfma",
+            # 4M-equivalent FLOP (8 N^2 K per channel, the metric's convention) per kernel time
             "achieved": round(achieved, 2),
             "peak": FP8_DENSE_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP8_DENSE_PEAK_TFLOPS, 4),
-            # measured for the headline shape only; null for other shapes or configurations
+            # MFMA FLOP the launch really executes (lower-triangle tiles only, about half
+            # of the 4M-equivalent count) per kernel time
+            "executed_flop": exec_flop,
+            "executed_tflops": round(exec_flop / (kern_ms * 1e-3) / 1e12, 2),
+            "executed_frac": round(exec_flop / (kern_ms * 1e-3) / 1e12 / FP8_DENSE_PEAK_TFLOPS, 4),
+            # HBM bytes per launch: not measured in this run -- the rocprofv3 FETCH_SIZE /
+            # WRITE_SIZE figures of the committed profile for this shape and tile config
             "traffic": (MEASURED_TRAFFIC_BYTES.get(cfg)
                         if (N, K, B) == (4096, 2048, 96) else None),
+            "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, summary in profiles/ "
+                               "(not measured by this run)"),
             "kernel": "herk_int4_fp8_plain_kernel",
             "kernel_ms": round(kern_ms, 4),
             "tile_config": cfg, "xcd_remap": remap,
@@ -769,6 +849,9 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if not parity["ok"] or (batched and not batched["grouped_parity"]["ok"]):
+        log("bench: PARITY FAILURE in the timed output")
+        return 3
     return 0
 
 

@@ -145,6 +145,43 @@ def test_grouped_matches_plain_loop_bytes(oracle, dev):
         assert torch.equal(Cp.view(torch.int32), Cg.view(torch.int32))
 
 
+@pytest.mark.parametrize("tile_config", ["auto", 4])
+def test_grouped_full_size_workload(oracle, dev, tile_config):
+    """The bench's batched workload itself: 512 problems (seed 3, N ~ U{256..4096},
+    K ~ U{256..4096}) on the bench's input, one grouped hipGraph launch.  Every problem
+    is checked by the exact weighted checksum; rows of three N = 4096 problems are
+    compared with the oracle."""
+    import torch
+    from ftd_amd import ftdx, selfcheck
+    Ns, Ks, a_off, c_off, oa, oc = selfcheck.uneven_problem_list(512, 3)
+    A = selfcheck.qc_uniform(oa, 3, dev)
+    C = torch.full((oc,), float("nan"), dtype=torch.float32, device=dev)
+    probs = [(A[a:a + N * K], C[c:c + N * (N + 1)], N, K) for a, c, N, K in zip(a_off, c_off, Ns, Ks)]
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_strategy_tune_verbosity(0)
+    cfg = ftdx.TILE_AUTO if tile_config == "auto" else tile_config
+    plan = ftdx.GroupedPlan(ctx, probs, tile_config=cfg, use_graph=True)
+    plan.run()
+    plan.run()  # graph replay: the same bytes again
+    torch.cuda.synchronize()
+    res = selfcheck.check_problem_list(torch, A, C, Ns, Ks, a_off, c_off)
+    assert res["ok"], res
+    big = [p for p, N in enumerate(Ns) if N == 4096][:3]
+    assert len(big) == 3
+    rows = [0, 1, 127, 128, 255, 256, 2047, 4095]
+    for p in big:
+        N, K = Ns[p], Ks[p]
+        Ah = A[a_off[p]:a_off[p] + N * K].cpu().numpy()
+        ref = np.zeros((1, N * (N + 1) // 2, 2), dtype=np.int64)
+        for r in rows:
+            oracle.herk_int64_rows(Ah, N, K, 1, r, r + 1, out=ref)
+        got = C[c_off[p]:c_off[p] + N * (N + 1)].view(-1, 2)
+        for r in rows:
+            s, e = r * (r + 1) // 2, (r + 1) * (r + 2) // 2
+            assert np.array_equal(got[s:e].cpu().numpy(), ref[0, s:e].astype(np.float32)), (p, r)
+    plan.close()
+
+
 def test_autotune_and_cache_file(oracle, dev, tmp_path):
     import torch
     from ftd_amd import ftdx

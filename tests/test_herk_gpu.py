@@ -216,19 +216,31 @@ def _weighted_checksum_check(torch, A_dev, C_dev, N, K, seed):
     return lhs, rhs
 
 
-@pytest.mark.parametrize("N,K,B", [(4096, 2048, 96)])
-def test_full_size_cfg2(oracle, dev, N, K, B):
-    """Full-size plain HERK (2048 dual-pol antennas, 96 channels, K=2048, seed 2):
-    exact row slices vs the oracle and exact weighted checksums for every channel."""
+@pytest.mark.parametrize("tile_config", [None, 4, 5], ids=["default", "t128x128_k64_p2", "t256x128_k64_p2"])
+def test_full_size_cfg2(oracle, dev, tile_config):
+    """Full-size plain HERK (2048 dual-pol antennas, 96 channels, K=2048, seed 2) on the
+    bench's own input (ftd_amd.selfcheck.qc_uniform), on the library default tile and on
+    the two tiles the autotuner picks at this shape: exact row slices vs the oracle and
+    exact weighted checksums for every channel."""
     import torch
-    from ftd_amd import ftdx
-    A_np = oracle.gen_qc_uniform(2, N * K * B)
-    A = torch.from_numpy(A_np).to(dev)
-    C = ftdx.herk_int4(A, N, K, B)
+    from ftd_amd import ftdx, selfcheck
+    N, K, B = 4096, 2048, 96
+    A = selfcheck.qc_uniform(N * K * B, 2, dev)
+    if tile_config is None:
+        C = ftdx.herk_int4(A, N, K, B)
+    else:
+        ctx = ftdx.CutlassComplexGemm()
+        ctx.set_launch_config(tile_config, 1)
+        C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+        rc = ctx.herk(A, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                      ftdx.OutputPrecision.FP32, tune=False)
+        assert rc == 0
+        assert ctx.query_config(N, K, B)[0] == tile_config
     torch.cuda.synchronize()
-    rows = [0, 1, 31, 32, 127, 128, 129, 1000, 2047, 2048, 4094, 4095]
+    rows = [0, 1, 31, 32, 127, 128, 129, 255, 256, 1000, 2047, 2048, 4094, 4095]
+    A2 = A.view(B, N * K)
     for b in (0, 47, 95):
-        Ab = A_np[b * N * K:(b + 1) * N * K]
+        Ab = A2[b].cpu().numpy()
         ref = np.zeros((1, N * (N + 1) // 2, 2), dtype=np.int64)
         for r in rows:
             oracle.herk_int64_rows(Ab, N, K, 1, r, r + 1, out=ref)
@@ -236,7 +248,6 @@ def test_full_size_cfg2(oracle, dev, N, K, B):
             s, e = r * (r + 1) // 2, (r + 1) * (r + 2) // 2
             got = C[b, s:e].cpu().numpy()
             assert np.array_equal(got, ref[0, s:e].astype(np.float32)), (b, r)
-    A2 = A.view(B, N * K)
     for b in range(B):
         lhs, rhs = _weighted_checksum_check(torch, A2[b], C[b], N, K, seed=100 + b)
         assert lhs == rhs, (b, lhs, rhs)
