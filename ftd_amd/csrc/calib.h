@@ -17,6 +17,15 @@ hipError_t launch_gain_loss_grad(const float2* V, const float2* M, const float2*
                                  float2* Gamma, float2* rowpart, float2* colpart, double* losspart,
                                  double* loss, float2* grad, hipStream_t s);
 
+// Bytes of scratch launch_gain_loss_grad needs: rowpart, colpart, then losspart, each
+// starting at a multiple of 8 bytes.
+size_t gain_scratch_bytes(int N, int batch);
+
+// Size of the fixed device array behind the allocation-free gain entry point, and its
+// device address on the current device (looked up once per device, then kept).
+constexpr size_t kGainStaticScratchBytes = (size_t)16 << 20;
+hipError_t gain_static_scratch(void** out);
+
 // dX [batch][N][K] complex = (Gamma + Gamma^H) X with X QC bytes [batch][N][K];
 // scratch holds adjoint_scratch_bytes(N, K, batch) bytes (16-byte aligned): the bf16
 // split planes of H and the transposed bf16 voltages.

@@ -23,8 +23,17 @@
 //                       over the three exact split terms of H; FP32 accumulation.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "calib.h"
 #include "herk_common.h"
+
+// Fixed device scratch of the allocation-free gain entry point
+// (ftdx_gain_loss_grad_noalloc): the row / column gradient partials and the loss
+// partials live here instead of in a context buffer.  Each device the library runs on
+// holds its own copy until the process ends.  Host code never passes this name to a
+// kernel: it uses the device address that gain_static_scratch() looked up.
+static __device__ __align__(256) uint8_t scratch[ftdx::kGainStaticScratchBytes];
 
 namespace ftdx {
 
@@ -474,6 +483,34 @@ hipError_t launch_herk_adjoint(const float2* Gamma, const uint8_t* X, void* scra
     hipLaunchKernelGGL(adjoint_gemm_kernel, dim3((unsigned)blocks), dim3(AT_THREADS), 0, s, H3, XT, dX,
                        N, K, Np, tm, tn);
     return hipGetLastError();
+}
+
+size_t gain_scratch_bytes(int N, int batch) {
+    if (N <= 0 || batch <= 0) return 0;
+    const size_t part = (size_t)batch * gain_blocks(N) * N * sizeof(float2);
+    return 2 * part + (size_t)batch * gain_tiles(N) * sizeof(double);
+}
+
+// Device address of `scratch` on the current device, looked up once per device (with
+// that device current) and kept; later calls return the kept address.
+hipError_t gain_static_scratch(void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return hipSuccess;
 }
 
 }  // namespace ftdx

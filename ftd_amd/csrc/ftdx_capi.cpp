@@ -349,6 +349,32 @@ int gemm_run(ftdx_context* ctx, const uint8_t* brows, GemmInput kind, const void
     return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
 }
 
+// Argument checks shared by the two gain entry points (sizes, pointers, alignment).
+int gain_check_args(const char* what, const void* V, const void* M, const void* g, int N, int batch,
+                    const void* loss, const void* grad_g, const void* Gamma) {
+    const std::string w(what);
+    if (N < 0 || batch < 0) return fail(FTDX_ERR_INVALID_ARG, w + ": negative size");
+    if (N == 0 || batch == 0) return FTDX_OK;
+    if (!V || !M || !g || !loss || !grad_g) return fail(FTDX_ERR_INVALID_ARG, w + ": NULL pointer");
+    const uintptr_t al = reinterpret_cast<uintptr_t>(V) | reinterpret_cast<uintptr_t>(M) |
+                         reinterpret_cast<uintptr_t>(g) | reinterpret_cast<uintptr_t>(loss) |
+                         reinterpret_cast<uintptr_t>(grad_g) | reinterpret_cast<uintptr_t>(Gamma);
+    if (al & 7u) return fail(FTDX_ERR_INVALID_ARG, w + ": buffers must be 8-byte aligned");
+    return FTDX_OK;
+}
+
+// Launches the gain kernels with their partials in `scr` (gain_scratch_bytes(N, batch)).
+int gain_launch(const char* what, uint8_t* scr, const void* V, const void* M, const void* g, int N, int batch,
+                void* loss, void* grad_g, void* Gamma, void* stream) {
+    const size_t part = (size_t)batch * gain_blocks(N) * N * sizeof(float2);
+    hipError_t e = launch_gain_loss_grad(
+        static_cast<const float2*>(V), static_cast<const float2*>(M), static_cast<const float2*>(g), N, batch,
+        static_cast<float2*>(Gamma), reinterpret_cast<float2*>(scr), reinterpret_cast<float2*>(scr + part),
+        reinterpret_cast<double*>(scr + 2 * part), static_cast<double*>(loss), static_cast<float2*>(grad_g),
+        static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, (std::string(what) + ": launch").c_str());
+}
+
 int mx_format(int compute) {
     return compute == FTDX_COMPUTE_FP4 ? MX_FP4_E2M1 : MX_FP6_E3M2;
 }
@@ -531,35 +557,40 @@ int ftdx_end_herk(ftdx_context* ctx) {
 int ftdx_gain_loss_grad(ftdx_context* ctx, const void* V, const void* M, const void* g, int N,
                         int batch_count, void* loss, void* grad_g, void* Gamma, void* stream) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: NULL context");
-    if (N < 0 || batch_count < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: negative size");
-    if (N == 0 || batch_count == 0) return FTDX_OK;
-    if (!V || !M || !g || !loss || !grad_g)
-        return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: NULL pointer");
-    const uintptr_t al = reinterpret_cast<uintptr_t>(V) | reinterpret_cast<uintptr_t>(M) |
-                         reinterpret_cast<uintptr_t>(g) | reinterpret_cast<uintptr_t>(loss) |
-                         reinterpret_cast<uintptr_t>(grad_g) | reinterpret_cast<uintptr_t>(Gamma);
-    if (al & 7u) return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: buffers must be 8-byte aligned");
+    int rc = ftdx::gain_check_args("ftdx_gain_loss_grad", V, M, g, N, batch_count, loss, grad_g, Gamma);
+    if (rc != FTDX_OK || N == 0 || batch_count == 0) return rc;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, "ftdx_gain_loss_grad: hipGetDevice");
     if (dev != ctx->device)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_gain_loss_grad: context belongs to another device (or none)");
-    const int nb = ftdx::gain_blocks(N), tiles = ftdx::gain_tiles(N);
-    const size_t part = (size_t)batch_count * nb * N * sizeof(float2);
-    const size_t lossp = (size_t)batch_count * tiles * sizeof(double);
-    int rc = ftdx::ensure_scratch(ctx->cal_part, ctx->cal_part_cap, 2 * part + lossp,
-                                  "ftdx_gain_loss_grad: scratch");
+    rc = ftdx::ensure_scratch(ctx->cal_part, ctx->cal_part_cap, ftdx::gain_scratch_bytes(N, batch_count),
+                              "ftdx_gain_loss_grad: scratch");
     if (rc != FTDX_OK) return rc;
-    float2* rowpart = reinterpret_cast<float2*>(ctx->cal_part);
-    float2* colpart = reinterpret_cast<float2*>(ctx->cal_part + part);
-    double* losspart = reinterpret_cast<double*>(ctx->cal_part + 2 * part);
-    e = ftdx::launch_gain_loss_grad(static_cast<const float2*>(V), static_cast<const float2*>(M),
-                                    static_cast<const float2*>(g), N, batch_count,
-                                    static_cast<float2*>(Gamma), rowpart, colpart, losspart,
-                                    static_cast<double*>(loss), static_cast<float2*>(grad_g),
-                                    static_cast<hipStream_t>(stream));
-    if (e != hipSuccess) return hip_fail(e, "ftdx_gain_loss_grad: launch");
-    return FTDX_OK;
+    return ftdx::gain_launch("ftdx_gain_loss_grad", ctx->cal_part, V, M, g, N, batch_count, loss, grad_g,
+                             Gamma, stream);
+}
+
+static_assert(ftdx::kGainStaticScratchBytes == FTDX_GAIN_NOALLOC_SCRATCH_BYTES,
+              "the fixed gain scratch must match its public size");
+
+int ftdx_gain_loss_grad_noalloc(const void* V, const void* M, const void* g, int N, int batch_count,
+                                void* loss, void* grad_g, void* Gamma, void* stream) {
+    int rc = ftdx::gain_check_args("ftdx_gain_loss_grad_noalloc", V, M, g, N, batch_count, loss, grad_g,
+                                   Gamma);
+    if (rc != FTDX_OK || N == 0 || batch_count == 0) return rc;
+    const size_t need = ftdx::gain_scratch_bytes(N, batch_count);
+    if (need > ftdx::kGainStaticScratchBytes)
+        return fail(FTDX_ERR_INVALID_ARG,
+                    "ftdx_gain_loss_grad_noalloc: N=" + std::to_string(N) + ", batch=" +
+                        std::to_string(batch_count) + " needs " + std::to_string(need) +
+                        " bytes of scratch, above the fixed " + std::to_string(ftdx::kGainStaticScratchBytes) +
+                        " (FTDX_GAIN_NOALLOC_SCRATCH_BYTES); use ftdx_gain_loss_grad");
+    void* scr = nullptr;
+    hipError_t e = ftdx::gain_static_scratch(&scr);
+    if (e != hipSuccess) return hip_fail(e, "ftdx_gain_loss_grad_noalloc: scratch address");
+    return ftdx::gain_launch("ftdx_gain_loss_grad_noalloc", static_cast<uint8_t*>(scr), V, M, g, N,
+                             batch_count, loss, grad_g, Gamma, stream);
 }
 
 int ftdx_herk_adjoint(ftdx_context* ctx, const void* Gamma, const void* X, void* dX, int N, int K,

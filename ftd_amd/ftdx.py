@@ -112,6 +112,7 @@ _SIGNATURES = {
     "ftdx_mx_quantize": (_I, [_VP, _I, _I, _VP, _VP, ctypes.c_int64, _I, _VP]),
     "ftdx_mx_layout": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ftdx_gain_loss_grad": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+    "ftdx_gain_loss_grad_noalloc": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
     "ftdx_herk_adjoint": (_I, [_VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP]),
     "ftdx_promote_qc_to_fp8": (_I, [_VP, _VP, ctypes.c_int64, _VP]),
     "ftdx_promote_qc_to_fp16_planar": (_I, [_VP, _VP, _VP, ctypes.c_int64, _VP]),
@@ -483,6 +484,18 @@ def gain_loss_grad(ctx: "CutlassComplexGemm", V, M, g, N: int, batch: int, want_
                                      _ptr(grad), _ptr(Gamma) if Gamma is not None else None,
                                      _stream_ptr(stream)), "gain_loss_grad")
     return loss, grad, Gamma
+
+
+GAIN_NOALLOC_SCRATCH_BYTES = 16 << 20
+
+
+def gain_loss_grad_noalloc(V, M, g, N: int, batch: int, loss, grad, Gamma=None, stream=None) -> None:
+    """ftdx_gain_loss_grad_noalloc: the gain loss and gradients into caller-provided
+    outputs (loss [batch] float64, grad [batch, N, 2] float32, Gamma like M or None);
+    the library allocates nothing (fixed per-device scratch)."""
+    _check(lib().ftdx_gain_loss_grad_noalloc(_ptr(V), _ptr(M), _ptr(g), N, batch, _ptr(loss), _ptr(grad),
+                                             _ptr(Gamma) if Gamma is not None else None,
+                                             _stream_ptr(stream)), "gain_loss_grad_noalloc")
 
 
 def herk_adjoint(ctx: "CutlassComplexGemm", Gamma, X, N: int, K: int, batch: int, stream=None):

@@ -204,6 +204,20 @@ int ftdx_mx_layout(int compute, int K, int* blocks, int* code_bytes);
 int ftdx_gain_loss_grad(ftdx_context* ctx, const void* V, const void* M, const void* g, int N,
                         int batch_count, void* loss, void* grad_g, void* Gamma, void* stream);
 
+/* Allocation-free ftdx_gain_loss_grad.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * ftdx_gain_loss_grad, without a context.  The gradient and loss partials live in a
+ * fixed device array of FTDX_GAIN_NOALLOC_SCRATCH_BYTES on each device (held until
+ * the process ends), so calls are limited to
+ *   2 * batch * nb * N * 8 + batch * nb (nb + 1) / 2 * 8 <= 16 MiB,  nb = ceil(N / 32)
+ * (N = 1024 with batch 16 needs 8.1 MB); larger calls fail with FTDX_ERR_INVALID_ARG.
+ * Every call on a device shares that array, so calls that overlap on one device
+ * (two streams, two threads) race.  Nothing in it needs clearing between calls. */
+#define FTDX_GAIN_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+int ftdx_gain_loss_grad_noalloc(const void* V, const void* M, const void* g, int N, int batch_count,
+                                void* loss, void* grad_g, void* Gamma, void* stream);
+
 /* HERK adjoint: for M = X X^H (packed) and an upstream gradient Gamma (packed,
  * the convention above), dX = (Gamma + Gamma^H) X, FP32 [batch][N][K][2].
  * X is QC bytes [batch][N][K] (input = FTDX_INPUT_INT4).  Complex GEMM on bf16
