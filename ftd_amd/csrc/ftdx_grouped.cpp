@@ -13,12 +13,11 @@
 
 #include <algorithm>
 #include <cstdint>
-#include <numeric>
-#include <queue>
 #include <string>
 #include <vector>
 
 #include "ftdx.h"
+#include "ftdx_host.h"
 #include "ftdx_internal.h"
 #include "herk_fp8.h"
 
@@ -44,25 +43,21 @@ namespace {
 
 int stage_samples(int cfg) { return cfg == TILE_128x128_K128 ? 128 : 64; }
 
-// Relative cost of one tile: K rounded up to the stage, plus a fixed epilogue
-// share; tiles on the diagonal do about three quarters of the work.
-double tile_cost(int cfg, int K, bool diag) {
-    const int kc = stage_samples(cfg);
-    const double k = (double)((K + kc - 1) / kc) * kc + 192.0;
-    return diag ? 0.75 * k : k;
-}
-
 }  // namespace
 
 int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int count, int cfg,
                   int out_fp16, int cu_count) {
-    const int TM = tile_config_rows(cfg), TN = tile_config_cols(cfg), r = TM / TN;
+    std::vector<int> Ns(count), Ks(count);
+    for (int p = 0; p < count; ++p) {
+        Ns[p] = problems[p].N;
+        Ks[p] = problems[p].K;
+    }
+    GroupSchedule sch;
+    const int rc = build_group_schedule(Ns.data(), Ks.data(), count, tile_config_rows(cfg), tile_config_cols(cfg),
+                                        stage_samples(cfg), tile_config_wg_per_cu(cfg), cu_count, sch);
+    if (rc == -2) return fail(FTDX_ERR_INVALID_ARG, "grouped: too many tiles");
+    if (rc != 0) return fail(FTDX_ERR_INVALID_ARG, "grouped: invalid problem list");
     std::vector<GroupProblemDev> probs(count);
-    struct Item {
-        double cost;
-        int prob, tile;
-    };
-    std::vector<Item> items;
     for (int p = 0; p < count; ++p) {
         const ftdx_herk_problem& q = problems[p];
         GroupProblemDev& d = probs[p];
@@ -73,61 +68,15 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
         d.alpha = q.alpha;
         d.beta = q.beta;
         d.vec = (q.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(q.A) & 15u) == 0);
-        d.tiles = herk_tiles(cfg, q.N);
-        int t = 0;
-        const int nbM = (q.N + TM - 1) / TM, nbN = (q.N + TN - 1) / TN;
-        for (int I = 0; I < nbM; ++I) {
-            const int cnt = std::min(nbN, r * (I + 1));
-            for (int J = 0; J < cnt; ++J, ++t) {
-                const bool diag = (J * TN >= I * TM);
-                items.push_back({tile_cost(cfg, q.K, diag), p, t});
-            }
-        }
+        d.tiles = sch.tiles[p];
     }
-    P->items = (int64_t)items.size();
-    if (items.size() > (size_t)0x7FFFFFFF) return fail(FTDX_ERR_INVALID_ARG, "grouped: too many tiles");
-    std::stable_sort(items.begin(), items.end(),
-                     [](const Item& a, const Item& b) { return a.cost > b.cost; });
-
-    const int wg_per_cu = tile_config_wg_per_cu(cfg);
-    int G = (int)std::min<int64_t>((int64_t)cu_count * wg_per_cu, (int64_t)items.size());
-    if (G < 1) G = 1;
-    // rank: workgroups of XCD 0 first, then XCD 1, ... (hardware ids go round-robin)
-    std::vector<int> rank(G);
-    {
-        const int q = G / 8, rem = G % 8;
-        for (int g = 0; g < G; ++g) {
-            const int x = g % 8, local = g / 8;
-            rank[g] = x * q + std::min(x, rem) + local;
-        }
-    }
-    using Bin = std::pair<double, int>;  // (load, rank)
-    std::priority_queue<Bin, std::vector<Bin>, std::greater<Bin>> heap;
-    std::vector<int> by_rank(G);
-    for (int g = 0; g < G; ++g) {
-        heap.push({0.0, rank[g]});
-        by_rank[rank[g]] = g;
-    }
-    std::vector<std::vector<GroupItemDev>> lists(G);
-    std::vector<double> load(G, 0.0);
-    for (const Item& it : items) {
-        Bin b = heap.top();
-        heap.pop();
-        const int g = by_rank[b.second];
-        lists[g].push_back({it.prob, it.tile});
-        load[g] = b.first + it.cost;
-        heap.push({load[g], b.second});
-    }
-    std::vector<int> begin(G + 1, 0);
-    std::vector<GroupItemDev> flat;
-    flat.reserve(items.size());
-    for (int g = 0; g < G; ++g) {
-        begin[g] = (int)flat.size();
-        flat.insert(flat.end(), lists[g].begin(), lists[g].end());
-    }
-    begin[G] = (int)flat.size();
-    P->max_load = *std::max_element(load.begin(), load.end());
-    P->mean_load = std::accumulate(load.begin(), load.end(), 0.0) / G;
+    static_assert(sizeof(GroupItemDev) == sizeof(ScheduleItem), "schedule items are uploaded as they are");
+    const std::vector<ScheduleItem>& flat = sch.items;
+    const std::vector<int>& begin = sch.begin;
+    const int G = sch.num_wg;
+    P->items = (int64_t)flat.size();
+    P->max_load = sch.max_load;
+    P->mean_load = sch.mean_load;
 
     hipError_t e;
     if ((e = hipMalloc(&P->d_probs, sizeof(GroupProblemDev) * std::max(count, 1))) != hipSuccess)

@@ -25,9 +25,9 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
-#include <thread>
 #include <vector>
 
+#include "ftdx_host.h"
 #include "ftdx_internal.h"
 
 struct ftdx_pipeline {
@@ -47,6 +47,7 @@ struct ftdx_pipeline {
 
 using ftdx::fail;
 using ftdx::hip_fail;
+using ftdx::host_copy;
 
 namespace {
 
@@ -66,33 +67,6 @@ void release(ftdx_pipeline* pl) {
     if (pl->s_h2d) (void)hipStreamDestroy(pl->s_h2d);
     if (pl->s_comp) (void)hipStreamDestroy(pl->s_comp);
     if (pl->s_d2h) (void)hipStreamDestroy(pl->s_d2h);
-}
-
-// memcpy split over worker threads in pieces of at least 4 MiB (page-aligned starts).
-void host_copy(void* dst, const void* src, size_t n) {
-    constexpr size_t kPiece = (size_t)4 << 20;
-    const unsigned hw = std::thread::hardware_concurrency();
-    const size_t want = std::min<size_t>({(size_t)8, (size_t)(hw ? hw : 1), (n + kPiece - 1) / kPiece});
-    if (want <= 1) {
-        std::memcpy(dst, src, n);
-        return;
-    }
-    const size_t per = ((n + want - 1) / want + 4095) & ~(size_t)4095;
-    auto piece = [=](size_t t) {
-        const size_t off = t * per;
-        if (off < n) std::memcpy(static_cast<uint8_t*>(dst) + off, static_cast<const uint8_t*>(src) + off,
-                                 std::min(per, n - off));
-    };
-    std::vector<std::thread> workers;
-    size_t t = 1;
-    try {
-        workers.reserve(want - 1);
-        for (; t < want; ++t) workers.emplace_back(piece, t);
-    } catch (...) {  // no thread available: copy the remaining pieces here
-        for (; t < want; ++t) piece(t);
-    }
-    piece(0);
-    for (auto& w : workers) w.join();
 }
 
 int herk_on(ftdx_pipeline* pl, const void* d_in, void* d_out, hipStream_t s) {

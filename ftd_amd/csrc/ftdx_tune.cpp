@@ -4,10 +4,14 @@
 //   herk <N> <K> <batch> <out_fp16> <vec> <cfg> <xcd_remap> <ms>     (tab-separated)
 #include "ftdx_tune.h"
 
+#include <unistd.h>
+
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <sstream>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -79,7 +83,14 @@ int TuneCache::load(const std::string& path, int* skipped) {
 }
 
 bool TuneCache::save(const std::string& path) const {
-    const std::string tmp = path + ".tmp";
+    // A temporary name of this process and call, renamed over `path` once complete:
+    // processes (one per GPU) and threads sharing one cache path never write into
+    // each other's file, and readers see either the old or the new cache whole.
+    static std::atomic<unsigned long> serial{0};
+    std::ostringstream tn;
+    tn << path << ".tmp." << getpid() << '.' << std::hash<std::thread::id>()(std::this_thread::get_id()) << '.'
+       << serial.fetch_add(1);
+    const std::string tmp = tn.str();
     {
         std::ofstream out(tmp, std::ios::trunc);
         if (!out) return false;
@@ -90,9 +101,17 @@ bool TuneCache::save(const std::string& path) const {
             out << "herk\t" << k.N << '\t' << k.K << '\t' << k.batch << '\t' << k.out_fp16 << '\t'
                 << k.vec << '\t' << e.cfg << '\t' << e.xcd_remap << '\t' << e.ms << "\n";
         }
-        if (!out) return false;
+        if (!out) {
+            out.close();
+            std::remove(tmp.c_str());
+            return false;
+        }
     }
-    return std::rename(tmp.c_str(), path.c_str()) == 0;
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        return false;
+    }
+    return true;
 }
 
 }  // namespace ftdx

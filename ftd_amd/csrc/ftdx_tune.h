@@ -6,8 +6,6 @@
 // with HIP events, keep the fastest, and persist it in a tab-separated file
 // whose header carries a build fingerprint (entries of another build are ignored).
 #pragma once
-#include <hip/hip_runtime.h>
-
 #include <map>
 #include <string>
 

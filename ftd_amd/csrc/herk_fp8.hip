@@ -324,16 +324,6 @@ hipError_t launch_grouped_cfg(const GroupProblemDev* probs, const GroupItemDev* 
 
 }  // namespace
 
-const char* tile_config_name(int cfg) {
-    static const char* const names[2][NUM_TILE_CONFIGS] = {
-        {"t128x128_k64", "t128x128_k128", "t64x64_k64", "t256x128_k64", "t128x128_k64_p2",
-         "t256x128_k64_p2"},
-        {"t128x128_k64_bf6", "t128x128_k128_bf6", "t64x64_k64_bf6", "t256x128_k64_bf6",
-         "t128x128_k64_p2_bf6", "t256x128_k64_p2_bf6"}};
-    if (!valid_tile_config(cfg)) return "unknown";
-    return names[(cfg & TILE_BF6) ? 1 : 0][tile_base(cfg)];
-}
-
 int tile_config_rows(int cfg) {
     return cfg_value(cfg, [](auto c) { return decltype(c)::TM; }, 0);
 }
