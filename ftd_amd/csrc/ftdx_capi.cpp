@@ -52,6 +52,7 @@ struct ftdx_context {
     struct GraphEntry {
         bool used = false;
         int N = 0, K = 0, batch = 0, input = 0, output = 0, mode = 0;
+        int cfg = -1, remap = -1;  // launch configuration the graph was captured with
         float alpha = 0.f, beta = 0.f;
         const void* A = nullptr;
         void* C = nullptr;
@@ -60,6 +61,7 @@ struct ftdx_context {
     };
     GraphEntry graphs[4];
     unsigned long long graph_clock = 0;
+    long long graph_captures = 0;  // graphs captured so far (ftdx_herk_graph_stats)
     hipStream_t cap_stream = nullptr;
     // complex GEMM: prepared conj(B) rows (prepare_b) and a separate buffer for the
     // one-shot gemm(), so it never clobbers a prepared B
@@ -446,6 +448,15 @@ int ftdx_set_launch_config(ftdx_context* ctx, int tile_config, int xcd_remap) {
 int ftdx_set_direct_herk_graph(ftdx_context* ctx, int enable) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_direct_herk_graph: NULL context");
     ctx->graph = enable ? 1 : 0;
+    return FTDX_OK;
+}
+
+int ftdx_herk_graph_stats(const ftdx_context* ctx, int* cached, long long* captures) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_graph_stats: NULL context");
+    int n = 0;
+    for (const auto& gph : ctx->graphs) n += gph.used ? 1 : 0;
+    if (cached) *cached = n;
+    if (captures) *captures = ctx->graph_captures;
     return FTDX_OK;
 }
 
@@ -850,8 +861,8 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
         ftdx_context::GraphEntry* hit = nullptr;
         for (auto& gph : ctx->graphs)
             if (gph.used && gph.N == N && gph.K == K && gph.batch == batch_count &&
-                gph.input == input && gph.output == output && gph.mode == mode &&
-                gph.alpha == alpha && gph.beta == beta && gph.A == A && gph.C == C)
+                gph.input == input && gph.output == output && gph.mode == mode && gph.cfg == cfg &&
+                gph.remap == remap && gph.alpha == alpha && gph.beta == beta && gph.A == A && gph.C == C)
                 hit = &gph;
         if (!hit) {
             if (!ctx->cap_stream &&
@@ -879,6 +890,8 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
             slot->used = true;
             slot->N = N; slot->K = K; slot->batch = batch_count;
             slot->input = input; slot->output = output; slot->mode = mode;
+            slot->cfg = cfg; slot->remap = remap;
+            ++ctx->graph_captures;
             slot->alpha = alpha; slot->beta = beta; slot->A = A; slot->C = C;
             slot->exec = exec;
             hit = slot;

@@ -99,6 +99,7 @@ _SIGNATURES = {
     "ftdx_print_build_info": (_I, []),
     "ftdx_herk_int4": (_I, [_VP, _VP, _I, _I, _I, _I, _F, _F, _VP]),
     "ftdx_set_launch_config": (_I, [_VP, _I, _I]),
+    "ftdx_herk_graph_stats": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)]),
     "ftdx_tune_load": (_I, [_VP, _S, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ftdx_tune_save": (_I, [_VP, _S]),
     "ftdx_herk_query_config": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
@@ -418,6 +419,12 @@ class CutlassComplexGemm:
     # ---- gfx950 launch configuration and tuning cache ----
     def set_launch_config(self, tile_config: int = -1, xcd_remap: int = -1) -> None:
         _check(lib().ftdx_set_launch_config(self._h, tile_config, xcd_remap), "set_launch_config")
+
+    def graph_stats(self) -> tuple[int, int]:
+        """(graphs cached, graphs captured so far) of the HERK graph replay."""
+        n, c = ctypes.c_int(), ctypes.c_longlong()
+        _check(lib().ftdx_herk_graph_stats(self._h, ctypes.byref(n), ctypes.byref(c)), "graph_stats")
+        return n.value, c.value
 
     def query_config(self, N: int, K: int, batch: int, output=OutputPrecision.FP32):
         c, r = ctypes.c_int(), ctypes.c_int()

@@ -125,6 +125,11 @@ int ftdx_end_herk(ftdx_context* ctx);
 int ftdx_set_herk_mode(ftdx_context* ctx, int mode);
 int ftdx_set_direct_tile_config(ftdx_context* ctx, int config);
 int ftdx_set_direct_herk_graph(ftdx_context* ctx, int enable);
+/* Graph-replay diagnostics (no reference counterpart): graphs held in the context's
+ * four-entry cache and graphs captured since it was created.  A call replays a cached
+ * graph only if its shape, precisions, scalars, pointers and launch configuration
+ * (tile configuration and workgroup order) all match the captured call. */
+int ftdx_herk_graph_stats(const ftdx_context* ctx, int* cached, long long* captures);
 int ftdx_set_tune_cache_path(ftdx_context* ctx, const char* path);
 int ftdx_set_strategy_tune_verbosity(ftdx_context* ctx, int level);
 int ftdx_set_kernel_tune_verbosity(ftdx_context* ctx, int level);

@@ -185,3 +185,31 @@ def test_graph_replay_lru(oracle, dev, mode):
             ref = oracle.herk_f32(qc, N, K, B, alpha=0.5)
             assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (rep, mode)
     g.end_herk()
+
+
+def test_graph_replay_follows_launch_config(oracle, dev):
+    """A graph captured under one tile configuration is not replayed after the launch
+    configuration changes: the call with the new configuration is captured anew, and
+    both give the oracle's bytes."""
+    import torch
+    from ftd_amd import ftdx
+    g = ftdx.CutlassComplexGemm()
+    g.set_herk_mode(ftdx.HerkMode.ForceDirect)
+    g.set_direct_herk_graph(True)
+    N, K, B = 130, 192, 2
+    qc = oracle.gen_qc_uniform(77, N * K * B)
+    A = torch.from_numpy(qc).to(dev)
+    C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    ref = oracle.herk_f32(qc, N, K, B)
+    seen = []
+    for cfg, remap in ((0, 1), (0, 1), (3, 1), (3, 1), (3, 0), (0, 1)):
+        g.set_launch_config(cfg, remap)
+        C.fill_(float("nan"))
+        assert g.herk(A, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                      ftdx.OutputPrecision.FP32, tune=False) == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (cfg, remap)
+        seen.append(g.graph_stats()[1])
+    # captures: (0,1) new, replay, (3,1) new, replay, (3,0) new, (0,1) replayed from the cache
+    assert seen == [1, 1, 2, 2, 3, 3], seen
+    g.end_herk()
