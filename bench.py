@@ -478,6 +478,62 @@ def pol_reduce_study(torch, ftdx, dev, N: int, K: int, n_ch: int, n_pol: int, n_
             **res, "speedup": round(res["unfused_ms"] / res["fused_ms"], 3), "identical_output": same}
 
 
+def grid_study(torch, ftdx, dev, N: int, K: int, n_ch: int, n_pol: int, Ng: int, seed: int, reps: int) -> dict:
+    """HERK + polarization sum + pillbox UV gridding: one pass (ftdx_herk_pol_grid, the
+    triangle never reaches HBM) against the HERK + pol-sum pass writing the triangle
+    followed by the stand-alone scatter.  UV from random antenna positions (autos at the
+    grid centre), frequencies 0.7-1.4 GHz.  FP32 atomic cell sums are order-dependent once
+    a cell's sum passes 2^24 (the centre cell collects every auto-correlation here), so
+    the grids are compared by their largest difference relative to the largest cell."""
+    import numpy as np
+    A = synth_qc(torch, N * K * n_ch * n_pol, seed, dev)
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1.0, 1.0, (N, 2)) * 600.0  # metres
+    ii, jj = np.tril_indices(N)
+    uv = torch.from_numpy((pos[ii] - pos[jj]).astype(np.float32)).to(dev)
+    freqs = torch.from_numpy(np.linspace(0.7e9, 1.4e9, n_ch)).to(dev)
+    # UV cell (wavelengths) so that the longest baseline at the highest frequency lands inside
+    cell = float((pos.max(axis=0) - pos.min(axis=0)).max() * 1.4e9 / 299792458.0 * 1.05 / (Ng // 2))
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_strategy_tune_verbosity(0)
+    fused = torch.zeros((n_ch, Ng, Ng, 2), dtype=torch.float32, device=dev)
+    unfused = torch.zeros_like(fused)
+    tri = torch.empty((n_ch, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    n_bl = N * (N + 1) // 2
+
+    def one_pass():
+        ftdx.herk_pol_grid(ctx, A, N, K, n_ch, n_pol, uv, freqs, fused, Ng, cell)
+
+    def two_pass():
+        ftdx.herk_pol_reduce(ctx, A, N, K, n_ch, n_pol, 1, C=tri)
+        ftdx.pillbox_grid(tri, uv, freqs, unfused, n_bl, Ng, n_ch, cell)
+
+    res = {}
+    for name, fn in (("unfused_ms", two_pass), ("fused_ms", one_pass)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = round(e0.elapsed_time(e1) / reps, 4)
+    fused.zero_()
+    unfused.zero_()
+    one_pass()
+    two_pass()
+    torch.cuda.synchronize()
+    diff = float((fused - unfused).abs().max() / unfused.abs().max())
+    flop = 8.0 * N * N * K * n_pol * n_ch
+    return {"inputs": N, "K": K, "channels": n_ch, "pols": n_pol, "grid": Ng, "seed": seed, **res,
+            "speedup": round(res["unfused_ms"] / res["fused_ms"], 3),
+            "fused_tflops_4m_equiv": round(flop / (res["fused_ms"] * 1e-3) / 1e12, 2),
+            "triangle_bytes_avoided": 8 * n_bl * n_ch * 2,
+            "max_rel_diff_vs_unfused": diff,
+            "identical_output": bool(torch.equal(fused.view(torch.int32), unfused.view(torch.int32)))}
+
+
 def streaming_study(torch, ftdx, dev, N: int, K: int, B: int, payloads: int, seed: int) -> dict:
     """Host payloads through ftdx_pipeline_* (H2D, HERK, D2H double-buffered): the
     PCIe-inclusive rate next to the HERK alone on resident data."""
@@ -603,6 +659,7 @@ def main() -> int:
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-beamformer", action="store_true")
     ap.add_argument("--no-polsum", action="store_true")
+    ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--voltage", action="store_true",
                     help="also time the voltage beamformer pipeline (a caller outside the correlator path)")
     ap.add_argument("--no-stages", action="store_true")
@@ -780,6 +837,14 @@ def main() -> int:
             f"({polsum['speedup']:.2f}x, identical={polsum['identical_output']})")
         torch.cuda.empty_cache()
 
+    grid = None
+    if not args.no_grid:
+        log("bench: HERK + pol sum + UV gridding, 4096 inputs x 16 ch x 2 pol, K=1024, 1024^2 grid ...")
+        grid = grid_study(torch, ftdx, dev, 4096, 1024, 16, 2, 1024, 12, 5)
+        log(f"bench: grid fused {grid['fused_ms']:.3f} ms vs HERK+pol sum then scatter {grid['unfused_ms']:.3f} ms "
+            f"({grid['speedup']:.2f}x, max rel diff {grid['max_rel_diff_vs_unfused']:.2e})")
+        torch.cuda.empty_cache()
+
     stream_res = None
     if not args.no_stream:
         log("bench: streaming host payloads 1024 x 16 x K=2048, 8 payloads ...")
@@ -843,6 +908,7 @@ def main() -> int:
         "streaming_host_payloads": stream_res,
         "beamformer_gemm_power": bf,
         "herk_pol_sum_fusion": polsum,
+        "herk_pol_grid_fusion": grid,
         "voltage_pipeline": volt,
         "stages_vs_hbm_peak": stages,
         "e3m2_operands": bf6,

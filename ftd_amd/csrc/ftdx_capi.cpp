@@ -1062,6 +1062,44 @@ int ftdx_herk_pol_reduce(ftdx_context* ctx, const void* A, void* C, int N, int K
     return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_herk_pol_reduce");
 }
 
+int ftdx_herk_pol_grid(ftdx_context* ctx, const void* A, int N, int K, int n_chan, int n_pol,
+                       const float* baseline_uv_m, const double* freq_hz, float* uv_grid, int Ng,
+                       int freq_offset, float cell_size_rad, float alpha, void* stream) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_grid: NULL context");
+    if (N < 0 || K < 0 || n_chan < 0 || n_pol <= 0 || Ng < 0 || freq_offset < 0)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_grid: bad sizes");
+    if ((int64_t)K * n_pol > 0x7FFFFFFF || (int64_t)n_chan * n_pol > 0x7FFFFFFF ||
+        (int64_t)Ng * Ng > ((int64_t)1 << 40))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_grid: sizes too large");
+    if (N == 0 || n_chan == 0 || Ng == 0) return FTDX_OK;
+    if (!uv_grid || !baseline_uv_m || !freq_hz || (!A && K > 0))
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_herk_pol_grid: NULL pointer");
+    ftdx::HerkPlainArgs a{};
+    a.A = static_cast<const uint8_t*>(A);
+    a.C = uv_grid;
+    a.N = N;
+    a.K = K;
+    a.batch = n_chan;
+    a.alpha = alpha;
+    a.beta = 0.0f;
+    a.a_bstride = (int64_t)N * K;
+    a.c_bstride = (int64_t)2 * Ng * Ng;
+    a.nseg = n_pol;
+    a.seg_group = 1;
+    a.uv_m = baseline_uv_m;
+    a.freq_hz = freq_hz;
+    a.freq_offset = freq_offset;
+    a.Ng = Ng;
+    a.cell = cell_size_rad;
+    const int cfg = ctx->forced_cfg >= 0 ? ftdx::tile_base(ctx->forced_cfg)
+                                         : (ftdx::direct_tile_to_cfg(ctx->direct_tile) >= 0
+                                                ? ftdx::direct_tile_to_cfg(ctx->direct_tile)
+                                                : ftdx::kDefaultCfg);
+    const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
+    hipError_t e = ftdx::launch_herk_grid(a, cfg, remap, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_herk_pol_grid");
+}
+
 int ftdx_pillbox_grid(const void* vis_tri, int vis_output, const float* baseline_uv_m,
                       const double* freq_hz, float* uv_grid, int n_baselines, int Ng, int Nf,
                       int freq_offset, float cell_size_rad, void* stream) {

@@ -39,13 +39,18 @@ int tile_config_wg_per_cu(int cfg);
 //   OUT_GEMM_BEAM     beamformer detect + integrate: 128-row tile = 64 samples of pol 0
 //                     then the same 64 of pol 1; out[col][m / nt] = sum over the nt
 //                     samples of (|pol 0|^2 + |pol 1|^2), quantized (uint8) or FP32
+//   OUT_GRID          pillbox UV gridding of each element (i, j <= i) instead of a store:
+//                     alpha * C_ij added into the FP32 [Ng][Ng][2] grid cell of baseline
+//                     i(i+1)/2 + j at this batch's frequency, and its conjugate at (-u, -v)
+//                     for i != j (PillboxGridScatter, include/kernels/visibility_bf.cuh:269-370)
 enum OutFormat {
     OUT_LOWER_PACKED = 0,
     OUT_UPPER_PACKED = 1,
     OUT_HERM_FULL = 2,
     OUT_GEMM_PLANAR = 3,
     OUT_GEMM_POWER = 4,
-    OUT_GEMM_BEAM = 5
+    OUT_GEMM_BEAM = 5,
+    OUT_GRID = 6
 };
 
 // Full M x N GEMM grid with B from its own rows (not the lower triangle of A A^H).
@@ -76,6 +81,14 @@ struct HerkPlainArgs {
     // n_time_inner).  Defaults: one input batch per output batch.
     int nseg = 1;
     int seg_group = 1;
+    // OUT_GRID (launch_herk_grid): C is the grid [batch][Ng][Ng][2] (c_bstride = 2 Ng^2
+    // floats); baseline UV in metres [N(N+1)/2][2] by packed lower index; output batch b
+    // uses frequency freq_hz[freq_offset + b] (Hz) and cell size `cell` (radians).
+    const float* uv_m = nullptr;
+    const double* freq_hz = nullptr;
+    int freq_offset = 0;
+    int Ng = 0;
+    float cell = 0.0f;
 };
 
 // Lower-triangle tiles per problem for a configuration.
@@ -89,6 +102,10 @@ bool mx4d_dense_ok(int cfg, int N, int64_t lda);
 // xcd_remap != 0.
 hipError_t launch_herk_int4_plain(const HerkPlainArgs& a, int cfg, bool out_fp16, int xcd_remap,
                                   hipStream_t stream);
+
+// HERK (QC input, K segments as in HerkPlainArgs::nseg) whose epilogue grids the
+// visibilities (OUT_GRID) instead of storing them; FP32 grid, accumulated atomically.
+hipError_t launch_herk_grid(const HerkPlainArgs& a, int cfg, int xcd_remap, hipStream_t stream);
 
 // GGP-convention outputs of a pre-cast (precast = 1) batch: fmt 1 = upper triangle
 // of M^H M packed row-major, fmt 2 = full N x N Hermitian row-major; element type

@@ -79,6 +79,12 @@ herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
         j.S = p.S + (int64_t)b * p.s_bstride;
         j.s_stage = p.s_stage;
     }
+    if constexpr (FMT == OUT_GRID) {
+        j.uv = p.uv_m;
+        j.lambda = 299792458.0 / p.freq_hz[p.freq_offset + b];  // speed of light / frequency
+        j.Ng = p.Ng;
+        j.cell = p.cell;
+    }
     TileRunner<Cfg>::template run<OutT, MODE, FMT>(j, lds);
 }
 
@@ -281,6 +287,23 @@ hipError_t launch_gemm_beam_cfg0(const GemmArgs& a, bool out_u8, int xcd_remap, 
     return hipGetLastError();
 }
 
+// Gridding epilogue on QC input (K segments allowed); FP32 grid.
+template <class Cfg>
+hipError_t launch_grid_cfg(const HerkPlainArgs& a, int xcd_remap, hipStream_t stream) {
+    const int tiles = tri_tiles(a.N, Cfg::TM, Cfg::TN);
+    const int64_t blocks = (int64_t)tiles * a.batch;
+    if (blocks > 0x7FFFFFFF || a.precast || !a.uv_m || !a.freq_hz || a.Ng <= 0) return hipErrorInvalidValue;
+    const bool vec = (a.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15u) == 0);
+    dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+    if (vec)
+        hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_VEC16, OUT_GRID>), grid, block, 0, stream, a,
+                           tiles, xcd_remap);
+    else
+        hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_BYTES, OUT_GRID>), grid, block, 0, stream, a,
+                           tiles, xcd_remap);
+    return hipGetLastError();
+}
+
 // GGP layouts: pre-cast FP8 operand rows only.
 template <class Cfg>
 hipError_t launch_ggp_cfg(const HerkPlainArgs& a, int fmt, bool out_fp64, int xcd_remap,
@@ -365,6 +388,11 @@ hipError_t launch_herk_ggp(const HerkPlainArgs& a, int cfg, int fmt, bool out_fp
     return with_cfg(cfg, [&](auto c) {
         return launch_ggp_cfg<decltype(c)>(a, fmt, out_fp64, xcd_remap, stream);
     });
+}
+
+hipError_t launch_herk_grid(const HerkPlainArgs& a, int cfg, int xcd_remap, hipStream_t stream) {
+    if (a.N <= 0 || a.batch <= 0) return hipSuccess;
+    return with_cfg(cfg, [&](auto c) { return launch_grid_cfg<decltype(c)>(a, xcd_remap, stream); });
 }
 
 hipError_t launch_gemm_beam(const GemmArgs& a, bool out_u8, int xcd_remap, hipStream_t stream) {

@@ -104,6 +104,12 @@ struct TileJob {
     // LOAD_MX4D: this problem's scales (row 0 of stage 0) and the bytes between stages
     const uint8_t* S;
     int64_t s_stage;
+    // OUT_GRID: baseline UV (metres, packed lower index), wavelength (metres), grid size
+    // and cell (radians); C is this batch's grid plane [Ng][Ng][2]
+    const float* uv;
+    double lambda;
+    int Ng;
+    float cell;
 };
 
 // 4 QC bytes (4 samples) -> u dwords: re = fp8(Re0..Re3), im = fp8(Im0..Im3).
@@ -562,6 +568,52 @@ struct TileRunner {
         }
     }
 
+    // Gridding epilogue (OUT_GRID): the pillbox scatter of the reference
+    // (include/kernels/visibility_bf.cuh:269-370) applied to each element straight from
+    // the accumulators, so the triangle never reaches HBM.  Element (i, j <= i) is
+    // baseline i(i+1)/2 + j; UV in wavelengths = float(uv_m / lambda) (double division);
+    // cell = roundf(u / cell + Ng/2); the visibility alpha * C_ij (Im = 0 on the diagonal)
+    // is added there and, for i != j, its conjugate at (-u, -v).
+    static __device__ __forceinline__ void epilogue_grid(const TileJob& j, const v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                         const v16f (&acc_im)[Cfg::FM][Cfg::FN], int wrow0,
+                                                         int wcol0, int lane) {
+        float* grid = static_cast<float*>(j.C);
+        const float half_ng = 0.5f * j.Ng;
+        const int n = lane & 31, g4 = (lane >> 5) * 4;
+#pragma unroll
+        for (int a = 0; a < Cfg::FM; ++a)
+#pragma unroll
+            for (int b = 0; b < Cfg::FN; ++b) {
+                const int gj = wcol0 + b * 32 + n;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int gi = wrow0 + a * 32 + (r >> 2) * 8 + g4 + (r & 3);
+                    if (gi >= j.N || gj > gi) continue;
+                    const float vre = acc_re[a][b][r] * j.alpha;
+                    const float vim = gi == gj ? 0.0f : acc_im[a][b][r] * j.alpha;
+                    const int64_t k = tri_index(gi, gj);
+                    const float u_lam = (float)((double)j.uv[2 * k] / j.lambda);
+                    const float v_lam = (float)((double)j.uv[2 * k + 1] / j.lambda);
+                    const int iu = (int)__builtin_roundf(u_lam / j.cell + half_ng);
+                    const int iv = (int)__builtin_roundf(v_lam / j.cell + half_ng);
+                    if (iu >= 0 && iu < j.Ng && iv >= 0 && iv < j.Ng) {
+                        float* g = grid + 2 * ((int64_t)iv * j.Ng + iu);
+                        unsafeAtomicAdd(g, vre);
+                        unsafeAtomicAdd(g + 1, vim);
+                    }
+                    if (gi != gj) {
+                        const int iuc = (int)__builtin_roundf(-u_lam / j.cell + half_ng);
+                        const int ivc = (int)__builtin_roundf(-v_lam / j.cell + half_ng);
+                        if (iuc >= 0 && iuc < j.Ng && ivc >= 0 && ivc < j.Ng) {
+                            float* g = grid + 2 * ((int64_t)ivc * j.Ng + iuc);
+                            unsafeAtomicAdd(g, vre);
+                            unsafeAtomicAdd(g + 1, -vim);
+                        }
+                    }
+                }
+            }
+    }
+
     // One stage of MFMAs on the staged rows in `st` (KC samples, all 32-sample steps).
     template <int FMT, int OPF = OPF_FP8>
     static __device__ __forceinline__ void mfma_stage(const uint8_t* st, int a_off, int b_off,
@@ -598,7 +650,7 @@ struct TileRunner {
                 const v8i av = conj_op<OPF>(au[a]);
 #pragma unroll
                 for (int b = 0; b < Cfg::FN; ++b) {
-                    if constexpr (FMT == OUT_LOWER_PACKED || gemm_layout(FMT)) {
+                    if constexpr (FMT == OUT_LOWER_PACKED || FMT == OUT_GRID || gemm_layout(FMT)) {
                         acc_re[a][b] = mfma_32x32x64<OPF>(au[a], bu[b], acc_re[a][b]);
                         acc_im[a][b] = mfma_32x32x64<OPF>(av, bu[b], acc_im[a][b]);
                     } else {  // transposed accumulator, same values
@@ -917,7 +969,7 @@ struct TileRunner {
                 const v8i av = conj_op<OPF>(au[a]);
 #pragma unroll
                 for (int b = 0; b < Cfg::FN; ++b) {
-                    if constexpr (FMT == OUT_LOWER_PACKED || gemm_layout(FMT)) {
+                    if constexpr (FMT == OUT_LOWER_PACKED || FMT == OUT_GRID || gemm_layout(FMT)) {
                         acc_re[a][b] = mfma_32x32x64<OPF>(au[a], bu[b], acc_re[a][b]);
                         acc_im[a][b] = mfma_32x32x64<OPF>(av, bu[b], acc_im[a][b]);
                     } else {
@@ -1096,6 +1148,9 @@ struct TileRunner {
 
         if constexpr (FMT == OUT_GEMM_BEAM) {
             epilogue_beam<OutT>(j, lds, acc_re, acc_im, active, wrow0, wcol0, lane, tid);
+            return;
+        } else if constexpr (FMT == OUT_GRID) {
+            if (active) epilogue_grid(j, acc_re, acc_im, wrow0, wcol0, lane);
             return;
         } else if constexpr (GEMM) {
             // B rows hold conj(B), so the HERK product A conj(B)^H = A B^T; lane =

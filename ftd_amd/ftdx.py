@@ -143,6 +143,7 @@ _SIGNATURES = {
     "ftdx_pol_reduce_triangle": (_I, [_VP, _VP, _I, _I, _I, _I, _I, _VP]),
     "ftdx_herk_pol_reduce": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _F, _F, _VP]),
     "ftdx_pillbox_grid": (_I, [_VP, _I, _VP, _VP, _VP, _I, _I, _I, _I, _F, _VP]),
+    "ftdx_herk_pol_grid": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP, _I, _I, _F, _F, _VP]),
     "ftdx_pipeline_create": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(_VP)]),
     "ftdx_pipeline_query": (_I, [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "ftdx_pipeline_correlate": (_I, [_VP, _VP, _VP]),
@@ -1036,4 +1037,15 @@ def pillbox_grid(vis_tri, baseline_uv_m, freq_hz, grid, n_baselines: int, Ng: in
     _check(lib().ftdx_pillbox_grid(_ptr(vis_tri), o, _ptr(baseline_uv_m), _ptr(freq_hz), _ptr(grid),
                                    n_baselines, Ng, Nf, freq_offset, cell_size_rad, _stream_ptr(stream)),
            "pillbox_grid")
+    return grid
+
+
+def herk_pol_grid(ctx: "CutlassComplexGemm", A, N: int, K: int, n_chan: int, n_pol: int, baseline_uv_m,
+                  freq_hz, grid, Ng: int, cell_size_rad: float, freq_offset: int = 0, alpha: float = 1.0,
+                  stream=None):
+    """HERK + polarization sum + pillbox gridding in one pass (ftdx_herk_pol_grid): QC bytes
+    A [n_chan, n_pol, N, K] -> adds into grid [n_chan, Ng, Ng, 2] float32 (caller zeroes it)."""
+    _check(lib().ftdx_herk_pol_grid(ctx.handle, _ptr(A), N, K, n_chan, n_pol, _ptr(baseline_uv_m),
+                                    _ptr(freq_hz), _ptr(grid), Ng, freq_offset, cell_size_rad, alpha,
+                                    _stream_ptr(stream)), "herk_pol_grid")
     return grid

@@ -376,6 +376,19 @@ int ftdx_herk_pol_reduce(ftdx_context* ctx, const void* A, void* C, int N, int K
 int ftdx_pillbox_grid(const void* vis_tri, int vis_output, const float* baseline_uv_m,
                       const double* freq_hz, float* uv_grid, int n_baselines, int Ng, int Nf,
                       int freq_offset, float cell_size_rad, void* stream);
+/* HERK, polarization sum and pillbox gridding in one pass: the triangle never reaches
+ * memory.  Replaces the reference's pol-reduce + grid sequence on the packed triangle
+ * (pol_reduce_triangle_kernel, lib/bench_pipeline.cu:254-302; PillboxGridScatter,
+ * include/kernels/visibility_bf.cuh:269-370, called at lib/imaging_pipeline.cu:338-356).
+ * A = QC bytes [n_chan][n_pol][N][K]; channel f's visibilities alpha * sum_pol X X^H
+ * (Im = 0 on the diagonal) go into uv_grid [n_chan][Ng][Ng][2] FP32 by the rules of
+ * ftdx_pillbox_grid at frequency freq_hz[freq_offset + f].  Accumulates: the caller
+ * zeroes the grid.  Equal to ftdx_herk_pol_reduce (n_time_inner = 1) followed by
+ * ftdx_pillbox_grid (Nf = n_chan) whenever the FP32 cell sums are exact (integer
+ * visibilities, |partial sums| < 2^24); otherwise equal up to the order of the adds. */
+int ftdx_herk_pol_grid(ftdx_context* ctx, const void* A, int N, int K, int n_chan, int n_pol,
+                       const float* baseline_uv_m, const double* freq_hz, float* uv_grid, int Ng,
+                       int freq_offset, float cell_size_rad, float alpha, void* stream);
 
 /* ---- complex GEMM with prepared B (the voltage beamformer's product) ----
  * C = alpha * A B^T + beta * C per batch, A [batch][M][K], B [batch][N][K], C

@@ -115,3 +115,72 @@ def test_pillbox_grid_large_baseline_index(oracle, dev):
     torch.cuda.synchronize()
     ref = oracle.pillbox_grid(vis, uv, freqs, n_bl, Ng, 1, cell)
     assert np.array_equal(grid.cpu().numpy(), ref)
+
+
+def _grid_ref(oracle, qc, N, K, n_ch, n_pol, uv, freqs, Ng, cell, freq_offset, alpha):
+    """Oracle: exact int64 HERK summed over polarizations, alpha applied in FP32, then the
+    oracle's pillbox scatter of each channel's triangle."""
+    per = oracle.herk_int64(qc, N, K, n_ch * n_pol).reshape(n_ch, n_pol, -1, 2).sum(axis=1)
+    vis = (per.astype(np.float32) * np.float32(alpha)).astype(np.float32)
+    vis[:, [i * (i + 1) // 2 + i for i in range(N)], 1] = 0.0
+    return oracle.pillbox_grid(vis, uv, freqs, N * (N + 1) // 2, Ng, n_ch, cell, freq_offset=freq_offset)
+
+
+GRID_CASES = [(96, 80, 3, 2, 0), (130, 64, 2, 2, 1), (64, 37, 2, 3, 2), (257, 128, 1, 1, 0)]
+
+
+@pytest.mark.parametrize("N,K,n_ch,n_pol,cfg", GRID_CASES)
+def test_fused_herk_pol_grid_bit_exact(oracle, dev, N, K, n_ch, n_pol, cfg):
+    """HERK + pol sum + gridding in one pass equals the oracle's HERK, pol sum and scatter
+    and the unfused GPU sequence (ftdx_herk_pol_reduce then ftdx_pillbox_grid), bit for bit:
+    integer visibilities keep every FP32 cell sum exact, so the atomic order is irrelevant.
+    UV from antenna positions (autos at the grid centre) plus baselines off the grid."""
+    import torch
+    from ftd_amd import ftdx
+    Ng, cell, alpha, fo = 64, 2e-3, 0.5, 1
+    qc = oracle.gen_qc_uniform(N * 7 + K, N * K * n_ch * n_pol)
+    A = torch.from_numpy(qc).to(dev)
+    rng = np.random.default_rng(N + K)
+    pos = rng.standard_normal((N, 2)) * 0.015
+    ii, jj = np.tril_indices(N)
+    uv = (pos[ii] - pos[jj]).astype(np.float32)
+    uv[::97] *= 40.0  # some baselines land outside the grid and are dropped
+    freqs = np.array([0.8e9, 0.95e9, 1.1e9, 1.25e9, 1.4e9])
+    uv_d, f_d = torch.from_numpy(uv).to(dev), torch.from_numpy(freqs).to(dev)
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, cfg & 1)
+    print(f"herk_pol_grid N={N} K={K} ch={n_ch} pol={n_pol} cfg={cfg}", flush=True)
+    fused = torch.zeros((n_ch, Ng, Ng, 2), dtype=torch.float32, device=dev)
+    ftdx.herk_pol_grid(g, A, N, K, n_ch, n_pol, uv_d, f_d, fused, Ng, cell, freq_offset=fo, alpha=alpha)
+    tri = ftdx.herk_pol_reduce(g, A, N, K, n_ch, n_pol, 1, alpha=alpha)
+    unfused = torch.zeros_like(fused)
+    ftdx.pillbox_grid(tri, uv_d, f_d, unfused, N * (N + 1) // 2, Ng, n_ch, cell, freq_offset=fo)
+    torch.cuda.synchronize()
+    ref = _grid_ref(oracle, qc, N, K, n_ch, n_pol, uv, freqs, Ng, cell, fo, alpha)
+    assert np.abs(ref).sum() > 0
+    assert np.array_equal(_bits(fused.cpu().numpy()), _bits(ref))
+    assert np.array_equal(_bits(unfused.cpu().numpy()), _bits(ref))
+
+
+def test_fused_herk_pol_grid_accumulates_and_validates(oracle, dev):
+    """The grid is accumulated into (two calls give twice the values) and bad arguments
+    are rejected before any launch."""
+    import torch
+    from ftd_amd import ftdx
+    N, K, n_ch, n_pol, Ng, cell = 64, 64, 1, 2, 32, 4e-3
+    qc = oracle.gen_qc_uniform(3, N * K * n_ch * n_pol)
+    A = torch.from_numpy(qc).to(dev)
+    rng = np.random.default_rng(1)
+    uv = torch.from_numpy((rng.standard_normal((N * (N + 1) // 2, 2)) * 0.01).astype(np.float32)).to(dev)
+    f = torch.tensor([1.0e9], dtype=torch.float64, device=dev)
+    g = ftdx.CutlassComplexGemm()
+    grid = torch.zeros((n_ch, Ng, Ng, 2), dtype=torch.float32, device=dev)
+    ftdx.herk_pol_grid(g, A, N, K, n_ch, n_pol, uv, f, grid, Ng, cell)
+    once = grid.clone()
+    ftdx.herk_pol_grid(g, A, N, K, n_ch, n_pol, uv, f, grid, Ng, cell)
+    torch.cuda.synchronize()
+    assert torch.equal(grid, 2 * once)
+    with pytest.raises(ftdx.FtdxError):
+        ftdx.herk_pol_grid(g, A, N, K, n_ch, 0, uv, f, grid, Ng, cell)
+    with pytest.raises(ftdx.FtdxError):
+        ftdx.herk_pol_grid(g, A, N, K, n_ch, n_pol, None, f, grid, Ng, cell)
