@@ -122,11 +122,13 @@ def test_mx_herk_every_tile_config(oracle, dev, compute, cfg, N):
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
 
 
-def test_mx_fp6_int4_bit_exact(oracle, dev):
-    """INT4 voltages are exact in FP6 E3M2 (scale 2^-2): the MX HERK is bit-exact."""
+@pytest.mark.parametrize("N", [130, 256])
+def test_mx_fp6_int4_bit_exact(oracle, dev, N):
+    """INT4 voltages are exact in FP6 E3M2 (scale 2^-2): the MX HERK is bit-exact, with
+    edge tiles (N = 130) and with every tile interior (N = 256)."""
     import torch
     from ftd_amd import ftdx
-    N, K, B = 130, 256, 2
+    K, B = 256, 2
     A = oracle.gen_qc_uniform(31, N * K * B)
     ctx = ftdx.CutlassComplexGemm()
     C = ftdx.herk_mx(ctx, torch.from_numpy(A).to(dev), N, K, B, ftdx.InputPrecision.INT4,
@@ -199,3 +201,59 @@ def test_herk_int4_fp6_is_lossless(oracle, dev, N, K, B, tune):
     torch.cuda.synchronize()
     ref = oracle.herk_f16(A, N, K, B, alpha=0.5, beta=2.0, C=C0)
     assert np.array_equal(C.cpu().numpy().view(np.uint16), ref)
+
+
+@pytest.mark.parametrize("compute", ["FP4"])
+@pytest.mark.parametrize("inp", ["FP16", "INT4"])
+@pytest.mark.parametrize("K", [300, 512])
+def test_mx_dense_rows_equal_inline_rows(oracle, dev, compute, inp, K):
+    """Dense E2M1 rows (32 code bytes per block, scales in a stage-major array) on the
+    default 256 x 128 tile give the same bytes as the 64-byte inline-scale rows on the
+    64 x 64 tile: every accumulator sees the same block-by-block sequence of scaled MFMAs.
+    Both input branches of the dense quantizer (FP16 and INT4 QC bytes)."""
+    import torch
+    from ftd_amd import ftdx
+    N, B = 512, 2
+    if inp == "FP16":
+        X = torch.from_numpy(_voltages(5 + K, B * N, K)).to(dev)
+    else:
+        X = torch.from_numpy(oracle.gen_bytes(6 + K, B * N * K)).to(dev)
+    comp = getattr(ftdx.ComputePrecision, compute)
+    ip = getattr(ftdx.InputPrecision, inp)
+    dense_ctx = ftdx.CutlassComplexGemm()          # default tile: all tiles interior at N = 512
+    inline_ctx = ftdx.CutlassComplexGemm()
+    inline_ctx.set_launch_config(2, 1)              # 64 x 64 tile: no dense rows
+    Cd = ftdx.herk_mx(dense_ctx, X, N, K, B, ip, comp, output="fp32")
+    Ci = ftdx.herk_mx(inline_ctx, X, N, K, B, ip, comp, output="fp32")
+    torch.cuda.synchronize()
+    assert torch.equal(Cd.view(torch.int32), Ci.view(torch.int32))
+
+
+def test_mx_fp6_rows_past_4gib(oracle, dev):
+    """FP6 MX HERK whose operand rows span more than 2^32 bytes per batch (256 inputs,
+    K = 8,499,200: 4.35 GB of 64-byte-per-block rows, row 255 wholly past 4 GiB), so the
+    tile loads take the 64-bit addressing path.  The voltages are zero except at 48 sample
+    positions spread over K (the last ones within the final 100k samples), where every
+    input gets an INT4 value; the result must equal the oracle HERK of those 48 columns bit
+    for bit (INT4 is exact in E3M2 and the sums stay far below 2^24)."""
+    import torch
+    from ftd_amd import ftdx
+    N, K, B = 256, 8_499_200, 1
+    rng = np.random.default_rng(42)
+    pos = np.concatenate([rng.choice(1_000_000, 16, replace=False),
+                          4_000_000 + rng.choice(1_000_000, 16, replace=False),
+                          K - 100_000 + rng.choice(100_000, 16, replace=False)])
+    pos = np.sort(pos)
+    sub = oracle.gen_qc_uniform(9, N * pos.size).reshape(N, pos.size)
+    A = torch.zeros((N, K), dtype=torch.uint8, device=dev)
+    A[:, torch.from_numpy(pos).to(dev)] = torch.from_numpy(sub).to(dev)
+    ctx = ftdx.CutlassComplexGemm()
+    print("herk_mx FP6 INT4 N=256 K=8499200 (rows past 4 GiB)", flush=True)
+    C = ftdx.herk_mx(ctx, A.view(-1), N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP6,
+                     output="fp32")
+    torch.cuda.synchronize()
+    ref = oracle.herk_f32(np.ascontiguousarray(sub).reshape(-1), N, pos.size, B)
+    assert np.array_equal(C.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    del A, C
+    ctx.end_herk()
+    torch.cuda.empty_cache()
