@@ -981,6 +981,33 @@ struct TileRunner {
         }
     }
 
+    // One stage of the direct-load loop: loads of the next stage into `nxt` (and its
+    // scales into `snx`), MFMAs on `cur`.  The buffers are distinct, and saying so
+    // (__restrict__) lets the compiler see that this stage's LDS reads cannot alias the
+    // loads in flight: it then waits for those loads only at the next barrier instead of
+    // before the first read of `cur`, so the loads overlap the MFMAs.
+    template <int FMT, bool ACTIVE, int OPF>
+    static __device__ __forceinline__ void dma_stage_body(const TileJob& j, const uint8_t* __restrict__ cur,
+                                                          uint8_t* __restrict__ nxt, const uint8_t* __restrict__ scur,
+                                                          uint8_t* __restrict__ snx, const uint8_t* B,
+                                                          const uint32_t (&off)[DOPS], int s, int nst, int lane,
+                                                          int wave, int a_row, int b_row,
+                                                          v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                          v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+        constexpr bool DENSE = OPF == OPF_MX4D;
+        if (s + 1 < nst) {
+            const int64_t k1 = (int64_t)(s + 1) * 2 * Cfg::KC;
+            dma_stage(j.A + k1, B + k1, nxt, wave, off);
+            if constexpr (DENSE) dma_scales(j, s + 1, snx, wave, lane);
+        }
+        if constexpr (ACTIVE) {
+            if constexpr (DENSE)
+                mfma_stage_mx4d<FMT>(cur, scur, a_row, b_row, lane, acc_re, acc_im);
+            else
+                mfma_stage_dma<FMT, OPF>(cur, a_row, b_row, lane, acc_re, acc_im);
+        }
+    }
+
     // Two stage buffers, loads one stage ahead, one barrier per stage: the barrier
     // that publishes stage s also retires every wave's reads of stage s - 1, whose
     // buffer the loads of stage s + 1 then overwrite.  Only these loads are in flight
@@ -995,24 +1022,16 @@ struct TileRunner {
         uint32_t off[DOPS];
         dma_offsets<GEMM>(j, lane, wave, off);
         constexpr bool DENSE = OPF == OPF_MX4D;
-        [[maybe_unused]] uint8_t* const scl = lds + 2 * DSTAGE;  // dense E2M1: two scale buffers after the codes
+        uint8_t* const scl = lds + 2 * DSTAGE;  // dense E2M1: two scale buffers after the codes
         dma_stage(j.A, B, lds, wave, off);
         if constexpr (DENSE) dma_scales(j, 0, scl, wave, lane);
         for (int s = 0; s < nst; ++s) {
             wait_vmcnt<0>();  // this wave's loads of stage s landed
             __syncthreads();
-            uint8_t* cur = lds + (s & 1) * DSTAGE;
-            if (s + 1 < nst) {
-                const int64_t k1 = (int64_t)(s + 1) * 2 * Cfg::KC;
-                dma_stage(j.A + k1, B + k1, lds + ((s + 1) & 1) * DSTAGE, wave, off);
-                if constexpr (DENSE) dma_scales(j, s + 1, scl + ((s + 1) & 1) * SC_STAGE, wave, lane);
-            }
-            if constexpr (ACTIVE) {
-                if constexpr (DENSE)
-                    mfma_stage_mx4d<FMT>(cur, scl + (s & 1) * SC_STAGE, a_row, b_row, lane, acc_re, acc_im);
-                else
-                    mfma_stage_dma<FMT, OPF>(cur, a_row, b_row, lane, acc_re, acc_im);
-            }
+            const int c = s & 1, n = c ^ 1;
+            dma_stage_body<FMT, ACTIVE, OPF>(j, lds + c * DSTAGE, lds + n * DSTAGE, scl + c * SC_STAGE,
+                                             scl + n * SC_STAGE, B, off, s, nst, lane, wave, a_row, b_row, acc_re,
+                                             acc_im);
         }
         __syncthreads();
     }
