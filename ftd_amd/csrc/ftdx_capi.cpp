@@ -342,10 +342,10 @@ int gemm_run(ftdx_context* ctx, const uint8_t* brows, GemmInput kind, const void
     g.b_bstride = (int64_t)N * Kp * 2;
     g.ldc = N;
     g.c_bstride = (int64_t)M * N;
-    // 128x128 tiles, two workgroups per CU: fastest at the beamformer shape (M = 256,
-    // N = 4000, K = 1664: 1.01 ms against 1.08 ms for 256-row tiles, which read each
-    // column block of B once but run one workgroup per CU)
-    const int cfg = ctx->forced_cfg >= 0 ? tile_base(ctx->forced_cfg) : TILE_128x128_K64;
+    // 256x128 tiles, one workgroup per CU, whose direct loads run two stages ahead in three
+    // LDS buffers: fastest at the beamformer shape (M = 256, N = 4000, K = 1664: 0.90 ms
+    // against 0.91-0.96 ms for the 128x128 tiles, two workgroups per CU)
+    const int cfg = ctx->forced_cfg >= 0 ? tile_base(ctx->forced_cfg) : TILE_256x128_K64;
     const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : kDefaultRemap;
     e = launch_gemm_fp8(g, cfg, fmt, out_fp16, remap, s);
     return e == hipSuccess ? FTDX_OK : hip_fail(e, what);

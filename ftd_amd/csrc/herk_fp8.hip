@@ -200,7 +200,11 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
 template <class Cfg, typename OutT, int FMT, int MODE = LOAD_FP8>
 __global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
 gemm_fp8_kernel(GemmArgs p, int tiles_m, int tiles, int xcd_remap) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
+    // three direct-load stage buffers where the configuration runs one workgroup per CU
+    constexpr int LDS = (MODE == LOAD_FP8 && TileRunner<Cfg>::DMA3_OK && TileRunner<Cfg>::DMA3_LDS > Cfg::LDS_BYTES)
+                            ? TileRunner<Cfg>::DMA3_LDS
+                            : Cfg::LDS_BYTES;
+    __shared__ __attribute__((aligned(16))) uint8_t lds[LDS];
     const int total = tiles * p.batch;
     const int w = xcd_remap ? xcd_logical((int)blockIdx.x, total) : (int)blockIdx.x;
     const int b = w / tiles;

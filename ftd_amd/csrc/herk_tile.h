@@ -1036,6 +1036,65 @@ struct TileRunner {
         __syncthreads();
     }
 
+    // ---- Three stage buffers, loads two stages ahead (GEMM layouts) ----
+    // The GEMM streams its column operand from HBM once per tile, so its loads need more
+    // time in flight than one stage.  With a third buffer the loads of stage s + 2 are
+    // issued right after the barrier that opens stage s, into the buffer of stage s - 1,
+    // which every wave finished reading before that barrier.  The barrier is written out
+    // (wait for this wave's loads of stage s + 1 only, then s_barrier): the release fence
+    // of __syncthreads() would also wait for the loads of stage s + 2.
+    static constexpr int DMA3_LDS = 3 * DSTAGE;
+    static constexpr bool DMA3_OK = DMA_OK && Cfg::WG_PER_CU == 1 && DMA3_LDS <= 160 * 1024;
+
+    template <int VM>
+    static __device__ __forceinline__ void stage_barrier() {
+        wait_vmcnt<VM>();
+        __builtin_amdgcn_s_waitcnt(0xC07F & ~0x0F00);  // lgkmcnt(0): this wave's LDS reads are done
+        __asm__ __volatile__("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        __asm__ __volatile__("" ::: "memory");
+    }
+
+    template <int FMT, bool ACTIVE>
+    static __device__ __forceinline__ void dma3_stage_body(const TileJob& j, const uint8_t* __restrict__ cur,
+                                                           uint8_t* __restrict__ fre, const uint8_t* B,
+                                                           const uint32_t (&off)[DOPS], int s, int nst, int lane,
+                                                           int wave, int a_row, int b_row,
+                                                           v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                           v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+        if (s + 2 < nst) {
+            const int64_t k2 = (int64_t)(s + 2) * 2 * Cfg::KC;
+            dma_stage(j.A + k2, B + k2, fre, wave, off);
+        }
+        if constexpr (ACTIVE) mfma_stage_dma<FMT, OPF_FP8>(cur, a_row, b_row, lane, acc_re, acc_im);
+    }
+
+    template <int FMT, bool ACTIVE>
+    static __device__ __forceinline__ void dma3_tile_loop(const TileJob& j, uint8_t* __restrict__ lds, int lane,
+                                                          int wave, int nst, int a_row, int b_row,
+                                                          v16f (&acc_re)[Cfg::FM][Cfg::FN],
+                                                          v16f (&acc_im)[Cfg::FM][Cfg::FN]) {
+        constexpr bool GEMM = gemm_layout(FMT);
+        const uint8_t* B = GEMM ? j.B : j.A;
+        uint32_t off[DOPS];
+        dma_offsets<GEMM>(j, lane, wave, off);
+        dma_stage(j.A, B, lds, wave, off);
+        if (nst > 1) dma_stage(j.A + 2 * Cfg::KC, B + 2 * Cfg::KC, lds + DSTAGE, wave, off);
+        int c = 0;  // buffer of stage s
+        for (int s = 0; s < nst; ++s) {
+            if (s + 1 < nst)
+                stage_barrier<DOPS>();  // stage s landed; stage s + 1 may still be in flight
+            else
+                stage_barrier<0>();
+            const int f = c == 0 ? 2 : c - 1;  // buffer of stage s - 1 = buffer of stage s + 2
+            dma3_stage_body<FMT, ACTIVE>(j, lds + c * DSTAGE, lds + f * DSTAGE, B, off, s, nst, lane, wave, a_row,
+                                         b_row, acc_re, acc_im);
+            c = c == 2 ? 0 : c + 1;
+        }
+        wait_vmcnt<0>();
+        __syncthreads();
+    }
+
     // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
     // live and would spill the second register set).
     template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false>
@@ -1112,7 +1171,17 @@ struct TileRunner {
                 looped = true;
             }
         }
-        if constexpr (PRE && DMA_OK) {
+        if constexpr (MODE == LOAD_FP8 && GEMM && DMA3_OK) {
+            if (dma_tile<GEMM>(j)) {
+                const int a_row = wm * Cfg::WM + (lane & 31);
+                const int b_row = Cfg::TM + wn * Cfg::WN + (lane & 31);
+                if (active)
+                    dma3_tile_loop<FMT, true>(j, lds, lane, wave, nst, a_row, b_row, acc_re, acc_im);
+                else
+                    dma3_tile_loop<FMT, false>(j, lds, lane, wave, nst, a_row, b_row, acc_re, acc_im);
+                looped = true;
+            }
+        } else if constexpr (PRE && DMA_OK) {
             if (dma_tile<GEMM>(j)) {
                 const int a_row = wm * Cfg::WM + (lane & 31);
                 const int b_row = (share_cols ? (j.col0 - j.row0) : Cfg::TM) + wn * Cfg::WN + (lane & 31);
