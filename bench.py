@@ -163,8 +163,9 @@ def batched_vs_plain(torch, ftdx, dev, count: int, seed: int, reps: int) -> dict
 def mx_study(torch, ftdx, dev, N: int, K: int, B: int, seed: int, reps: int) -> dict:
     """MX block-scaled HERK at 4096 inputs x 32 channels x K=4096: complex
     Gaussian voltages (E|x|^2 = 1) with 1% impulsive 30-sigma outliers of uniform
-    phase, as FP16; FP6 E3M2 and FP4 E2M1 with 32-sample E8M0 scales.  Error per
-    channel vs the FP64 HERK of the same FP16 voltages (complex128 matmul)."""
+    phase, as FP16; FP6 E3M2 and FP4 E2M1 with 32-sample E8M0 scales, under the
+    reference's element converters (the default) and under OCP nearest-even rounding.
+    Error per channel vs the FP64 HERK of the same FP16 voltages (complex128 matmul)."""
     g = torch.Generator(device=dev).manual_seed(seed)
     x = torch.complex(torch.randn((B, N, K), generator=g, device=dev),
                       torch.randn((B, N, K), generator=g, device=dev)) * (0.5 ** 0.5)
@@ -177,34 +178,44 @@ def mx_study(torch, ftdx, dev, N: int, K: int, B: int, seed: int, reps: int) -> 
     ctx = ftdx.CutlassComplexGemm()
     flop = 8.0 * N * N * K * B
     out = {"inputs": N, "channels": B, "K": K, "seed": seed,
-           "data": "complex Gaussian sigma=1 + 1% 30-sigma outliers, FP16"}
+           "data": "complex Gaussian sigma=1 + 1% 30-sigma outliers, FP16",
+           "rounding": "reference converters (default); nearest_even entries: OCP rounding"}
+    runs = {}
     for name, comp in (("fp6_e3m2", ftdx.ComputePrecision.FP6), ("fp4_e2m1", ftdx.ComputePrecision.FP4)):
-        C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
-        ftdx.herk_mx(ctx, X16, N, K, B, ftdx.InputPrecision.FP16, comp, output="fp32", C=C)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
+        for rname, rnd in (("reference", ftdx.MxRounding.Reference), ("nearest_even", ftdx.MxRounding.NearestEven)):
+            ctx.set_mx_rounding(rnd)
+            C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
             ftdx.herk_mx(ctx, X16, N, K, B, ftdx.InputPrecision.FP16, comp, output="fp32", C=C)
-        e1.record()
-        e1.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        fro, mxe = [], []
-        for b in range(B):
-            xb = X16[b].double()
-            xc = torch.complex(xb[..., 0], xb[..., 1])
-            R = (xc @ xc.conj().transpose(0, 1).contiguous())[ii, jj]
-            R = torch.view_as_real(R).clone()
-            R[ii == jj, 1] = 0.0
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                ftdx.herk_mx(ctx, X16, N, K, B, ftdx.InputPrecision.FP16, comp, output="fp32", C=C)
+            e1.record()
+            e1.synchronize()
+            runs[(name, rname)] = (e0.elapsed_time(e1) / reps, C, [], [])
+    for b in range(B):
+        xb = X16[b].double()
+        xc = torch.complex(xb[..., 0], xb[..., 1])
+        R = (xc @ xc.conj().transpose(0, 1).contiguous())[ii, jj]
+        R = torch.view_as_real(R).clone()
+        R[ii == jj, 1] = 0.0
+        for ms, C, fro, mxe in runs.values():
             d = C[b].double() - R
             fro.append(float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(R)))
             mxe.append(float(d.abs().max() / R.abs().max()))
-            del xb, xc, R, d
-        out[name] = {"ms": round(ms, 4), "tflops_4m_equiv": round(flop / (ms * 1e-3) / 1e12, 2),
-                     "pct_mx_dense_peak": round(100 * flop / (ms * 1e-3) / 1e12 / MX_DENSE_PEAK_TFLOPS, 2),
-                     "rel_frobenius": {"min": min(fro), "median": sorted(fro)[B // 2], "max": max(fro)},
-                     "rel_max_element": {"min": min(mxe), "median": sorted(mxe)[B // 2], "max": max(mxe)}}
-        del C
+            del d
+        del xb, xc, R
+    for (name, rname), (ms, C, fro, mxe) in runs.items():
+        rec = {"ms": round(ms, 4), "tflops_4m_equiv": round(flop / (ms * 1e-3) / 1e12, 2),
+               "pct_mx_dense_peak": round(100 * flop / (ms * 1e-3) / 1e12 / MX_DENSE_PEAK_TFLOPS, 2),
+               "rel_frobenius": {"min": min(fro), "median": sorted(fro)[B // 2], "max": max(fro)},
+               "rel_max_element": {"min": min(mxe), "median": sorted(mxe)[B // 2], "max": max(mxe)}}
+        if rname == "reference":
+            out[name] = rec
+        else:
+            out[name]["nearest_even"] = rec
+    del runs
     torch.cuda.empty_cache()
     return out
 

@@ -37,6 +37,8 @@ struct ftdx_context {
     bool tune_loaded = false;
     ftdx::TuneCache tune;
     int init_N = 0, init_K = 0, init_batch = 0;
+    // MX element rounding (ftdx_set_mx_rounding): the reference's converters by default
+    int mx_rounding = FTDX_MX_ROUND_REFERENCE;
     // MX path scratch (operand rows: codes with their scales inline), grown on demand
     uint8_t* mx_codes = nullptr;
     size_t mx_codes_cap = 0;
@@ -445,6 +447,16 @@ int ftdx_set_launch_config(ftdx_context* ctx, int tile_config, int xcd_remap) {
     return FTDX_OK;
 }
 
+int ftdx_set_mx_rounding(ftdx_context* ctx, int rounding) {
+    if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_mx_rounding: NULL context");
+    if (rounding != FTDX_MX_ROUND_REFERENCE && rounding != FTDX_MX_ROUND_NEAREST_EVEN)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_mx_rounding: unknown rounding");
+    ctx->mx_rounding = rounding;
+    return FTDX_OK;
+}
+
+int ftdx_get_mx_rounding(const ftdx_context* ctx) { return ctx ? ctx->mx_rounding : -1; }
+
 int ftdx_set_direct_herk_graph(ftdx_context* ctx, int enable) {
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_set_direct_herk_graph: NULL context");
     ctx->graph = enable ? 1 : 0;
@@ -641,8 +653,15 @@ int ftdx_mx_layout(int compute, int K, int* blocks, int* code_bytes) {
 
 int ftdx_mx_quantize(const void* X, int input, int compute, void* codes, void* scales,
                      int64_t rows, int K, void* stream) {
+    return ftdx_mx_quantize_rounding(X, input, compute, FTDX_MX_ROUND_REFERENCE, codes, scales, rows, K, stream);
+}
+
+int ftdx_mx_quantize_rounding(const void* X, int input, int compute, int rounding, void* codes, void* scales,
+                              int64_t rows, int K, void* stream) {
     if (compute != FTDX_COMPUTE_FP6 && compute != FTDX_COMPUTE_FP4)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: compute must be FP6 or FP4");
+    if (rounding != FTDX_MX_ROUND_REFERENCE && rounding != FTDX_MX_ROUND_NEAREST_EVEN)
+        return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: unknown rounding");
     if (input != FTDX_INPUT_INT4 && input != FTDX_INPUT_FP16)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: unknown input precision");
     if (rows < 0 || K < 0) return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: negative size");
@@ -651,7 +670,7 @@ int ftdx_mx_quantize(const void* X, int input, int compute, void* codes, void* s
     if ((reinterpret_cast<uintptr_t>(codes) & 7u) || (input == FTDX_INPUT_FP16 && (reinterpret_cast<uintptr_t>(X) & 3u)))
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_mx_quantize: misaligned buffer");
     hipError_t e = ftdx::launch_mx_quantize(
-        X, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16, ftdx::mx_format(compute),
+        X, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16, ftdx::mx_format(compute), rounding,
         static_cast<uint8_t*>(codes), static_cast<uint8_t*>(scales), rows, K,
         static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "ftdx_mx_quantize: launch");
@@ -695,7 +714,7 @@ int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int ba
         if (rc != FTDX_OK) return rc;
         uint8_t* scales = ctx->mx_codes + code_bytes;  // code_bytes is a multiple of 128
         e = ftdx::launch_mx_quantize_dense(A, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16,
-                                           ctx->mx_codes, scales, rows, N, K, s);
+                                           ctx->mx_rounding, ctx->mx_codes, scales, rows, N, K, s);
         if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: quantize launch");
         ftdx::HerkPlainArgs a{};
         a.A = ctx->mx_codes;
@@ -727,7 +746,7 @@ int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int ba
     if (rc != FTDX_OK) return rc;
     if (nblkp > 0) {
         e = ftdx::launch_mx_quantize_rows(A, nullptr, input == FTDX_INPUT_INT4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16,
-                                          fmt, false, ctx->mx_codes, rows, K, s);
+                                          fmt, ctx->mx_rounding, false, ctx->mx_codes, rows, K, s);
         if (e != hipSuccess) return hip_fail(e, "ftdx_herk_mx: quantize launch");
     }
     ftdx::HerkPlainArgs a{};
@@ -1137,7 +1156,8 @@ int prepare_rows(ftdx_context* ctx, uint8_t*& buf, size_t& cap,
         const int64_t rows = (int64_t)N * batch;
         int rc = ftdx::ensure_scratch(buf, cap, (size_t)rows * nblkp * 64 + 16, what);
         if (rc != FTDX_OK) return rc;
-        hipError_t e = ftdx::launch_mx_quantize_rows(B_re, B_im, ftdx::MX_IN_FP16_PLANAR, fmt, true, buf, rows, K,
+        hipError_t e = ftdx::launch_mx_quantize_rows(B_re, B_im, ftdx::MX_IN_FP16_PLANAR, fmt, ctx->mx_rounding,
+                                                     true, buf, rows, K,
                                                      static_cast<hipStream_t>(stream));
         return e == hipSuccess ? FTDX_OK : hip_fail(e, what);
     }
@@ -1183,7 +1203,7 @@ int gemm_mx_run(ftdx_context* ctx, const uint8_t* brows, int fmt, bool int4, con
     int rc = ftdx::ensure_scratch(ctx->mx_codes, ctx->mx_codes_cap, (size_t)(rows * pitch) + 16, what);
     if (rc != FTDX_OK) return rc;
     hipError_t e = ftdx::launch_mx_quantize_rows(a0, a1, int4 ? ftdx::MX_IN_INT4 : ftdx::MX_IN_FP16_PLANAR, fmt,
-                                                 false, ctx->mx_codes, rows, K, s);
+                                                 ctx->mx_rounding, false, ctx->mx_codes, rows, K, s);
     if (e != hipSuccess) return hip_fail(e, what);
     ftdx::GemmArgs g;
     g.A = ctx->mx_codes;

@@ -18,12 +18,18 @@
 //                 Re C_ij = sum (re_i re_j + im_i im_j)     A = [re | im],  B = [re | im]
 //                 Im C_ij = sum (im_i re_j - re_i im_j)     A = [im | -re], B = [re | im]
 //              each lane's 32 values share one scale, which the MFMA applies.
-// Element conversion: nearest-even with subnormals, saturating at the format
-// maximum (OCP Microscaling Formats v1.0), by the gfx950 converters
-// (v_cvt_scalef32_pk_fp4_f32, v_cvt_scalef32_2xpk16_bf6_f32) on the values times the
-// block's inverse scale; checked against the CPU restatement's codes for every case the
-// tests cover.  Codes per value are packed LSB first (FP4: nibble i at bits 4i; FP6:
-// bits 6i..6i+5 of the lane's dwords).
+// Element conversion of the values times the block's inverse scale, two rules:
+//   MX_ROUND_REFERENCE     the reference's converters (fp6_kernels.hpp:27-58 for E3M2,
+//                          fp4_kernels.hpp:7-30 for E2M1): clamp to +-max, magnitudes
+//                          below 0.25 to a zero with the sign of x < 0, exponent clamped
+//                          to the normal range, E3M2 mantissa rounded to nearest even and
+//                          clamped to 3 (no carry into the exponent), E2M1 mantissa bit
+//                          truncated; computed here on the float's bit fields.
+//   MX_ROUND_NEAREST_EVEN  nearest-even with subnormals, saturating at the format maximum
+//                          (OCP Microscaling Formats v1.0), by the gfx950 converters
+//                          (v_cvt_scalef32_pk_fp4_f32, v_cvt_scalef32_2xpk16_bf6_f32).
+// Both are checked against the CPU restatement's codes.  Codes per value are packed LSB
+// first (FP4: nibble i at bits 4i; FP6: bits 6i..6i+5 of the lane's dwords).
 #include <hip/hip_runtime.h>
 
 #include "herk_common.h"
@@ -71,6 +77,33 @@ __device__ __forceinline__ uint32_t mx_scale_exponent(float amax) {
     return (uint32_t)b;
 }
 
+// The reference's element converter on a scaled value (fp6_kernels.hpp:27-58,
+// fp4_kernels.hpp:7-30), from the float's bit fields: a >= 0.25 is a normal float whose
+// exponent e gives the E3M2 biased exponent e + 3 (1..7 for a in [0.25, 28]) or the E2M1
+// one max(e + 1, 1); the E3M2 mantissa is the top 23 - 21 bits rounded to nearest even
+// (rint(4 * (frac - 1)) of the reference) and clamped to 3, the E2M1 bit is bit 22.
+template <int FMT>
+__device__ __forceinline__ uint32_t ref_element_code(float x) {
+    constexpr float MAXV = MxFmt<FMT>::MAXV;
+    constexpr int S = FMT == MX_FP4_E2M1 ? 3 : 5;
+    float v = __builtin_fminf(__builtin_fmaxf(x, -MAXV), MAXV);
+    if (x != x) v = -MAXV;  // fmaxf(NaN, -max) = -max in the reference
+    const uint32_t sign = v < 0.0f ? 1u : 0u;
+    const float a = __builtin_fabsf(v);
+    if (a < 0.25f) return sign << S;
+    const uint32_t bits = __float_as_uint(a);
+    const int e = (int)(bits >> 23) - 127;
+    const uint32_t man = bits & 0x7FFFFFu;
+    if constexpr (FMT == MX_FP4_E2M1) {
+        const int eb = e + 1 < 1 ? 1 : e + 1;
+        return (sign << 3) | ((uint32_t)eb << 1) | (man >> 22);
+    } else {
+        uint32_t m = (man + 0xFFFFFu + ((man >> 21) & 1u)) >> 21;
+        m = m > 3u ? 3u : m;
+        return (sign << 5) | ((uint32_t)(e + 3) << 2) | m;
+    }
+}
+
 // Sign-magnitude nibble -> value; the "-0" nibble gives +0 (integer negate first).
 __device__ __forceinline__ float qc_nibble_value(uint32_t n) {
     const int m = (int)(n & 7u);
@@ -88,7 +121,7 @@ __device__ __forceinline__ float qc_nibble_value(uint32_t n) {
 // DENSE (E2M1, with ROWS): codes = rows [rows][nblkp][2][16] (the 16 code bytes of the Re
 // part, then of the Im part), scales = [rows / nrow][nblkp / 4][nrow][8]: per stage of 4
 // blocks, each row's (Re, Im) scale bytes of those blocks.
-template <int FMT, int IN, bool ROWS = false, bool DENSE = false>
+template <int FMT, int IN, int RND, bool ROWS = false, bool DENSE = false>
 __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict__ X,
                                                           const void* __restrict__ X2,
                                                           uint8_t* __restrict__ codes,
@@ -186,7 +219,19 @@ __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict
         // v * inv is exact (a power of two within range); the gfx950 converters then
         // round to nearest even with subnormals and saturate, the OCP MX element rule
         uint32_t w[6] = {0u, 0u, 0u, 0u, 0u, 0u};
-        if constexpr (FMT == MX_FP4_E2M1) {
+        if constexpr (RND == MX_ROUND_REFERENCE) {
+#pragma unroll
+            for (int i = 0; i < 32; ++i) {
+                const uint32_t c = ref_element_code<FMT>(v[part][i] * inv);
+                if constexpr (FMT == MX_FP4_E2M1) {
+                    w[i >> 3] |= c << (4 * (i & 7));
+                } else {
+                    const int p = 6 * i;
+                    w[p >> 5] |= c << (p & 31);
+                    if ((p & 31) > 26) w[(p >> 5) + 1] |= c >> (32 - (p & 31));
+                }
+            }
+        } else if constexpr (FMT == MX_FP4_E2M1) {
 #pragma unroll
             for (int i = 0; i < 32; i += 2) {
                 const uint32_t b = __builtin_amdgcn_cvt_scalef32_pk_fp4_f32(0u, v[part][i] * inv, v[part][i + 1] * inv,
@@ -233,9 +278,9 @@ __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict
 
 constexpr int MX_BLK_PER_STAGE = 2;  // 32-sample blocks per 64-sample tile stage
 
-template <int FMT, bool ROWS = false>
-hipError_t quantize_fmt(const void* X, const void* X2, int input, bool conj, uint8_t* codes,
-                        uint8_t* scales, int64_t rows, int K, int nblkp, hipStream_t s) {
+template <int FMT, int RND, bool ROWS>
+hipError_t quantize_fmt_rnd(const void* X, const void* X2, int input, bool conj, uint8_t* codes,
+                            uint8_t* scales, int64_t rows, int K, int nblkp, hipStream_t s) {
     const int64_t total = rows * nblkp;
     const unsigned blocks = (unsigned)((total + 255) / 256);
     const int cj = conj ? 1 : 0;
@@ -244,15 +289,26 @@ hipError_t quantize_fmt(const void* X, const void* X2, int input, bool conj, uin
     const int vec = ((int64_t)K * esz % 16 == 0 &&
                      ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(X2)) & 15u) == 0) ? 1 : 0;
     if (input == MX_IN_INT4)
-        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_INT4, ROWS>), dim3(blocks), dim3(256), 0, s, X, X2,
+        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_INT4, RND, ROWS>), dim3(blocks), dim3(256), 0, s, X, X2,
                            codes, scales, rows, K, nblkp, cj, vec);
     else if (input == MX_IN_FP16_PLANAR)
-        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_FP16_PLANAR, ROWS>), dim3(blocks), dim3(256), 0, s,
-                           X, X2, codes, scales, rows, K, nblkp, cj, vec);
+        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_FP16_PLANAR, RND, ROWS>), dim3(blocks), dim3(256), 0,
+                           s, X, X2, codes, scales, rows, K, nblkp, cj, vec);
     else
-        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_FP16, ROWS>), dim3(blocks), dim3(256), 0, s, X, X2,
+        hipLaunchKernelGGL((mx_quantize_kernel<FMT, MX_IN_FP16, RND, ROWS>), dim3(blocks), dim3(256), 0, s, X, X2,
                            codes, scales, rows, K, nblkp, cj, vec);
     return hipGetLastError();
+}
+
+template <int FMT, bool ROWS = false>
+hipError_t quantize_fmt(const void* X, const void* X2, int input, int rounding, bool conj, uint8_t* codes,
+                        uint8_t* scales, int64_t rows, int K, int nblkp, hipStream_t s) {
+    if (rounding == MX_ROUND_REFERENCE)
+        return quantize_fmt_rnd<FMT, MX_ROUND_REFERENCE, ROWS>(X, X2, input, conj, codes, scales, rows, K, nblkp, s);
+    if (rounding == MX_ROUND_NEAREST_EVEN)
+        return quantize_fmt_rnd<FMT, MX_ROUND_NEAREST_EVEN, ROWS>(X, X2, input, conj, codes, scales, rows, K, nblkp,
+                                                                  s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -264,55 +320,65 @@ int mx_padded_blocks(int K) {
     return (nblk + MX_BLK_PER_STAGE - 1) / MX_BLK_PER_STAGE * MX_BLK_PER_STAGE;
 }
 
-hipError_t launch_mx_quantize_ex(const void* X, const void* X2, int input, int fmt, bool conj,
+hipError_t launch_mx_quantize_ex(const void* X, const void* X2, int input, int fmt, int rounding, bool conj,
                                  uint8_t* codes, uint8_t* scales, int64_t rows, int K, hipStream_t s) {
     if (rows <= 0) return hipSuccess;
     const int nblkp = mx_padded_blocks(K);
     if (nblkp == 0) return hipSuccess;
     if (fmt == MX_FP4_E2M1)
-        return quantize_fmt<MX_FP4_E2M1>(X, X2, input, conj, codes, scales, rows, K, nblkp, s);
+        return quantize_fmt<MX_FP4_E2M1>(X, X2, input, rounding, conj, codes, scales, rows, K, nblkp, s);
     if (fmt == MX_FP6_E3M2)
-        return quantize_fmt<MX_FP6_E3M2>(X, X2, input, conj, codes, scales, rows, K, nblkp, s);
+        return quantize_fmt<MX_FP6_E3M2>(X, X2, input, rounding, conj, codes, scales, rows, K, nblkp, s);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_mx_quantize_rows(const void* X, const void* X2, int input, int fmt, bool conj, uint8_t* rows_out,
-                                   int64_t rows, int K, hipStream_t s) {
+hipError_t launch_mx_quantize_rows(const void* X, const void* X2, int input, int fmt, int rounding, bool conj,
+                                   uint8_t* rows_out, int64_t rows, int K, hipStream_t s) {
     if (rows <= 0) return hipSuccess;
     const int nblkp = mx_padded_blocks(K);
     if (nblkp == 0) return hipSuccess;
     if (fmt == MX_FP4_E2M1)
-        return quantize_fmt<MX_FP4_E2M1, true>(X, X2, input, conj, rows_out, nullptr, rows, K, nblkp, s);
+        return quantize_fmt<MX_FP4_E2M1, true>(X, X2, input, rounding, conj, rows_out, nullptr, rows, K, nblkp, s);
     if (fmt == MX_FP6_E3M2)
-        return quantize_fmt<MX_FP6_E3M2, true>(X, X2, input, conj, rows_out, nullptr, rows, K, nblkp, s);
+        return quantize_fmt<MX_FP6_E3M2, true>(X, X2, input, rounding, conj, rows_out, nullptr, rows, K, nblkp, s);
     return hipErrorInvalidValue;
 }
 
 int mx_dense_blocks(int K) { return ((K + 31) / 32 + 3) / 4 * 4; }
 
-hipError_t launch_mx_quantize_dense(const void* X, int input, uint8_t* rows_out, uint8_t* scales, int64_t rows,
-                                    int64_t nrow, int K, hipStream_t s) {
-    if (rows <= 0) return hipSuccess;
-    const int nblkp = mx_dense_blocks(K);
-    if (nblkp == 0 || nrow <= 0 || rows % nrow != 0) return hipErrorInvalidValue;
+template <int RND>
+hipError_t quantize_dense_rnd(const void* X, int input, uint8_t* rows_out, uint8_t* scales, int64_t rows,
+                              int64_t nrow, int K, int nblkp, hipStream_t s) {
     const int64_t total = rows * nblkp;
     const unsigned blocks = (unsigned)((total + 255) / 256);
     const int esz = input == MX_IN_INT4 ? 1 : 4;
     const int vec = ((int64_t)K * esz % 16 == 0 && (reinterpret_cast<uintptr_t>(X) & 15u) == 0) ? 1 : 0;
     if (input == MX_IN_INT4)
-        hipLaunchKernelGGL((mx_quantize_kernel<MX_FP4_E2M1, MX_IN_INT4, true, true>), dim3(blocks), dim3(256), 0, s,
-                           X, nullptr, rows_out, scales, rows, K, nblkp, 0, vec, nrow);
+        hipLaunchKernelGGL((mx_quantize_kernel<MX_FP4_E2M1, MX_IN_INT4, RND, true, true>), dim3(blocks), dim3(256), 0,
+                           s, X, nullptr, rows_out, scales, rows, K, nblkp, 0, vec, nrow);
     else if (input == MX_IN_FP16)
-        hipLaunchKernelGGL((mx_quantize_kernel<MX_FP4_E2M1, MX_IN_FP16, true, true>), dim3(blocks), dim3(256), 0, s,
-                           X, nullptr, rows_out, scales, rows, K, nblkp, 0, vec, nrow);
+        hipLaunchKernelGGL((mx_quantize_kernel<MX_FP4_E2M1, MX_IN_FP16, RND, true, true>), dim3(blocks), dim3(256), 0,
+                           s, X, nullptr, rows_out, scales, rows, K, nblkp, 0, vec, nrow);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-hipError_t launch_mx_quantize(const void* X, int input, int fmt, uint8_t* codes, uint8_t* scales,
+hipError_t launch_mx_quantize_dense(const void* X, int input, int rounding, uint8_t* rows_out, uint8_t* scales,
+                                    int64_t rows, int64_t nrow, int K, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    const int nblkp = mx_dense_blocks(K);
+    if (nblkp == 0 || nrow <= 0 || rows % nrow != 0) return hipErrorInvalidValue;
+    if (rounding == MX_ROUND_REFERENCE)
+        return quantize_dense_rnd<MX_ROUND_REFERENCE>(X, input, rows_out, scales, rows, nrow, K, nblkp, s);
+    if (rounding == MX_ROUND_NEAREST_EVEN)
+        return quantize_dense_rnd<MX_ROUND_NEAREST_EVEN>(X, input, rows_out, scales, rows, nrow, K, nblkp, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_mx_quantize(const void* X, int input, int fmt, int rounding, uint8_t* codes, uint8_t* scales,
                               int64_t rows, int K, hipStream_t s) {
-    return launch_mx_quantize_ex(X, nullptr, input, fmt, false, codes, scales, rows, K, s);
+    return launch_mx_quantize_ex(X, nullptr, input, fmt, rounding, false, codes, scales, rows, K, s);
 }
 
 }  // namespace ftdx

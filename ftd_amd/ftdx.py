@@ -44,6 +44,14 @@ class OutputPrecision(enum.IntEnum):
     FP32 = 1
 
 
+class MxRounding(enum.IntEnum):
+    """Element rounding of the MX (FP6 / FP4) quantizer.  Reference: the reference's
+    converters code for code (gemm_complex_sm100/fp6_kernels.hpp:27-58,
+    fp4_kernels.hpp:7-30), the default.  NearestEven: OCP MX v1.0 rounding."""
+    Reference = 0
+    NearestEven = 1
+
+
 class OutputFormat(enum.IntEnum):
     PackedTriangle = 0
 
@@ -111,6 +119,9 @@ _SIGNATURES = {
     "ftdx_herk_each": (_I, [_VP, _VP, _I, _I, _VP]),
     "ftdx_herk_mx": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _F, _F, _VP]),
     "ftdx_mx_quantize": (_I, [_VP, _I, _I, _VP, _VP, ctypes.c_int64, _I, _VP]),
+    "ftdx_mx_quantize_rounding": (_I, [_VP, _I, _I, _I, _VP, _VP, ctypes.c_int64, _I, _VP]),
+    "ftdx_set_mx_rounding": (_I, [_VP, _I]),
+    "ftdx_get_mx_rounding": (_I, [_VP]),
     "ftdx_mx_layout": (_I, [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ftdx_gain_loss_grad": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
     "ftdx_gain_loss_grad_noalloc": (_I, [_VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
@@ -421,6 +432,13 @@ class CutlassComplexGemm:
     def set_launch_config(self, tile_config: int = -1, xcd_remap: int = -1) -> None:
         _check(lib().ftdx_set_launch_config(self._h, tile_config, xcd_remap), "set_launch_config")
 
+    def set_mx_rounding(self, rounding: MxRounding) -> None:
+        """Element rounding of this context's FP6 / FP4 paths (default: the reference's)."""
+        _check(lib().ftdx_set_mx_rounding(self._h, int(rounding)), "set_mx_rounding")
+
+    def mx_rounding(self) -> MxRounding:
+        return MxRounding(lib().ftdx_get_mx_rounding(self._h))
+
     def graph_stats(self) -> tuple[int, int]:
         """(graphs cached, graphs captured so far) of the HERK graph replay."""
         n, c = ctypes.c_int(), ctypes.c_longlong()
@@ -451,15 +469,15 @@ def mx_layout(compute: ComputePrecision, K: int) -> tuple[int, int]:
 
 
 def mx_quantize(X, input: InputPrecision, compute: ComputePrecision, rows: int, K: int,
-                stream=None):
+                stream=None, rounding: MxRounding = MxRounding.Reference):
     """MX quantization stage on the device: returns (codes, scales) uint8 tensors of
     shapes [rows, blocks, 2, code_bytes] and [rows, blocks, 2]."""
     import torch
     blocks, cb = mx_layout(compute, K)
     codes = torch.zeros((rows, blocks, 2, cb), dtype=torch.uint8, device=X.device)
     scales = torch.zeros((rows, blocks, 2), dtype=torch.uint8, device=X.device)
-    _check(lib().ftdx_mx_quantize(_ptr(X), int(input), int(compute), _ptr(codes), _ptr(scales),
-                                  rows, K, _stream_ptr(stream)), "mx_quantize")
+    _check(lib().ftdx_mx_quantize_rounding(_ptr(X), int(input), int(compute), int(rounding), _ptr(codes),
+                                           _ptr(scales), rows, K, _stream_ptr(stream)), "mx_quantize")
     return codes, scales
 
 

@@ -61,6 +61,15 @@ extern "C" {
 #define FTDX_HERK_FORCE_DIRECT 1   /* HerkMode::ForceDirect : fused single-kernel path */
 #define FTDX_HERK_FORCE_BASELINE 2 /* HerkMode::ForceBaseline : unfused promote + HERK path */
 
+/* Element rounding of the MX (FP6 / FP4) quantizer; the E8M0 block scale is the same
+ * for both.  REFERENCE reproduces the reference's converters code for code
+ * (cutlass_interface/gemm_complex_sm100/fp6_kernels.hpp:27-58, fp4_kernels.hpp:7-30:
+ * clamp, |x| < 0.25 to signed zero, E3M2 mantissa rounded without carry, E2M1
+ * mantissa truncated) and is the default.  NEAREST_EVEN (no reference counterpart)
+ * is the OCP MX v1.0 rule: nearest even with subnormals, saturating. */
+#define FTDX_MX_ROUND_REFERENCE 0
+#define FTDX_MX_ROUND_NEAREST_EVEN 1
+
 #define FTDX_DIRECT_TILE_AUTO 0    /* DirectTileConfig values (cutlass_gemm_api.h:113-119) */
 #define FTDX_DIRECT_TILE_K32_B3 1
 #define FTDX_DIRECT_TILE_K64_B2 2
@@ -130,6 +139,12 @@ int ftdx_set_direct_herk_graph(ftdx_context* ctx, int enable);
  * graph only if its shape, precisions, scalars, pointers and launch configuration
  * (tile configuration and workgroup order) all match the captured call. */
 int ftdx_herk_graph_stats(const ftdx_context* ctx, int* cached, long long* captures);
+/* MX element rounding for this context's FP6 / FP4 paths (HERK, prepared and one-shot
+ * GEMM): FTDX_MX_ROUND_REFERENCE (default) or FTDX_MX_ROUND_NEAREST_EVEN.  No
+ * reference counterpart (the reference has only its own converters).  The getter
+ * returns -1 for a NULL context. */
+int ftdx_set_mx_rounding(ftdx_context* ctx, int rounding);
+int ftdx_get_mx_rounding(const ftdx_context* ctx);
 int ftdx_set_tune_cache_path(ftdx_context* ctx, const char* path);
 int ftdx_set_strategy_tune_verbosity(ftdx_context* ctx, int level);
 int ftdx_set_kernel_tune_verbosity(ftdx_context* ctx, int level);
@@ -180,8 +195,9 @@ int ftdx_herk_each(ftdx_context* ctx, const ftdx_herk_problem* problems, int cou
  * (FTDX_INPUT_INT4) or interleaved complex FP16 [batch][N][K][2]
  * (FTDX_INPUT_FP16).  Re and Im are quantized separately per (row, 32-sample
  * block) with an E8M0 scale 2^ceil(log2(amax / max)) (max 28 for FP6 E3M2,
- * 6 for FP4 E2M1; mxfp_kernels.hpp:147-166) and nearest-even element
- * conversion, then multiplied on scaled FP6/FP4 MFMA with FP32 accumulation.
+ * 6 for FP4 E2M1; mxfp_kernels.hpp:147-166) and the context's element rounding
+ * (ftdx_set_mx_rounding; the reference's converters by default), then
+ * multiplied on scaled FP6/FP4 MFMA with FP32 accumulation.
  * Unlike the reference (FP16 output only) both FP16 and FP32 output work here;
  * ftdx_herk keeps the reference's rule and rejects FP32 output with FP6/FP4.
  * The context owns the quantized scratch (grown on demand, freed by
@@ -190,12 +206,16 @@ int ftdx_herk_mx(ftdx_context* ctx, const void* A, void* C, int N, int K, int ba
                  int input, int compute, int output, float alpha, float beta, void* stream);
 
 /* The MX quantization stage alone (fused_mxfp_preprocess_kernel_sm100,
- * mxfp_kernels.hpp:265-360).  X as for ftdx_herk_mx with `rows` rows of K
- * samples.  codes: rows * blocks * 2 * code_bytes bytes laid out
- * [row][block][Re|Im][code_bytes], scales: rows * blocks * 2 E8M0 bytes
- * [row][block][Re|Im], with blocks and code_bytes from ftdx_mx_layout. */
+ * mxfp_kernels.hpp:265-360), with the reference's element converters.  X as for
+ * ftdx_herk_mx with `rows` rows of K samples.  codes: rows * blocks * 2 *
+ * code_bytes bytes laid out [row][block][Re|Im][code_bytes], scales: rows *
+ * blocks * 2 E8M0 bytes [row][block][Re|Im], with blocks and code_bytes from
+ * ftdx_mx_layout. */
 int ftdx_mx_quantize(const void* X, int input, int compute, void* codes, void* scales,
                      int64_t rows, int K, void* stream);
+/* The same with an explicit element rounding (FTDX_MX_ROUND_*). */
+int ftdx_mx_quantize_rounding(const void* X, int input, int compute, int rounding, void* codes,
+                              void* scales, int64_t rows, int K, void* stream);
 int ftdx_mx_layout(int compute, int K, int* blocks, int* code_bytes);
 
 /* Differentiable gain calibration on packed visibilities (no reference

@@ -11,8 +11,9 @@
  *                         of complex<double>: exact, same values)
  *   cpu_herk_ab           cutlass_interface/test_gemm_api.cu:726-751
  *   FP32/FP16 epilogue    cutlass_interface/shared/herk_kernel_common.hpp:405-420
- *   MX scale / codes      gemm_complex_sm100/mxfp_kernels.hpp:122-176 (scale rule) and
- *                         the OCP MX v1.0 element formats (E3M2, E2M1)
+ *   MX scale / codes      gemm_complex_sm100/mxfp_kernels.hpp:122-176 (scale rule);
+ *                         elements by the reference's converters (fp6_kernels.hpp:27-88,
+ *                         fp4_kernels.hpp:7-30) or by OCP MX v1.0 nearest-even rounding
  *   gain loss / adjoint   FP64 closed forms of the calibration gradients
  *   promote_to_float      tests/xengine_test.cpp:162-167 (GGP two's complement)
  *   cpu_convert_qc_sm_to_ftd  tests/visibility_pipeline_test.cpp:101-112
@@ -339,6 +340,45 @@ uint8_t ftdo_mx_encode(float x, int fmt) {
     return (uint8_t)((sign << mx_sign_shift(fmt)) | code);
 }
 
+/* The reference's element converters, step for step:
+ *   FP6 E3M2  cutlass_interface/gemm_complex_sm100/fp6_kernels.hpp:27-58
+ *   FP6 E2M3  fp6_kernels.hpp:64-88
+ *   FP4 E2M1  cutlass_interface/gemm_complex_sm100/fp4_kernels.hpp:7-30
+ * Clamp to +-max with fminf/fmaxf (a NaN becomes -max), magnitudes below 0.25
+ * (E2M3: 0.0625) become a zero carrying the sign of x < 0, otherwise the
+ * frexp exponent is clamped to the normal range and the mantissa is
+ * rint(frac * 2^mbits) clamped to 2^mbits - 1 (no carry into the exponent);
+ * E2M1 takes its one mantissa bit as frac >= 0.5 (truncation). */
+uint8_t ftdo_mx_encode_ref(float x, int fmt) {
+    int mbits, bias;
+    float maxv;
+    mx_params(fmt, &mbits, &bias, &maxv);
+    const int s = mx_sign_shift(fmt);
+    const int emax_biased = (fmt == 1) ? 7 : 3;
+    const float flush = (fmt == 2) ? 0.0625f : 0.25f;
+    const float v = fminf(fmaxf(x, -maxv), maxv);
+    const unsigned sign = (v < 0.0f) ? 1u : 0u;
+    const float a = fabsf(v);
+    if (a < flush) return (uint8_t)(sign << s);
+    int e = 0;
+    float frac = frexpf(a, &e);
+    frac *= 2.0f;
+    e -= 1;
+    int eb = e + bias;
+    if (eb < 1) eb = 1;
+    if (eb > emax_biased) eb = emax_biased;
+    const float mf = frac - 1.0f;
+    int m;
+    if (fmt == 0) {
+        m = (mf >= 0.5f) ? 1 : 0;
+    } else {
+        m = (int)rintf(mf * (float)(1 << mbits));
+        if (m > (1 << mbits) - 1) m = (1 << mbits) - 1;
+        if (m < 0) m = 0;
+    }
+    return (uint8_t)((sign << s) | ((unsigned)eb << mbits) | (unsigned)m);
+}
+
 double ftdo_mx_decode(uint8_t code, int fmt) {
     int mbits, bias;
     float maxv;
@@ -368,6 +408,11 @@ uint8_t ftdo_mx_scale_exponent(float amax, int fmt) {
 }
 
 void ftdo_mx_quantize(const float* X, int rows, int K, int fmt, uint8_t* codes, uint8_t* scales) {
+    ftdo_mx_quantize_mode(X, rows, K, fmt, 1, codes, scales);
+}
+
+void ftdo_mx_quantize_mode(const float* X, int rows, int K, int fmt, int rounding, uint8_t* codes,
+                           uint8_t* scales) {
     const int nblk = (K + 31) / 32;
     for (int r = 0; r < rows; ++r)
         for (int b = 0; b < nblk; ++b)
@@ -387,7 +432,7 @@ void ftdo_mx_quantize(const float* X, int rows, int K, int fmt, uint8_t* codes, 
                     const int k = b * 32 + i;
                     const float v = (k < K) ? X[((int64_t)r * K + k) * 2 + part] : 0.0f;
                     codes[(((int64_t)r * nblk + b) * 2 + part) * 32 + i] =
-                        ftdo_mx_encode(v * inv, fmt);
+                        rounding == 0 ? ftdo_mx_encode_ref(v * inv, fmt) : ftdo_mx_encode(v * inv, fmt);
                 }
             }
 }

@@ -70,11 +70,20 @@ float    ftdo_f16_to_f32(uint16_t h);
  * saturating at +-max (OCP Microscaling Formats v1.0 conversion).
  * Codes are returned one per byte (sign in the format's top bit). */
 uint8_t ftdo_mx_encode(float x, int fmt);
+/* The reference's own converters (fp6_kernels.hpp:27-88, fp4_kernels.hpp:7-30):
+ * clamp to +-max (NaN -> -max), |x| below 0.25 (E2M3: 0.0625) -> signed zero,
+ * exponent clamped to the normal range, mantissa rint(frac * 2^mbits) clamped
+ * to 2^mbits - 1 without carry (E2M1: one bit, frac >= 0.5). */
+uint8_t ftdo_mx_encode_ref(float x, int fmt);
 double  ftdo_mx_decode(uint8_t code, int fmt);
 uint8_t ftdo_mx_scale_exponent(float amax, int fmt);
 /* X: complex samples as float pairs [rows][K][2].  codes: [rows][nblk][2][32],
  * scales: [rows][nblk][2] with nblk = ceil(K/32) (tail samples quantize as 0). */
 void ftdo_mx_quantize(const float* X, int rows, int K, int fmt, uint8_t* codes, uint8_t* scales);
+/* rounding: 0 = the reference's converters (ftdo_mx_encode_ref), 1 = nearest even
+ * with subnormals (ftdo_mx_encode, what ftdo_mx_quantize uses). */
+void ftdo_mx_quantize_mode(const float* X, int rows, int K, int fmt, int rounding, uint8_t* codes,
+                           uint8_t* scales);
 /* FP64 HERK of complex double input [batch][N][K][2], packed lower triangle
  * [batch][N(N+1)/2][2], Im(diag) = 0. */
 void ftdo_herk_c64(const double* X, double* C, int N, int K, int batch, int threads);

@@ -74,6 +74,11 @@ def lib():
         L.ftdo_mx_quantize.restype = None
         L.ftdo_mx_quantize.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, u8p, u8p]
+        L.ftdo_mx_encode_ref.restype = ctypes.c_uint8
+        L.ftdo_mx_encode_ref.argtypes = [ctypes.c_float, ctypes.c_int]
+        L.ftdo_mx_quantize_mode.restype = None
+        L.ftdo_mx_quantize_mode.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, u8p, u8p]
         dp = ctypes.POINTER(ctypes.c_double)
         L.ftdo_gain_loss_grad_f64.restype = None
         L.ftdo_gain_loss_grad_f64.argtypes = [dp, dp, dp, ctypes.c_int, ctypes.c_int, dp, dp, dp]
@@ -211,8 +216,17 @@ def f32_to_f16_rne(x: float) -> int:
 MX_FP4_E2M1, MX_FP6_E3M2, MX_FP6_E2M3 = 0, 1, 2
 
 
+# element rounding: the reference's converters (fp6_kernels.hpp:27-88,
+# fp4_kernels.hpp:7-30) or OCP nearest-even with subnormals
+MX_ROUND_REFERENCE, MX_ROUND_NEAREST_EVEN = 0, 1
+
+
 def mx_encode(x: float, fmt: int) -> int:
     return int(lib().ftdo_mx_encode(x, fmt))
+
+
+def mx_encode_ref(x: float, fmt: int) -> int:
+    return int(lib().ftdo_mx_encode_ref(x, fmt))
 
 
 def mx_decode(code: int, fmt: int) -> float:
@@ -223,7 +237,7 @@ def mx_scale_exponent(amax: float, fmt: int) -> int:
     return int(lib().ftdo_mx_scale_exponent(amax, fmt))
 
 
-def mx_quantize(X: np.ndarray, fmt: int):
+def mx_quantize(X: np.ndarray, fmt: int, rounding: int = MX_ROUND_NEAREST_EVEN):
     """X: float32 complex pairs [rows, K, 2].  Returns (codes [rows, nblk, 2, 32] uint8,
     scales [rows, nblk, 2] uint8)."""
     X = np.ascontiguousarray(X, dtype=np.float32)
@@ -231,8 +245,8 @@ def mx_quantize(X: np.ndarray, fmt: int):
     nblk = (K + 31) // 32
     codes = np.zeros((rows, nblk, 2, 32), dtype=np.uint8)
     scales = np.zeros((rows, nblk, 2), dtype=np.uint8)
-    lib().ftdo_mx_quantize(_ptr(X, ctypes.c_float), rows, K, fmt, _ptr(codes, ctypes.c_uint8),
-                           _ptr(scales, ctypes.c_uint8))
+    lib().ftdo_mx_quantize_mode(_ptr(X, ctypes.c_float), rows, K, fmt, rounding,
+                                _ptr(codes, ctypes.c_uint8), _ptr(scales, ctypes.c_uint8))
     return codes, scales
 
 
@@ -394,12 +408,14 @@ def fp16_to_e4m3_values(w16: np.ndarray) -> np.ndarray:
 def voltage_weights(weights: np.ndarray, n_ch: int, n_beam: int, n_ant: int, compute: str) -> np.ndarray:
     """FP32 [n_ch, n_beam, n_ant, 2] weights as the GEMM sees them (complex128): FP32 ->
     FP16 nearest-even (deinterleave_fp32_to_fp16_kernel, lib/voltage_pipeline.cu:484-498),
-    then E4M3 (compute 'fp8') or MX E3M2 with 32-sample scales along antennas ('fp6')."""
+    then E4M3 (compute 'fp8') or MX E3M2 with 32-sample scales along antennas ('fp6', the
+    reference's element converter)."""
     w16 = np.asarray(weights, dtype=np.float32).reshape(n_ch, n_beam, n_ant, 2).astype(np.float16)
     if compute == "fp8":
         q = fp16_to_e4m3_values(w16)
         return q[..., 0] + 1j * q[..., 1]
-    codes, scales = mx_quantize(w16.astype(np.float32).reshape(n_ch * n_beam, n_ant, 2), MX_FP6_E3M2)
+    codes, scales = mx_quantize(w16.astype(np.float32).reshape(n_ch * n_beam, n_ant, 2), MX_FP6_E3M2,
+                                MX_ROUND_REFERENCE)
     return mx_dequantize(codes, scales, MX_FP6_E3M2, n_ant).reshape(n_ch, n_beam, n_ant)
 
 

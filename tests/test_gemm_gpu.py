@@ -208,21 +208,24 @@ def _gauss_planes(rng, shape, scale):
             (rng.standard_normal(shape) * scale).astype(np.float16))
 
 
-def _mx_dequant(oracle, re, im, fmt):
-    """FP64 values the MFMA sees after MX quantization of [batch, rows, K] planes."""
+def _mx_dequant(oracle, re, im, fmt, rounding=0):
+    """FP64 values the MFMA sees after MX quantization of [batch, rows, K] planes (the
+    contexts' default element rounding: the reference's converters)."""
     Bt, R, K = re.shape
     X = np.stack([re, im], axis=-1).astype(np.float32).reshape(Bt * R, K, 2)
-    codes, scales = oracle.mx_quantize(X, fmt)
+    codes, scales = oracle.mx_quantize(X, fmt, rounding)
     return oracle.mx_dequantize(codes, scales, fmt, K).reshape(Bt, R, K)
 
 
 @pytest.mark.parametrize("compute", ["FP6", "FP4"])
 @pytest.mark.parametrize("output", ["fp32", "fp16"])
 @pytest.mark.parametrize("M,N,K,B", [(96, 130, 200, 2), (256, 256, 256, 1)])
-def test_mx_gemm_prepared_matches_fp64_of_quantized(oracle, dev, compute, output, M, N, K, B):
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_mx_gemm_prepared_matches_fp64_of_quantized(oracle, dev, compute, output, M, N, K, B, rounding):
     """FP6 / FP4 prepare_b + gemm_prepared (block-scaled, cutlass_gemm_api.h:40-47, 215-299):
     the product equals the FP64 GEMM of the MX-dequantized A and B, to the MFMA's
-    accumulation (same bounds as the MX HERK), with alpha and beta applied."""
+    accumulation (same bounds as the MX HERK), with alpha and beta applied; both element
+    roundings (0: the reference's converters, 1: nearest even)."""
     import torch
     from ftd_amd import ftdx
     fmt = 0 if compute == "FP4" else 1
@@ -236,13 +239,14 @@ def test_mx_gemm_prepared_matches_fp64_of_quantized(oracle, dev, compute, output
     C_re = torch.from_numpy(old_re).to(dev).to(dt)
     C_im = torch.from_numpy(old_im).to(dev).to(dt)
     g = ftdx.CutlassComplexGemm()
+    g.set_mx_rounding(ftdx.MxRounding(rounding))
     g.prepare_b(dB_re, dB_im, N, K, B, compute=getattr(ftdx.ComputePrecision, compute))
     op = ftdx.OutputPrecision.FP32 if output == "fp32" else ftdx.OutputPrecision.FP16
-    print(f"mx gemm {compute} {output} M={M} N={N} K={K} B={B}", flush=True)
+    print(f"mx gemm {compute} {output} M={M} N={N} K={K} B={B} rounding={rounding}", flush=True)
     assert g.gemm_prepared(dA_re, dA_im, C_re, C_im, M, N, K, B, output=op, alpha=0.5, beta=2.0) == 0
     torch.cuda.synchronize()
-    Aq = _mx_dequant(oracle, a_re, a_im, fmt)
-    Bq = _mx_dequant(oracle, b_re, b_im, fmt)
+    Aq = _mx_dequant(oracle, a_re, a_im, fmt, rounding)
+    Bq = _mx_dequant(oracle, b_re, b_im, fmt, rounding)
     ref = 0.5 * np.einsum("bmk,bnk->bmn", Aq, Bq) + 2.0 * (old_re + 1j * old_im)
     got = C_re.double().cpu().numpy() + 1j * C_im.double().cpu().numpy()
     scale = np.abs(ref).max()

@@ -1,7 +1,10 @@
 """GPU tests of the MX block-scaled HERK (FP6 E3M2 / FP4 E2M1, 32-sample E8M0
 scales along K, FP32 accumulate).
 
-- The device quantizer's codes and scales equal the oracle's, byte for byte.
+- The device quantizer's codes and scales equal the oracle's, byte for byte, under
+  both element roundings: the reference's converters (the default,
+  gemm_complex_sm100/fp6_kernels.hpp:27-58, fp4_kernels.hpp:7-30) and OCP nearest
+  even.
 - The MX HERK equals the FP64 HERK of the *dequantized* input to the MFMA's
   accumulation precision (FP32 rounding for FP4; ~1e-4 of the largest entry for
   FP6 E3M2, whose wide product range exceeds the MFMA's internal sum width).
@@ -48,13 +51,35 @@ def _pack(codes, fmt, nblkp):
 
 
 @pytest.mark.parametrize("compute", ["FP4", "FP6"])
-@pytest.mark.parametrize("inp", ["FP16", "INT4"])
-def test_quantizer_bytes_match_oracle(oracle, dev, compute, inp):
+@pytest.mark.parametrize("inp", ["FP16", "INT4", "FP16-edges"])
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_quantizer_bytes_match_oracle(oracle, dev, compute, inp, rounding):
+    """FP16-edges: Gaussian rows with one block per row holding the format maximum
+    (scale 1, so planted values reach the converter unscaled) and values at the
+    converters' edges: the flush threshold, rounding ties, the no-carry region, -0; and,
+    for the reference rounding, a NaN (the reference's fmaxf clamp makes it -max)."""
     import torch
     from ftd_amd import ftdx
     fmt = 0 if compute == "FP4" else 1
     rows, K = 37, 200
-    if inp == "FP16":
+    if inp == "FP16-edges":
+        X16 = _voltages(13, rows, K, outlier_frac=0.0)
+        rng = np.random.default_rng(14)
+        maxv = 6.0 if fmt == 0 else 28.0
+        edges = np.array([0.0, -0.0, 0.0625, 0.24, 0.25, 0.26, 0.5, 0.75, 1.125, 1.375, 1.9, 1.99,
+                          2.5, 3.5, 3.9, 5.9, 6.0, 14.5, 27.9, 28.0], dtype=np.float16)
+        edges = edges[np.abs(edges) <= maxv]
+        for r in range(rows):
+            b = r % (K // 32)
+            X16[r, b * 32, 0] = maxv
+            X16[r, b * 32 + 1:b * 32 + 1 + edges.size, 0] = edges * rng.choice([-1, 1], edges.size).astype(np.float16)
+            X16[r, b * 32 + 1:b * 32 + 1 + edges.size, 1] = -edges
+        if rounding == 0:
+            X16[3, 100, 1] = np.float16(np.nan)
+        X = torch.from_numpy(X16).to(dev)
+        Xf = X16.astype(np.float32)
+        inp = "FP16"
+    elif inp == "FP16":
         X16 = _voltages(11, rows, K)
         X = torch.from_numpy(X16).to(dev)
         Xf = X16.astype(np.float32)
@@ -64,9 +89,10 @@ def test_quantizer_bytes_match_oracle(oracle, dev, compute, inp):
         nib = np.array([0, 1, 2, 3, 4, 5, 6, 7, 0, -1, -2, -3, -4, -5, -6, -7], dtype=np.float32)
         Xf = np.stack([nib[A >> 4], nib[A & 15]], axis=-1).reshape(rows, K, 2)
     codes, scales = ftdx.mx_quantize(X, getattr(ftdx.InputPrecision, inp),
-                                     getattr(ftdx.ComputePrecision, compute), rows, K)
+                                     getattr(ftdx.ComputePrecision, compute), rows, K,
+                                     rounding=ftdx.MxRounding(rounding))
     torch.cuda.synchronize()
-    ref_codes, ref_scales = oracle.mx_quantize(Xf, fmt)
+    ref_codes, ref_scales = oracle.mx_quantize(Xf, fmt, rounding)
     nblkp = codes.shape[1]
     assert np.array_equal(scales.cpu().numpy()[:, :ref_scales.shape[1]], ref_scales)
     assert not np.any(scales.cpu().numpy()[:, ref_scales.shape[1]:])
@@ -76,17 +102,19 @@ def test_quantizer_bytes_match_oracle(oracle, dev, compute, inp):
 @pytest.mark.parametrize("compute", ["FP4", "FP6"])
 @pytest.mark.parametrize("output", ["fp32", "fp16"])
 @pytest.mark.parametrize("N,K,B", [(200, 300, 2), (129, 64, 1), (33, 1000, 1), (256, 300, 2), (512, 1000, 1)])
-def test_mx_herk_matches_fp64_of_quantized(oracle, dev, compute, output, N, K, B):
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_mx_herk_matches_fp64_of_quantized(oracle, dev, compute, output, N, K, B, rounding):
     import torch
     from ftd_amd import ftdx
     fmt = 0 if compute == "FP4" else 1
     X16 = _voltages(21 + N, B * N, K)
     ctx = ftdx.CutlassComplexGemm()
+    ctx.set_mx_rounding(ftdx.MxRounding(rounding))
     C = ftdx.herk_mx(ctx, torch.from_numpy(X16).to(dev), N, K, B, ftdx.InputPrecision.FP16,
                      getattr(ftdx.ComputePrecision, compute), output=output)
     torch.cuda.synchronize()
     got = C.cpu().numpy().astype(np.float64)
-    codes, scales = oracle.mx_quantize(X16.astype(np.float32), fmt)
+    codes, scales = oracle.mx_quantize(X16.astype(np.float32), fmt, rounding)
     Xq = oracle.mx_dequantize(codes, scales, fmt, K).reshape(B, N, K)
     ref = oracle.herk_c64(Xq, N, K, B)
     scale = np.abs(ref).max()
@@ -115,7 +143,7 @@ def test_mx_herk_every_tile_config(oracle, dev, compute, cfg, N):
     C = ftdx.herk_mx(ctx, torch.from_numpy(X16).to(dev), N, K, B, ftdx.InputPrecision.FP16,
                      getattr(ftdx.ComputePrecision, compute), output="fp32")
     torch.cuda.synchronize()
-    codes, scales = oracle.mx_quantize(X16.astype(np.float32), fmt)
+    codes, scales = oracle.mx_quantize(X16.astype(np.float32), fmt, oracle.MX_ROUND_REFERENCE)
     ref = oracle.herk_c64(oracle.mx_dequantize(codes, scales, fmt, K).reshape(B, N, K), N, K, B)
     tol = 2e-6 if compute == "FP4" else 5e-4
     got = C.cpu().numpy().astype(np.float64)
@@ -257,3 +285,86 @@ def test_mx_fp6_rows_past_4gib(oracle, dev):
     del A, C
     ctx.end_herk()
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("rounding", [0, 1])
+@pytest.mark.parametrize("N", [130, 256])
+def test_mx_fp4_int4_bit_exact(oracle, dev, rounding, N):
+    """INT4 voltages in FP4 E2M1: every block scale is 2^k with k in -2..1 and every
+    dequantized value an integer, so the products sum exactly in FP32 and the MX HERK is
+    bit-exact against the FP64 HERK of the oracle's dequantized values; under the
+    reference rounding a block whose largest magnitude is 7 maps 1 -> 2 and 7 -> 6."""
+    import torch
+    from ftd_amd import ftdx
+    K, B = 320, 2
+    A = oracle.gen_qc_uniform(41 + N, N * K * B)
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_mx_rounding(ftdx.MxRounding(rounding))
+    C = ftdx.herk_mx(ctx, torch.from_numpy(A).to(dev), N, K, B, ftdx.InputPrecision.INT4,
+                     ftdx.ComputePrecision.FP4, output="fp32")
+    torch.cuda.synchronize()
+    nib = np.array([0, 1, 2, 3, 4, 5, 6, 7, 0, -1, -2, -3, -4, -5, -6, -7], dtype=np.float32)
+    Xf = np.stack([nib[A >> 4], nib[A & 15]], axis=-1).reshape(B * N, K, 2)
+    codes, scales = oracle.mx_quantize(Xf, 0, rounding)
+    Xq = oracle.mx_dequantize(codes, scales, 0, K)
+    assert np.array_equal(Xq, np.round(Xq))
+    ref = oracle.herk_c64(Xq.reshape(B, N, K), N, K, B).astype(np.float32)
+    assert np.array_equal(C.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def _reference_study_input(N, batch, K_values):
+    """The inputs of the reference's precision study (test_precision_accuracy.cu:205-215):
+    std::mt19937(42), uniform_real_distribution<float>(-0.5, 0.5), fresh interleaved FP16
+    data [batch][N][K][2] for each K in turn from the one generator.  libstdc++ draws one
+    32-bit word per float: r = float(u) / 2^32 (1 -> the float below 1), x = 1 * r - 0.5
+    in float, then FP16 nearest even."""
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(42)
+    out = {}
+    for K in K_values:
+        n = 2 * N * K * batch
+        u = bg.random_raw(n).astype(np.uint32)
+        r = u.astype(np.float32) / np.float32(2.0 ** 32)
+        r = np.where(r >= 1, np.nextafter(np.float32(1), np.float32(0)), r).astype(np.float32)
+        out[K] = (r + np.float32(-0.5)).astype(np.float32).astype(np.float16)
+    return out
+
+
+def test_reference_study_input_generator():
+    """First draws of std::mt19937(42) through uniform_real_distribution<float>(-0.5, 0.5)
+    as libstdc++ produces them (known answers)."""
+    x = _reference_study_input(1, 1, [16])[16][:5].astype(np.float32)
+    want = np.array([-0.12545988, 0.296543002, 0.45071429, -0.316565216, 0.231993914], dtype=np.float32)
+    assert np.array_equal(x, want.astype(np.float16).astype(np.float32))
+
+
+def test_reference_precision_study(oracle, dev):
+    """The reference's precision sweep (cutlass_interface/test_precision_accuracy.cu:172-357)
+    on its own inputs: N = 512, batch 4, FP16 input, FP16 output, each precision's
+    peak-relative error max|err| / max|ref| against the FP64 HERK of the FP16 values must
+    stay under the reference's thresholds (FP8 15%, FP6 25%, FP4 55%; :149-160).  MX runs
+    the reference's element converters (the default).  The K sweep is the reference's,
+    every K's data drawn in turn so each K sees the reference's bytes; K = 16 .. 4096 are
+    run at 256, 1024 and 4096."""
+    import torch
+    from ftd_amd import ftdx
+    N, batch = 512, 4
+    sweep = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+    data = _reference_study_input(N, batch, sweep)
+    g = ftdx.CutlassComplexGemm()
+    g.set_strategy_tune_verbosity(0)
+    thresholds = {"FP8": 0.15, "FP6": 0.25, "FP4": 0.55}
+    for K in (256, 1024, 4096):
+        X16 = data[K].reshape(batch, N, K, 2)
+        X = (X16[..., 0].astype(np.float64) + 1j * X16[..., 1].astype(np.float64))
+        ref = oracle.herk_c64(X, N, K, batch)
+        max_ref = np.abs(ref).max()
+        dX = torch.from_numpy(X16.reshape(-1)).to(dev)
+        for prec, thr in thresholds.items():
+            C = torch.zeros((batch, N * (N + 1) // 2, 2), dtype=torch.float16, device=dev)
+            assert g.herk(dX, C, N, K, batch, ftdx.InputPrecision.FP16, getattr(ftdx.ComputePrecision, prec),
+                          ftdx.OutputPrecision.FP16, tune=False) == 0
+            torch.cuda.synchronize()
+            err = np.abs(C.cpu().numpy().astype(np.float64) - ref).max()
+            print(f"K={K} {prec}: max_abs {err:.4e} peak-relative {err / max_ref:.4e}", flush=True)
+            assert err / max_ref < thr

@@ -116,3 +116,98 @@ def test_herk_c64_vs_numpy(oracle):
         im = G[il, jl].imag.copy()
         im[il == jl] = 0
         assert np.allclose(C[b, :, 1], im, rtol=1e-13, atol=1e-12)
+
+
+# ---- the reference's own element converters (oracle.mx_encode_ref) ----
+# Known answers worked by hand from the reference's arithmetic
+# (cutlass_interface/gemm_complex_sm100/fp6_kernels.hpp:27-58, fp4_kernels.hpp:7-30):
+# clamp, flush below 0.25 to a signed zero, frexp exponent clamped to the normal
+# range, E3M2 mantissa rint(4 * (frac - 1)) clamped to 3 (no carry), E2M1 mantissa
+# bit frac - 1 >= 0.5 (truncation).  Each value is (x, expected code).
+REF_E3M2_CASES = [
+    (0.0, 0x00), (-0.0, 0x00),            # -0.0 < 0 is false: no sign bit
+    (0.2, 0x00), (-0.2, 0x20),            # flushed, sign of x < 0 kept
+    (0.0625, 0x00), (0.24, 0x00),         # OCP subnormals: flushed here
+    (0.25, 0x04), (0.3, 0x05),            # 0.3 = 1.2 * 2^-2 -> rint(0.8) = 1 -> 0.3125
+    (0.475, 0x07),                        # 1.9 * 2^-2 -> rint(3.6) = 4 -> 3: 0.4375, not 0.5
+    (1.125, 0x0C), (1.375, 0x0E),         # ties: rint(0.5) = 0, rint(1.5) = 2
+    (1.9, 0x0F), (3.9, 0x13),             # no carry: 1.75, 3.5
+    (27.9, 0x1F), (28.0, 0x1F), (30.0, 0x1F), (1e30, 0x1F), (-1e30, 0x3F),
+    (float("nan"), 0x3F),                 # fmaxf(NaN, -28) = -28
+]
+REF_E2M1_CASES = [
+    (0.0, 0x0), (0.2, 0x0), (-0.2, 0x8),
+    (0.25, 0x2), (0.3, 0x2), (0.4, 0x3),  # [0.25, 1): exponent clamped up to 1.0 / 1.5
+    (0.5, 0x2), (0.75, 0x3),              # 0.5 -> 1.0 (OCP keeps 0.5)
+    (1.0, 0x2), (1.4, 0x2), (1.5, 0x3), (1.99, 0x3),   # truncation within the binade
+    (2.5, 0x4), (3.5, 0x5), (5.9, 0x6),   # 2.5 -> 2, 3.5 -> 3 (OCP: 4), 5.9 -> 4
+    (6.0, 0x7), (7.0, 0x7), (-7.0, 0xF), (float("nan"), 0xF),
+]
+
+
+@pytest.mark.parametrize("fmt,cases", [(1, REF_E3M2_CASES), (0, REF_E2M1_CASES)])
+def test_ref_encode_known_answers(oracle, fmt, cases):
+    for x, want in cases:
+        assert oracle.mx_encode_ref(x, fmt) == want, (x, hex(oracle.mx_encode_ref(x, fmt)), hex(want))
+
+
+def _ref_value_by_search(x: float, fmt: int) -> float:
+    """The reference converter's value found by searching the format's grid, not by
+    its bit arithmetic: E3M2 nearest of the four values of x's binade (ties to the
+    even mantissa, 2^(e+1) not a candidate); E2M1 the largest grid value <= x from
+    1.0 up, and 1.0 / 1.5 by the first significand bit below 1."""
+    if math.isnan(x):
+        x = -1e30
+    maxv = 28.0 if fmt == 1 else 6.0
+    c = min(abs(x), maxv)
+    if c < 0.25:
+        return 0.0
+    m, e = math.frexp(c)  # c = m * 2^e, m in [0.5, 1)
+    e -= 1
+    if fmt == 1:
+        cands = [2.0 ** e * (1 + j / 4) for j in range(4)]
+        d = [abs(c - v) for v in cands]
+        best = min(d)
+        js = [j for j in range(4) if d[j] == best]
+        return cands[js[0] if len(js) == 1 else [j for j in js if j % 2 == 0][0]]
+    if c < 1.0:
+        return 1.5 if 2 * m >= 1.5 else 1.0
+    return max(v for v in (1.0, 1.5, 2.0, 3.0, 4.0, 6.0) if v <= c)
+
+
+@pytest.mark.parametrize("fmt,table,sbit", [(1, FP6_E3M2, 5), (0, FP4_E2M1, 3)])
+def test_ref_encode_every_fp16_value(oracle, fmt, table, sbit):
+    """All finite FP16 values times the power-of-two factors the block scale applies."""
+    h = np.arange(0, 0x7C00, dtype=np.uint16).view(np.float16).astype(np.float32)
+    rng = np.random.default_rng(11)
+    for f in (1.0, 2.0 ** -3, 2.0 ** 5):
+        xs = np.concatenate([h, -h]) * np.float32(f)
+        for x in rng.choice(xs, 6000, replace=False):
+            x = float(x)
+            code = oracle.mx_encode_ref(x, fmt)
+            want = _ref_value_by_search(x, fmt)
+            assert table[code & ((1 << sbit) - 1)] == want, (x, hex(code), want)
+            assert (code >> sbit) & 1 == (1 if x < 0 else 0), (x, hex(code))
+
+
+def test_ref_quantize_int4_blocks(oracle):
+    """INT4 samples through the reference's quantizer: lossless in FP6 E3M2 (the
+    reference's claim for ComputePrecision::FP6, cutlass_gemm_api.h:45); in FP4 a
+    block whose largest magnitude is 7 (scale 2) maps 1..7 to 2, 2, 3, 4, 4, 6, 6,
+    and one whose largest is 5 (scale 1) maps 1..5 to 1, 2, 3, 4, 4."""
+    A = oracle.gen_qc_uniform(3, 4 * 64)
+    nib = np.array([0, 1, 2, 3, 4, 5, 6, 7, 0, -1, -2, -3, -4, -5, -6, -7], dtype=np.float32)
+    X = np.stack([nib[A >> 4], nib[A & 15]], axis=-1).reshape(4, 64, 2)
+    codes, scales = oracle.mx_quantize(X, 1, oracle.MX_ROUND_REFERENCE)
+    deq = oracle.mx_dequantize(codes, scales, 1, 64)
+    assert np.array_equal(deq, X[..., 0] + 1j * X[..., 1])
+    blk = np.zeros((2, 32, 2), dtype=np.float32)
+    blk[0, :7, 0] = np.arange(1, 8)
+    blk[0, 7:14, 1] = -np.arange(1, 8)
+    blk[1, :5, 0] = np.arange(1, 6)
+    codes, scales = oracle.mx_quantize(blk, 0, oracle.MX_ROUND_REFERENCE)
+    deq = oracle.mx_dequantize(codes, scales, 0, 32)
+    assert list(scales[0, 0]) == [128, 128] and list(scales[1, 0]) == [127, 0]
+    assert list(deq[0, :7].real) == [2, 2, 3, 4, 4, 6, 6]
+    assert list(deq[0, 7:14].imag) == [-2, -2, -3, -4, -4, -6, -6]
+    assert list(deq[1, :5].real) == [1, 2, 3, 4, 4]
