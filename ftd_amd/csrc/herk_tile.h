@@ -290,6 +290,13 @@ struct TileRunner {
     template <bool GEMM = false, bool BF6 = false>
     static __device__ __forceinline__ void load_stage_fp8(const TileJob& j, int k0, bool share_cols,
                                                           int tid, uint4 (&g)[Cfg::SEGS8_PER_THREAD]) {
+        // the operand fields as values: selecting between j.A and j.B (or their pitches)
+        // directly lets the compiler select between the fields' addresses instead, which
+        // keeps the job in private memory
+        const uint8_t* const A = j.A;
+        const uint8_t* const B = GEMM ? j.B : j.A;
+        const int64_t lda = j.lda, ldb = GEMM ? j.ldb : j.lda;
+        const int nrow = j.N, ncol = GEMM ? j.Ncol : j.N;
 #pragma unroll
         for (int q = 0; q < Cfg::SEGS8_PER_THREAD; ++q) {
             const int s = tid + Cfg::THREADS * q;
@@ -298,9 +305,9 @@ struct TileRunner {
             const int grow = is_col ? j.col0 + (lrow - Cfg::TM) : j.row0 + lrow;
             const int kk = k0 + seg * 8;
             const bool from_b = GEMM && is_col;
-            const uint8_t* base = from_b ? j.B : j.A;
-            const int64_t pitch = from_b ? j.ldb : j.lda;
-            const int lim = from_b ? j.Ncol : j.N;
+            const uint8_t* base = from_b ? B : A;
+            const int64_t pitch = from_b ? ldb : lda;
+            const int lim = from_b ? ncol : nrow;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             // pre-cast rows are zero-padded to a multiple of 16 samples; an E3M2 slot's
             // two 16-byte halves both carry all 16 of its samples
