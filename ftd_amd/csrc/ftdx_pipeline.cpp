@@ -16,7 +16,9 @@
 //   staging_out[b] read by the host after D2H of its payload       (host waits)
 // The host copies between caller memory (pageable) and the pinned staging are the
 // driver's bottleneck (one thread copies at 10-15 GB/s, the DMA engines move several
-// times that), so large copies are split over up to 8 threads.
+// times that), so large copies are split over up to 8 threads.  A caller buffer that is
+// itself page-locked (hipHostMalloc, torch pin_memory) skips its staging copy: the DMA
+// engines read or write it directly.
 // All streams use the default priority.
 #include "ftdx.h"
 
@@ -50,6 +52,18 @@ using ftdx::hip_fail;
 using ftdx::host_copy;
 
 namespace {
+
+// True if p is page-locked host memory that the DMA engines can read or write directly.
+// hipPointerGetAttributes rejects pageable pointers; that error is cleared here so the
+// next call does not report it.
+bool pinned_host(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
 
 void release(ftdx_pipeline* pl) {
     if (pl->s_comp) (void)hipStreamSynchronize(pl->s_comp);
@@ -146,27 +160,31 @@ int ftdx_pipeline_correlate_many(ftdx_pipeline* pl, void* const* vis_outputs,
             return fail(FTDX_ERR_INVALID_ARG, "pipeline_correlate: NULL payload pointer");
     hipError_t e;
     int rc = FTDX_OK;
-    // copy-out of payload q from its pinned staging to the caller
+    std::vector<char> direct_out(n_payloads > 0 ? n_payloads : 1, 0);
+    // copy-out of payload q from its pinned staging to the caller (nothing to copy when
+    // the D2H wrote the caller's page-locked buffer itself)
     auto drain = [&](int q) -> int {
         const int b = q & 1;
         hipError_t de = hipEventSynchronize(pl->ev_d2h[b]);
         if (de != hipSuccess) return hip_fail(de, "pipeline_correlate: D2H");
-        host_copy(vis_outputs[q], pl->h_out[b], pl->out_bytes);
+        if (!direct_out[q]) host_copy(vis_outputs[q], pl->h_out[b], pl->out_bytes);
         return FTDX_OK;
     };
     for (int p = 0; p < n_payloads && rc == FTDX_OK; ++p) {
         const int b = p & 1;
+        const bool direct_in = pl->in_bytes > 0 && pinned_host(raw_inputs[p]);
+        direct_out[p] = pinned_host(vis_outputs[p]) ? 1 : 0;
         if (pl->used[b]) {
             // staging_in[b] was last read by the H2D two payloads back
             if ((e = hipEventSynchronize(pl->ev_h2d[b])) != hipSuccess)
                 return hip_fail(e, "pipeline_correlate: H2D");
         }
-        if (pl->in_bytes > 0) host_copy(pl->h_in[b], raw_inputs[p], pl->in_bytes);
+        if (pl->in_bytes > 0 && !direct_in) host_copy(pl->h_in[b], raw_inputs[p], pl->in_bytes);
         if (pl->used[b] && (e = hipStreamWaitEvent(pl->s_h2d, pl->ev_comp[b], 0)) != hipSuccess)
             return hip_fail(e, "pipeline_correlate: wait");
         if (pl->in_bytes > 0 &&
-            (e = hipMemcpyAsync(pl->d_in[b], pl->h_in[b], pl->in_bytes, hipMemcpyHostToDevice,
-                                pl->s_h2d)) != hipSuccess)
+            (e = hipMemcpyAsync(pl->d_in[b], direct_in ? raw_inputs[p] : pl->h_in[b], pl->in_bytes,
+                                hipMemcpyHostToDevice, pl->s_h2d)) != hipSuccess)
             return hip_fail(e, "pipeline_correlate: H2D");
         if ((e = hipEventRecord(pl->ev_h2d[b], pl->s_h2d)) != hipSuccess ||
             (e = hipStreamWaitEvent(pl->s_comp, pl->ev_h2d[b], 0)) != hipSuccess)
@@ -178,8 +196,8 @@ int ftdx_pipeline_correlate_many(ftdx_pipeline* pl, void* const* vis_outputs,
             (e = hipStreamWaitEvent(pl->s_d2h, pl->ev_comp[b], 0)) != hipSuccess)
             return hip_fail(e, "pipeline_correlate: events");
         // staging_out[b] is free: its previous payload (p-2) was drained in iteration p-1
-        if ((e = hipMemcpyAsync(pl->h_out[b], pl->d_out[b], pl->out_bytes, hipMemcpyDeviceToHost,
-                                pl->s_d2h)) != hipSuccess ||
+        if ((e = hipMemcpyAsync(direct_out[p] ? vis_outputs[p] : pl->h_out[b], pl->d_out[b], pl->out_bytes,
+                                hipMemcpyDeviceToHost, pl->s_d2h)) != hipSuccess ||
             (e = hipEventRecord(pl->ev_d2h[b], pl->s_d2h)) != hipSuccess)
             return hip_fail(e, "pipeline_correlate: D2H");
         pl->used[b] = true;

@@ -497,7 +497,9 @@ int ftdx_blas_gemm_fp32(ftdx_context* ctx, const void* A, const void* B, void* C
  * pipeline owns two device buffer pairs, two pinned staging pairs and three
  * default-priority streams; payload p's H2D, HERK and D2H overlap its neighbours'.
  * The HERK uses ctx's settings (mode, launch configuration, graph replay).  Host
- * buffers may be pageable.  Returns when every output has been written. */
+ * buffers may be pageable; a page-locked one (hipHostMalloc) is read or written by
+ * the DMA engines directly, without its staging copy.  Returns when every output has
+ * been written. */
 typedef struct ftdx_pipeline ftdx_pipeline;
 int ftdx_pipeline_create(ftdx_context* ctx, int N, int K, int batch_count, int output,
                          ftdx_pipeline** out);

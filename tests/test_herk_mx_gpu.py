@@ -368,3 +368,44 @@ def test_reference_precision_study(oracle, dev):
             err = np.abs(C.cpu().numpy().astype(np.float64) - ref).max()
             print(f"K={K} {prec}: max_abs {err:.4e} peak-relative {err / max_ref:.4e}", flush=True)
             assert err / max_ref < thr
+
+
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_mx_fp6_one_step_sum_width(oracle, dev, rounding):
+    """Where the FP6 tolerance comes from: one 32-sample block (K = 32, one scaled MFMA step
+    per output) of values that E3M2 holds exactly at scale 1, so quantization is exact.
+      narrow: every value in {16, 20, 24, 28} (sign random): products within a factor of 3
+              of each other, their sums exact in FP32 -- the output equals FP64 bit for bit;
+      wide:   values from 0.25 to 28 in one step: products span 2^13.6, the sums are still
+              multiples of 1/16 below 2^16 (exact in FP32), yet the MFMA's internal sum may
+              round them -- the deviation stays within the 5e-4 x max bound the larger
+              tests use, and is printed."""
+    import torch
+    from ftd_amd import ftdx
+    N, K, B = 32, 32, 1
+    rng = np.random.default_rng(90 + rounding)
+    sign = rng.choice([-1.0, 1.0], size=(N, K, 2))
+    narrow = rng.choice([16.0, 20.0, 24.0, 28.0], size=(N, K, 2)) * sign
+    wide = rng.choice([0.25, 0.3125, 0.4375, 1.5, 3.5, 7.0, 14.0, 28.0], size=(N, K, 2)) * sign
+    for X in (narrow, wide):
+        X[:, 0, :] = 28.0  # every part's block maximum is 28: scale 2^0
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_mx_rounding(ftdx.MxRounding(rounding))
+    errs = {}
+    for name, X in (("narrow", narrow), ("wide", wide)):
+        X16 = X.astype(np.float16)
+        assert np.array_equal(X16.astype(np.float64), X)
+        C = ftdx.herk_mx(ctx, torch.from_numpy(X16.reshape(-1)).to(dev), N, K, B, ftdx.InputPrecision.FP16,
+                         ftdx.ComputePrecision.FP6, output="fp32")
+        torch.cuda.synchronize()
+        codes, scales = oracle.mx_quantize(X16.astype(np.float32).reshape(N, K, 2), 1, rounding)
+        assert np.all(scales[:, 0, :] == 127)
+        Xq = oracle.mx_dequantize(codes, scales, 1, K)
+        assert np.array_equal(Xq, X[..., 0] + 1j * X[..., 1])  # quantization exact
+        ref = oracle.herk_c64(Xq.reshape(B, N, K), N, K, B)
+        got = C.cpu().numpy().astype(np.float64)
+        errs[name] = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"one-step FP6 MX, rounding {rounding}: narrow max err {errs['narrow']:.3e}, "
+          f"wide max err {errs['wide']:.3e} (of the largest output)")
+    assert errs["narrow"] == 0.0
+    assert errs["wide"] <= 5e-4

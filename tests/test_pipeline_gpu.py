@@ -46,3 +46,38 @@ def test_pipeline_uses_context_settings(oracle, dev):
         assert np.array_equal(_bits(outs[p]), _bits(oracle.herk_f32(ins[p], N, K, B)))
     assert pl.correlate_many([]) == []
     pl.close()
+
+
+@pytest.mark.parametrize("pin_in,pin_out", [(True, True), (True, False), (False, True)])
+def test_pipeline_page_locked_buffers(oracle, dev, pin_in, pin_out):
+    """Page-locked payloads and outputs (torch pin_memory, i.e. hipHostMalloc) skip the
+    staging copies; mixed with pageable ones in the same call, every output stays
+    bit-exact against the oracle."""
+    import torch
+    from ftd_amd import ftdx
+    N, K, B, n = 128, 256, 4, 5
+    g = ftdx.CutlassComplexGemm()
+    pl = ftdx.Pipeline(g, N, K, B)
+    src = [oracle.gen_bytes(500 + p, N * K * B) for p in range(n)]
+    ins, outs, keep = [], [], []
+    for p in range(n):
+        # every other payload page-locked, so both paths share the double buffers
+        pinned = p % 2 == 0
+        if pin_in and pinned:
+            t = torch.empty(N * K * B, dtype=torch.uint8, pin_memory=True)
+            t.numpy()[:] = src[p]
+            keep.append(t)
+            ins.append(t.numpy())
+        else:
+            ins.append(src[p])
+        if pin_out and pinned:
+            t = torch.empty(pl.out_shape(), dtype=torch.float32, pin_memory=True)
+            keep.append(t)
+            outs.append(t.numpy())
+        else:
+            outs.append(np.empty(pl.out_shape(), dtype=np.float32))
+    print(f"pipeline page-locked in={pin_in} out={pin_out}", flush=True)
+    pl.correlate_many(ins, outs)
+    for p in range(n):
+        assert np.array_equal(_bits(outs[p]), _bits(oracle.herk_f32(src[p], N, K, B))), p
+    pl.close()
