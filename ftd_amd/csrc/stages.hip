@@ -173,28 +173,6 @@ promote_qc_bf6_planar_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict
     }
 }
 
-// float -> OCP E4M3 byte, nearest-even, inputs already clamped to +-448
-__device__ __forceinline__ uint32_t f32_to_e4m3(float x) {
-    const uint32_t sign = __builtin_signbitf(x) ? 0x80u : 0u;
-    const float a = __builtin_fabsf(x);
-    uint32_t code;
-    int e;
-    __builtin_frexpf(a, &e);
-    e -= 1;
-    if (a == 0.0f || e < -6) {
-        code = (uint32_t)__builtin_rintf(__builtin_ldexpf(a, 9));  // subnormal step 2^-9
-    } else {
-        uint32_t q = (uint32_t)__builtin_rintf(__builtin_ldexpf(a, 3 - e));
-        if (q == 16u) {
-            q = 8u;
-            ++e;
-        }
-        code = ((uint32_t)(e + 7) << 3) | (q - 8u);
-    }
-    if (code > 0x7Eu) code = 0x7Eu;  // 448: largest finite E4M3
-    return sign | code;
-}
-
 __device__ __forceinline__ float half_bits_to_float(uint32_t h) {
     _Float16 v;
     const uint16_t b = (uint16_t)h;
@@ -202,34 +180,57 @@ __device__ __forceinline__ float half_bits_to_float(uint32_t h) {
     return (float)v;
 }
 
-__device__ __forceinline__ uint32_t fp16_to_e4m3_clamped(uint32_t h) {
-    float f = half_bits_to_float(h);
-    f = __builtin_fminf(__builtin_fmaxf(f, -448.0f), 448.0f);
-    return f32_to_e4m3(f);
+// Four FP16 values (two dwords of bit pairs) -> four OCP E4M3 bytes: clamped to +-448 first
+// (a NaN becomes -448, as fmaxf(NaN, -448) in cast_kernels_common.hpp:36-37), then gfx950's
+// v_cvt_pk_fp8_f32 rounds to nearest even, subnormals included.  Checked on every FP16 bit
+// pattern against the CPU restatement.
+__device__ __forceinline__ uint32_t fp16x4_to_e4m3(uint32_t lo, uint32_t hi) {
+    float v[4] = {half_bits_to_float(lo & 0xFFFFu), half_bits_to_float(lo >> 16), half_bits_to_float(hi & 0xFFFFu),
+                  half_bits_to_float(hi >> 16)};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = __builtin_fminf(__builtin_fmaxf(v[e], -448.0f), 448.0f);
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
+    return (uint32_t)w;
 }
 
-// FP16 [rows][K][2] -> pre-cast rows; 4 samples (16 input bytes, 8 output bytes) per thread
+// FP16 [rows][K][2] -> pre-cast rows; 8 samples (32 input bytes, 16 output bytes) per
+// thread, 16-byte loads when the row's 8 samples are whole and 16-byte aligned.
 __global__ void __launch_bounds__(ST_THREADS)
 promote_fp16_fp8_planar_kernel(const uint16_t* __restrict__ in, uint8_t* __restrict__ out,
-                               int64_t rows, int K, int Kp) {
-    const int groups = Kp / 4;
+                               int64_t rows, int K, int Kp, int vec) {
+    const int groups = Kp / 8;
     const int64_t total = rows * groups;
     const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
     for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride) {
         const int64_t r = t / groups;
         const int gi = (int)(t - r * groups);
-        uint32_t re = 0u, im = 0u;
+        const int k0 = gi * 8;
+        uint32_t d[8];  // FP16 pairs of samples k0 .. k0+7: (re | im << 16) per sample
+        if (vec && k0 + 8 <= K) {
+            const uint4* src = reinterpret_cast<const uint4*>(in + (r * K + k0) * 2);
+            const uint4 a = src[0], b = src[1];
+            d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+            d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+        } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int k = gi * 4 + e;
-            if (k < K) {
-                const uint16_t* p = in + (r * K + k) * 2;
-                re |= fp16_to_e4m3_clamped(p[0]) << (8 * e);
-                im |= fp16_to_e4m3_clamped(p[1]) << (8 * e);
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + e;
+                d[e] = k < K ? reinterpret_cast<const uint32_t*>(in)[r * K + k] : 0u;
             }
         }
-        uint2* dst = reinterpret_cast<uint2*>(out + r * (int64_t)Kp * 2 + (int64_t)gi * 8);
-        *dst = make_uint2(re, im);
+        uint32_t re[2], im[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t* q = d + 4 * h;
+            // gather the Re halves and the Im halves of four samples
+            re[h] = fp16x4_to_e4m3(__builtin_amdgcn_perm(q[1], q[0], 0x05040100u),
+                                   __builtin_amdgcn_perm(q[3], q[2], 0x05040100u));
+            im[h] = fp16x4_to_e4m3(__builtin_amdgcn_perm(q[1], q[0], 0x07060302u),
+                                   __builtin_amdgcn_perm(q[3], q[2], 0x07060302u));
+        }
+        uint4* dst = reinterpret_cast<uint4*>(out + r * (int64_t)Kp * 2 + (int64_t)k0 * 2);
+        *dst = make_uint4(re[0], im[0], re[1], im[1]);
     }
 }
 
@@ -237,24 +238,39 @@ promote_fp16_fp8_planar_kernel(const uint16_t* __restrict__ in, uint8_t* __restr
 // operand of the GEMM holds conj(B)).  4 samples per thread.
 __global__ void __launch_bounds__(ST_THREADS)
 promote_fp16_planar_rows_kernel(const uint16_t* __restrict__ re_in, const uint16_t* __restrict__ im_in,
-                                uint8_t* __restrict__ out, int64_t rows, int K, int Kp, uint32_t im_sign) {
-    const int groups = Kp / 4;
+                                uint8_t* __restrict__ out, int64_t rows, int K, int Kp, uint32_t im_sign, int vec) {
+    const int groups = Kp / 8;
     const int64_t total = rows * groups;
     const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
     for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride) {
         const int64_t r = t / groups;
         const int gi = (int)(t - r * groups);
-        uint32_t re = 0u, im = 0u;
+        const int k0 = gi * 8;
+        uint32_t rr[4], ii[4];   // the 8 samples' FP16 bits, two per dword
+        uint32_t neg[2] = {0u, 0u};  // conj: sign bytes of the samples that exist
+        if (vec && k0 + 8 <= K) {
+            const uint4 a = *reinterpret_cast<const uint4*>(re_in + r * K + k0);
+            const uint4 b = *reinterpret_cast<const uint4*>(im_in + r * K + k0);
+            rr[0] = a.x; rr[1] = a.y; rr[2] = a.z; rr[3] = a.w;
+            ii[0] = b.x; ii[1] = b.y; ii[2] = b.z; ii[3] = b.w;
+            neg[0] = neg[1] = im_sign * 0x01010101u;
+        } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int k = gi * 4 + e;
-            if (k < K) {
-                re |= fp16_to_e4m3_clamped(re_in[r * K + k]) << (8 * e);
-                im |= (fp16_to_e4m3_clamped(im_in[r * K + k]) ^ im_sign) << (8 * e);
+            for (int d = 0; d < 4; ++d) {
+                const int k = k0 + 2 * d;
+                const uint32_t r0 = k < K ? re_in[r * K + k] : 0u, r1 = k + 1 < K ? re_in[r * K + k + 1] : 0u;
+                const uint32_t i0 = k < K ? im_in[r * K + k] : 0u, i1 = k + 1 < K ? im_in[r * K + k + 1] : 0u;
+                rr[d] = r0 | (r1 << 16);
+                ii[d] = i0 | (i1 << 16);
             }
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                if (k0 + e < K) neg[e >> 2] |= im_sign << (8 * (e & 3));
         }
-        uint2* dst = reinterpret_cast<uint2*>(out + r * (int64_t)Kp * 2 + (int64_t)gi * 8);
-        *dst = make_uint2(re, im);
+        const uint32_t re0 = fp16x4_to_e4m3(rr[0], rr[1]), re1 = fp16x4_to_e4m3(rr[2], rr[3]);
+        const uint32_t im0 = fp16x4_to_e4m3(ii[0], ii[1]) ^ neg[0], im1 = fp16x4_to_e4m3(ii[2], ii[3]) ^ neg[1];
+        uint4* dst = reinterpret_cast<uint4*>(out + r * (int64_t)Kp * 2 + (int64_t)k0 * 2);
+        *dst = make_uint4(re0, im0, re1, im1);
     }
 }
 
@@ -695,8 +711,10 @@ hipError_t launch_promote_fp16_fp8_planar(const uint16_t* in, uint8_t* out, int6
                                           hipStream_t s) {
     if (rows <= 0 || K <= 0) return hipSuccess;
     const int Kp = precast_row_samples(K);
-    hipLaunchKernelGGL(promote_fp16_fp8_planar_kernel, dim3(grid_for(rows * (Kp / 4))),
-                       dim3(ST_THREADS), 0, s, in, out, rows, K, Kp);
+    // 16-byte loads need every row (K samples of 4 bytes) to start 16-byte aligned
+    const int vec = (K % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 15u) == 0) ? 1 : 0;
+    hipLaunchKernelGGL(promote_fp16_fp8_planar_kernel, dim3(grid_for(rows * (Kp / 8))),
+                       dim3(ST_THREADS), 0, s, in, out, rows, K, Kp, vec);
     return hipGetLastError();
 }
 
@@ -704,8 +722,11 @@ hipError_t launch_promote_fp16_planar_rows(const uint16_t* re, const uint16_t* i
                                            int64_t rows, int K, bool conj, hipStream_t s) {
     if (rows <= 0 || K <= 0) return hipSuccess;
     const int Kp = precast_row_samples(K);
-    hipLaunchKernelGGL(promote_fp16_planar_rows_kernel, dim3(grid_for(rows * (Kp / 4))),
-                       dim3(ST_THREADS), 0, s, re, im, out, rows, K, Kp, conj ? 0x80u : 0u);
+    // 16-byte loads need every row (K samples of 2 bytes) to start 16-byte aligned
+    const int vec = (K % 8 == 0 && ((reinterpret_cast<uintptr_t>(re) | reinterpret_cast<uintptr_t>(im)) & 15u) == 0)
+                        ? 1 : 0;
+    hipLaunchKernelGGL(promote_fp16_planar_rows_kernel, dim3(grid_for(rows * (Kp / 8))),
+                       dim3(ST_THREADS), 0, s, re, im, out, rows, K, Kp, conj ? 0x80u : 0u, vec);
     return hipGetLastError();
 }
 

@@ -566,8 +566,8 @@ uint16_t ftdo_qc_nibble_fp16(unsigned nibble) {
 }
 
 uint8_t ftdo_f32_to_e4m3_clamped(float x) {
-    if (x > 448.0f) x = 448.0f;
-    if (x < -448.0f) x = -448.0f;
+    /* fminf(fmaxf(x, -448), 448) as cast_kernels_common.hpp:36-37: a NaN becomes -448 */
+    x = fminf(fmaxf(x, -448.0f), 448.0f);
     const unsigned sign = signbit(x) ? 0x80u : 0u;
     const float a = fabsf(x);
     unsigned code;

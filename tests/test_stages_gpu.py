@@ -138,6 +138,35 @@ def test_fp16_input_general_values(oracle, dev):
     assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
+def test_fp16_input_every_fp16_value(oracle, dev):
+    """Every FP16 bit pattern through the FP16-input HERK's E4M3 conversion: N = 2, K = 1,
+    row 0 = 1 + 0i, row 1 = (a, b), so output element (1, 0) is exactly (q(a), q(b)) in FP32.
+    32768 channels pair the 65536 patterns as (Re, Im), a second call swaps them; q is the
+    oracle's clamp-and-round (NaN -> -448, infinities -> +-448)."""
+    import torch
+    from ftd_amd import ftdx
+    bits = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    vals = bits.view(np.float16)
+    q = np.array([oracle.fp8e4m3_to_float(oracle.f32_to_e4m3_clamped(float(v)))
+                  for v in vals.astype(np.float32)], dtype=np.float32)
+    B = 32768
+    g = ftdx.CutlassComplexGemm()
+    for swap in (False, True):
+        X = np.zeros((B, 2, 1, 2), dtype=np.float16)
+        X[:, 0, 0, 0] = 1.0
+        a, b = (1, 0) if swap else (0, 1)
+        X[:, 1, 0, 0] = vals[a::2]
+        X[:, 1, 0, 1] = vals[b::2]
+        C = torch.empty((B, 3, 2), dtype=torch.float32, device=dev)
+        assert g.herk(torch.from_numpy(X.reshape(-1)).to(dev), C, 2, 1, B, ftdx.InputPrecision.FP16,
+                      ftdx.ComputePrecision.FP8, ftdx.OutputPrecision.FP32, tune=False) == 0
+        torch.cuda.synchronize()
+        got = C.cpu().numpy()[:, 1, :]
+        want = np.stack([q[a::2], q[b::2]], axis=-1)
+        # values, not bits: a -0 product added to the +0 accumulator is +0
+        assert np.array_equal(got, want), swap
+
+
 @pytest.mark.parametrize("N,K,B", [(128, 1024, 4), (100, 70, 2)])
 def test_force_baseline_equals_fused(oracle, dev, N, K, B):
     import torch

@@ -41,6 +41,20 @@ def test_fp16_to_e4m3_vs_torch(oracle):
     assert np.array_equal(got, ref)
 
 
+def test_fp16_to_e4m3_every_fp16_value(oracle):
+    """All 65536 FP16 bit patterns: finite values and infinities clamp to +-448 and round
+    as torch's float8_e4m3fn; a NaN becomes -448 (0xFE), what the reference's
+    fminf(fmaxf(x, -448), 448) makes of it (cast_kernels_common.hpp:36-37)."""
+    import torch
+    h = np.arange(65536, dtype=np.uint32).astype(np.uint16).view(np.float16).astype(np.float32)
+    nan = np.isnan(h)
+    clamped = np.clip(np.where(nan, 0, h), -448, 448).astype(np.float32)
+    ref = torch.from_numpy(clamped).to(torch.float8_e4m3fn).view(torch.uint8).numpy().copy()
+    ref[nan] = 0xFE
+    got = np.array([oracle.f32_to_e4m3_clamped(float(v)) for v in h], dtype=np.uint8)
+    assert np.array_equal(got, ref)
+
+
 def test_tri_from_herm_upper_row_major(oracle):
     N, B = 7, 2
     full = np.zeros((B, N, N, 2))
