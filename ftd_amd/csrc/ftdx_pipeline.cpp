@@ -176,35 +176,51 @@ int ftdx_pipeline_correlate_many(ftdx_pipeline* pl, void* const* vis_outputs,
         direct_out[p] = pinned_host(vis_outputs[p]) ? 1 : 0;
         if (pl->used[b]) {
             // staging_in[b] was last read by the H2D two payloads back
-            if ((e = hipEventSynchronize(pl->ev_h2d[b])) != hipSuccess)
-                return hip_fail(e, "pipeline_correlate: H2D");
+            if ((e = hipEventSynchronize(pl->ev_h2d[b])) != hipSuccess) {
+                rc = hip_fail(e, "pipeline_correlate: H2D");
+                break;
+            }
         }
         if (pl->in_bytes > 0 && !direct_in) host_copy(pl->h_in[b], raw_inputs[p], pl->in_bytes);
-        if (pl->used[b] && (e = hipStreamWaitEvent(pl->s_h2d, pl->ev_comp[b], 0)) != hipSuccess)
-            return hip_fail(e, "pipeline_correlate: wait");
+        if (pl->used[b] && (e = hipStreamWaitEvent(pl->s_h2d, pl->ev_comp[b], 0)) != hipSuccess) {
+            rc = hip_fail(e, "pipeline_correlate: wait");
+            break;
+        }
         if (pl->in_bytes > 0 &&
             (e = hipMemcpyAsync(pl->d_in[b], direct_in ? raw_inputs[p] : pl->h_in[b], pl->in_bytes,
-                                hipMemcpyHostToDevice, pl->s_h2d)) != hipSuccess)
-            return hip_fail(e, "pipeline_correlate: H2D");
+                                hipMemcpyHostToDevice, pl->s_h2d)) != hipSuccess) {
+            rc = hip_fail(e, "pipeline_correlate: H2D");
+            break;
+        }
         if ((e = hipEventRecord(pl->ev_h2d[b], pl->s_h2d)) != hipSuccess ||
-            (e = hipStreamWaitEvent(pl->s_comp, pl->ev_h2d[b], 0)) != hipSuccess)
-            return hip_fail(e, "pipeline_correlate: events");
-        if (pl->used[b] && (e = hipStreamWaitEvent(pl->s_comp, pl->ev_d2h[b], 0)) != hipSuccess)
-            return hip_fail(e, "pipeline_correlate: wait");
+            (e = hipStreamWaitEvent(pl->s_comp, pl->ev_h2d[b], 0)) != hipSuccess) {
+            rc = hip_fail(e, "pipeline_correlate: events");
+            break;
+        }
+        if (pl->used[b] && (e = hipStreamWaitEvent(pl->s_comp, pl->ev_d2h[b], 0)) != hipSuccess) {
+            rc = hip_fail(e, "pipeline_correlate: wait");
+            break;
+        }
         if ((rc = herk_on(pl, pl->d_in[b], pl->d_out[b], pl->s_comp)) != FTDX_OK) break;
         if ((e = hipEventRecord(pl->ev_comp[b], pl->s_comp)) != hipSuccess ||
-            (e = hipStreamWaitEvent(pl->s_d2h, pl->ev_comp[b], 0)) != hipSuccess)
-            return hip_fail(e, "pipeline_correlate: events");
+            (e = hipStreamWaitEvent(pl->s_d2h, pl->ev_comp[b], 0)) != hipSuccess) {
+            rc = hip_fail(e, "pipeline_correlate: events");
+            break;
+        }
         // staging_out[b] is free: its previous payload (p-2) was drained in iteration p-1
         if ((e = hipMemcpyAsync(direct_out[p] ? vis_outputs[p] : pl->h_out[b], pl->d_out[b], pl->out_bytes,
                                 hipMemcpyDeviceToHost, pl->s_d2h)) != hipSuccess ||
-            (e = hipEventRecord(pl->ev_d2h[b], pl->s_d2h)) != hipSuccess)
-            return hip_fail(e, "pipeline_correlate: D2H");
+            (e = hipEventRecord(pl->ev_d2h[b], pl->s_d2h)) != hipSuccess) {
+            rc = hip_fail(e, "pipeline_correlate: D2H");
+            break;
+        }
         pl->used[b] = true;
         if (p > 0) rc = drain(p - 1);
     }
     if (rc == FTDX_OK && n_payloads > 0) rc = drain(n_payloads - 1);
     if (rc != FTDX_OK) {
+        // nothing may stay queued against the caller's buffers (page-locked ones are read
+        // and written by the copies directly)
         (void)hipStreamSynchronize(pl->s_h2d);
         (void)hipStreamSynchronize(pl->s_comp);
         (void)hipStreamSynchronize(pl->s_d2h);
