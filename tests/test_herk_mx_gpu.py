@@ -312,33 +312,7 @@ def test_mx_fp4_int4_bit_exact(oracle, dev, rounding, N):
     assert np.array_equal(C.cpu().numpy().view(np.uint32), ref.view(np.uint32))
 
 
-def _reference_study_input(N, batch, K_values):
-    """The inputs of the reference's precision study (test_precision_accuracy.cu:205-215):
-    std::mt19937(42), uniform_real_distribution<float>(-0.5, 0.5), fresh interleaved FP16
-    data [batch][N][K][2] for each K in turn from the one generator.  libstdc++ draws one
-    32-bit word per float: r = float(u) / 2^32 (1 -> the float below 1), x = 1 * r - 0.5
-    in float, then FP16 nearest even."""
-    bg = np.random.MT19937(0)
-    bg._legacy_seeding(42)
-    out = {}
-    for K in K_values:
-        n = 2 * N * K * batch
-        u = bg.random_raw(n).astype(np.uint32)
-        r = u.astype(np.float32) / np.float32(2.0 ** 32)
-        r = np.where(r >= 1, np.nextafter(np.float32(1), np.float32(0)), r).astype(np.float32)
-        out[K] = (r + np.float32(-0.5)).astype(np.float32).astype(np.float16)
-    return out
-
-
-def test_reference_study_input_generator():
-    """First draws of std::mt19937(42) through uniform_real_distribution<float>(-0.5, 0.5)
-    as libstdc++ produces them (known answers)."""
-    x = _reference_study_input(1, 1, [16])[16][:5].astype(np.float32)
-    want = np.array([-0.12545988, 0.296543002, 0.45071429, -0.316565216, 0.231993914], dtype=np.float32)
-    assert np.array_equal(x, want.astype(np.float16).astype(np.float32))
-
-
-def test_reference_precision_study(oracle, dev):
+def test_reference_precision_study(oracle, dev, reference_study_input):
     """The reference's precision sweep (cutlass_interface/test_precision_accuracy.cu:172-357)
     on its own inputs: N = 512, batch 4, FP16 input, FP16 output, each precision's
     peak-relative error max|err| / max|ref| against the FP64 HERK of the FP16 values must
@@ -350,7 +324,7 @@ def test_reference_precision_study(oracle, dev):
     from ftd_amd import ftdx
     N, batch = 512, 4
     sweep = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
-    data = _reference_study_input(N, batch, sweep)
+    data = reference_study_input(N, batch, sweep)
     g = ftdx.CutlassComplexGemm()
     g.set_strategy_tune_verbosity(0)
     thresholds = {"FP8": 0.15, "FP6": 0.25, "FP4": 0.55}

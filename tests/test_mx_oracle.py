@@ -211,3 +211,11 @@ def test_ref_quantize_int4_blocks(oracle):
     assert list(deq[0, :7].real) == [2, 2, 3, 4, 4, 6, 6]
     assert list(deq[0, 7:14].imag) == [-2, -2, -3, -4, -4, -6, -6]
     assert list(deq[1, :5].real) == [1, 2, 3, 4, 4]
+
+
+def test_reference_study_input_generator(reference_study_input):
+    """First draws of std::mt19937(42) through uniform_real_distribution<float>(-0.5, 0.5)
+    as a g++ (libstdc++) build prints them, rounded to FP16 as the reference's study does."""
+    x = reference_study_input(1, 1, [16])[16][:5].astype(np.float32)
+    want = np.array([-0.12545988, 0.296543002, 0.45071429, -0.316565216, 0.231993914], dtype=np.float32)
+    assert np.array_equal(x, want.astype(np.float16).astype(np.float32))
