@@ -1,0 +1,105 @@
+"""Seeded random shapes through the HERK entry points, each checked against the oracle.
+
+Shapes are drawn from a fixed seed so a failure names a reproducible case: N from 1 to
+700 (edge tiles of every configuration), K from 1 to 1100 (tails of every stage length),
+1 to 3 channels, QC bytes with -0 nibbles, alpha / beta with exactly representable
+values, and every tile configuration forced in turn.
+- INT4 -> FP8 -> FP32 / FP16: bit-exact against the integer oracle (with its epilogue).
+- FP16 input holding the same integers: bit-exact, through the FP16 -> E4M3 pass.
+- MX FP6 / FP4 on FP16 voltages, both element roundings: within the MFMA's accumulation of
+  the FP64 HERK of the oracle's dequantized values.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = 32
+
+
+def _draw(seed):
+    rng = np.random.default_rng(seed)
+    N = int(rng.integers(1, 701))
+    K = int(rng.integers(1, 1101))
+    B = int(rng.integers(1, 4))
+    cfg = int(rng.integers(0, 6))
+    alpha = float(rng.choice([1.0, 0.5, -2.0]))
+    beta = float(rng.choice([0.0, 1.0, 0.25]))
+    return rng, N, K, B, cfg, alpha, beta
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view({4: np.uint32, 2: np.uint16}[a.dtype.itemsize])
+
+
+@pytest.mark.parametrize("case", range(CASES))
+@pytest.mark.parametrize("output", ["fp32", "fp16"])
+def test_int4_random_shape(oracle, dev, case, output):
+    import torch
+    from ftd_amd import ftdx
+    rng, N, K, B, cfg, alpha, beta = _draw(1000 + case)
+    if output == "fp16":
+        K = min(K, 250)  # |alpha| * 98 K + |beta| * 40 stays below the FP16 maximum
+    A = oracle.gen_bytes(2000 + case, N * K * B)
+    P = N * (N + 1) // 2
+    C0 = rng.integers(-40, 40, size=(B, P, 2)).astype(np.float32)
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, int(case % 2))
+    odt = torch.float32 if output == "fp32" else torch.float16
+    C = torch.from_numpy(C0).to(dev).to(odt)
+    print(f"random INT4 case {case}: N={N} K={K} B={B} cfg={cfg} alpha={alpha} beta={beta} {output}",
+          flush=True)
+    out = ftdx.OutputPrecision.FP32 if output == "fp32" else ftdx.OutputPrecision.FP16
+    assert g.herk(torch.from_numpy(A).to(dev), C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                  out, alpha=alpha, beta=beta, tune=False) == 0
+    torch.cuda.synchronize()
+    if output == "fp32":
+        ref = oracle.herk_f32(A, N, K, B, alpha=alpha, beta=beta, C=C0)
+    else:
+        ref = oracle.herk_f16(A, N, K, B, alpha=alpha, beta=beta, C=C0.astype(np.float16))
+    assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref))
+
+
+@pytest.mark.parametrize("case", range(CASES))
+def test_fp16_input_random_shape(oracle, dev, case):
+    """The QC integers as interleaved FP16 pairs: the FP16 -> E4M3 pass is exact on them."""
+    import torch
+    from ftd_amd import ftdx
+    rng, N, K, B, cfg, alpha, beta = _draw(3000 + case)
+    A = oracle.gen_qc_uniform(4000 + case, N * K * B)
+    nib = np.array([0, 1, 2, 3, 4, 5, 6, 7, 0, -1, -2, -3, -4, -5, -6, -7], dtype=np.float16)
+    X16 = np.stack([nib[A >> 4], nib[A & 15]], axis=-1)
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, 1)
+    C = torch.empty((B, N * (N + 1) // 2, 2), dtype=torch.float32, device=dev)
+    print(f"random FP16-input case {case}: N={N} K={K} B={B} cfg={cfg}", flush=True)
+    assert g.herk(torch.from_numpy(X16.reshape(-1)).to(dev), C, N, K, B, ftdx.InputPrecision.FP16,
+                  ftdx.ComputePrecision.FP8, ftdx.OutputPrecision.FP32, tune=False) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(_bits(C.cpu().numpy()), _bits(oracle.herk_f32(A, N, K, B)))
+
+
+@pytest.mark.parametrize("case", range(CASES // 3))
+@pytest.mark.parametrize("compute", ["FP6", "FP4"])
+@pytest.mark.parametrize("rounding", [0, 1])
+def test_mx_random_shape(oracle, dev, case, compute, rounding):
+    import torch
+    from ftd_amd import ftdx
+    rng, N, K, B, cfg, _, _ = _draw(5000 + case)
+    N, K = min(N, 400), min(K, 700)
+    fmt = 0 if compute == "FP4" else 1
+    x = (rng.standard_normal((B * N, K)) + 1j * rng.standard_normal((B * N, K))) * (0.5 ** 0.5)
+    X16 = np.stack([x.real, x.imag], axis=-1).astype(np.float16)
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_launch_config(cfg, 1)
+    ctx.set_mx_rounding(ftdx.MxRounding(rounding))
+    print(f"random MX case {case}: {compute} rounding={rounding} N={N} K={K} B={B} cfg={cfg}", flush=True)
+    C = ftdx.herk_mx(ctx, torch.from_numpy(X16).to(dev), N, K, B, ftdx.InputPrecision.FP16,
+                     getattr(ftdx.ComputePrecision, compute), output="fp32")
+    torch.cuda.synchronize()
+    codes, scales = oracle.mx_quantize(X16.astype(np.float32), fmt, rounding)
+    ref = oracle.herk_c64(oracle.mx_dequantize(codes, scales, fmt, K).reshape(B, N, K), N, K, B)
+    got = C.cpu().numpy().astype(np.float64)
+    tol = 2e-6 if compute == "FP4" else 5e-4
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
