@@ -103,3 +103,40 @@ def test_mx_random_shape(oracle, dev, case, compute, rounding):
     got = C.cpu().numpy().astype(np.float64)
     tol = 2e-6 if compute == "FP4" else 5e-4
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("case", range(CASES // 2))
+def test_gemm_random_shape(oracle, dev, case):
+    """Complex GEMM with prepared conj(B) (prepare_b, gemm_prepared_direct[_int4],
+    gemm_prepared_power_int4) on integer data: planar FP32 outputs bit-exact against the
+    oracle's complex product, the power against its single-rounding float steps."""
+    import torch
+    from ftd_amd import ftdx
+    rng, N, K, B, cfg, _, _ = _draw(7000 + case)
+    M = int(rng.integers(1, 400))
+    re = rng.integers(-7, 8, (B, M, K)).astype(np.float16)
+    im = rng.integers(-7, 8, (B, M, K)).astype(np.float16)
+    b_re = rng.integers(-7, 8, (B, N, K)).astype(np.float16)
+    b_im = rng.integers(-7, 8, (B, N, K)).astype(np.float16)
+    qc = oracle.gen_bytes(8000 + case, M * K * B)
+    Aq = oracle.decode_qc_array(qc).reshape(B, M, K)
+    dA_re, dA_im, dB_re, dB_im, dqc = (torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+                                       for x in (re, im, b_re, b_im, qc))
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, int(case % 2))
+    print(f"random GEMM case {case}: M={M} N={N} K={K} B={B} cfg={cfg}", flush=True)
+    g.prepare_b(dB_re, dB_im, N, K, B)
+    C = [torch.empty((B, M, N), dtype=torch.float32, device=dev) for _ in range(5)]
+    assert g.gemm_prepared_direct(dA_re, dA_im, C[0], C[1], M, N, K, B) == 0
+    assert g.gemm_prepared_direct_int4(dqc, C[2], C[3], M, N, K, B) == 0
+    assert g.gemm_prepared_power_int4(dqc, C[4], M, N, K, B) == 0
+    torch.cuda.synchronize()
+    Bc = b_re.astype(np.float64) + 1j * b_im
+    ref = oracle.gemm_c64(re.astype(np.float64) + 1j * im, Bc)
+    assert np.array_equal(_bits(C[0].cpu().numpy()), _bits(ref.real.astype(np.float32)))
+    assert np.array_equal(_bits(C[1].cpu().numpy()), _bits(ref.imag.astype(np.float32)))
+    ref4 = oracle.gemm_c64(Aq, Bc)
+    re32, im32 = ref4.real.astype(np.float32), ref4.imag.astype(np.float32)
+    assert np.array_equal(_bits(C[2].cpu().numpy()), _bits(re32))
+    assert np.array_equal(_bits(C[3].cpu().numpy()), _bits(im32))
+    assert np.array_equal(_bits(C[4].cpu().numpy()), _bits(oracle.power_f32(re32, im32).reshape(B, M, N)))
