@@ -140,3 +140,35 @@ def test_gemm_random_shape(oracle, dev, case):
     assert np.array_equal(_bits(C[2].cpu().numpy()), _bits(re32))
     assert np.array_equal(_bits(C[3].cpu().numpy()), _bits(im32))
     assert np.array_equal(_bits(C[4].cpu().numpy()), _bits(oracle.power_f32(re32, im32).reshape(B, M, N)))
+
+
+@pytest.mark.parametrize("case", range(CASES // 4))
+def test_grouped_random_list(oracle, dev, case):
+    """A random list of 1 to 14 problems (N 1-600, K 1-900, raw QC bytes, alpha / beta) as
+    one grouped plan on a random tile configuration, run twice; every problem bit-exact."""
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(9000 + case)
+    count = int(rng.integers(1, 15))
+    cfg = int(rng.integers(-1, 6))
+    probs, refs = [], []
+    for i in range(count):
+        N, K = int(rng.integers(1, 601)), int(rng.integers(1, 901))
+        alpha = float(rng.choice([1.0, 0.5, -2.0]))
+        beta = float(rng.choice([0.0, 1.0]))
+        A_np = oracle.gen_bytes(9100 + 20 * case + i, N * K)
+        C0 = rng.integers(-40, 40, size=(N * (N + 1) // 2, 2)).astype(np.float32)
+        probs.append((torch.from_numpy(A_np).to(dev), torch.from_numpy(C0).to(dev), N, K, alpha, beta))
+        refs.append((C0, oracle.herk_f32(A_np, N, K, 1, alpha=alpha, beta=beta, C=C0[None])[0]))
+    print(f"random grouped case {case}: {count} problems, cfg={cfg}: "
+          + " ".join(f"{p[2]}x{p[3]}" for p in probs), flush=True)
+    ctx = ftdx.CutlassComplexGemm()
+    plan = ftdx.GroupedPlan(ctx, probs, tile_config=cfg, use_graph=bool(case % 2))
+    for rep in range(2):
+        for (A, C, N, K, _, _), (C0, _) in zip(probs, refs):
+            C.copy_(torch.from_numpy(C0))  # beta reads the old values again
+        plan.run()
+        torch.cuda.synchronize()
+        for (A, C, N, K, _, _), (_, ref) in zip(probs, refs):
+            assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (rep, N, K)
+    plan.close()
