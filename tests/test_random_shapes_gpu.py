@@ -172,3 +172,42 @@ def test_grouped_random_list(oracle, dev, case):
         for (A, C, N, K, _, _), (_, ref) in zip(probs, refs):
             assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (rep, N, K)
     plan.close()
+
+
+@pytest.mark.parametrize("case", range(CASES // 4))
+def test_pol_reduce_and_grid_random(oracle, dev, case):
+    """The polarization-summed HERK (polarizations as K segments) and the fused gridding
+    epilogue on random shapes: bit-exact against the oracle's int64 HERK, polarization sum
+    and pillbox scatter.  N * K * n_pol * 98 stays below 2^24, so every grid cell (the
+    centre collects every auto-correlation) is an exact FP32 sum."""
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(11000 + case)
+    N, K = int(rng.integers(1, 201)), int(rng.integers(1, 201))
+    n_ch, n_pol, n_ti = int(rng.integers(1, 4)), int(rng.integers(1, 4)), int(rng.integers(1, 3))
+    cfg = int(rng.integers(0, 6))
+    qc = oracle.gen_bytes(12000 + case, N * K * n_ch * n_pol * n_ti)
+    A = torch.from_numpy(qc).to(dev)
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, int(case % 2))
+    print(f"random pol/grid case {case}: N={N} K={K} ch={n_ch} pol={n_pol} ti={n_ti} cfg={cfg}", flush=True)
+    fused = ftdx.herk_pol_reduce(g, A, N, K, n_ch, n_pol, n_ti)
+    torch.cuda.synchronize()
+    per = oracle.herk_int64(qc, N, K, n_ch * n_pol * n_ti).reshape(n_ch, n_pol, n_ti, -1, 2).sum(axis=1)
+    assert np.array_equal(_bits(fused.cpu().numpy()), _bits(per.reshape(n_ch * n_ti, -1, 2).astype(np.float32)))
+    # gridding on the first time-inner slice layout: n_ch channels of n_pol polarizations
+    Ng, cell, alpha = 64, 2e-3, 0.5
+    qg = qc[:N * K * n_ch * n_pol]
+    pos = rng.standard_normal((N, 2)) * 0.015
+    ii, jj = np.tril_indices(N)
+    uv = (pos[ii] - pos[jj]).astype(np.float32)
+    freqs = np.array([0.9e9, 1.1e9, 1.3e9])
+    grid = torch.zeros((n_ch, Ng, Ng, 2), dtype=torch.float32, device=dev)
+    ftdx.herk_pol_grid(g, torch.from_numpy(qg).to(dev), N, K, n_ch, n_pol, torch.from_numpy(uv).to(dev),
+                       torch.from_numpy(freqs).to(dev), grid, Ng, cell, alpha=alpha)
+    torch.cuda.synchronize()
+    pg = oracle.herk_int64(qg, N, K, n_ch * n_pol).reshape(n_ch, n_pol, -1, 2).sum(axis=1)
+    vis = (pg.astype(np.float32) * np.float32(alpha)).astype(np.float32)
+    vis[:, [i * (i + 1) // 2 + i for i in range(N)], 1] = 0.0
+    ref = oracle.pillbox_grid(vis, uv, freqs, N * (N + 1) // 2, Ng, n_ch, cell)
+    assert np.array_equal(_bits(grid.cpu().numpy()), _bits(ref))
