@@ -216,8 +216,9 @@ __global__ void __launch_bounds__(256) mx_quantize_kernel(const void* __restrict
         for (int i = 0; i < 32; ++i) amax = __builtin_fmaxf(amax, __builtin_fabsf(v[part][i]));
         const uint32_t se = mx_scale_exponent<FMT>(amax);
         const float inv = __builtin_ldexpf(1.0f, 127 - (int)se);
-        // v * inv is exact (a power of two within range); the gfx950 converters then
-        // round to nearest even with subnormals and saturate, the OCP MX element rule
+        // v * inv is exact (a power of two within range); then either the reference's
+        // converter per element, or the gfx950 converters, which round to nearest even
+        // with subnormals and saturate (the OCP MX element rule)
         uint32_t w[6] = {0u, 0u, 0u, 0u, 0u, 0u};
         if constexpr (RND == MX_ROUND_REFERENCE) {
 #pragma unroll

@@ -235,7 +235,8 @@ promote_fp16_fp8_planar_kernel(const uint16_t* __restrict__ in, uint8_t* __restr
 }
 
 // Planar FP16 Re / Im [rows][K] -> pre-cast rows; conj negates Im (the prepared B
-// operand of the GEMM holds conj(B)).  4 samples per thread.
+// operand of the GEMM holds conj(B)).  8 samples per thread, 16-byte loads when the row's
+// 8 samples are whole and 16-byte aligned.
 __global__ void __launch_bounds__(ST_THREADS)
 promote_fp16_planar_rows_kernel(const uint16_t* __restrict__ re_in, const uint16_t* __restrict__ im_in,
                                 uint8_t* __restrict__ out, int64_t rows, int K, int Kp, uint32_t im_sign, int vec) {
