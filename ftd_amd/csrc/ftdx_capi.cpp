@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -158,6 +159,86 @@ void pick_config(ftdx_context* ctx, const TuneKey& key, int& cfg, int& remap, bo
     if (remap < 0) remap = kDefaultRemap;
 }
 
+// Compute units of the current device, cached per device.
+int device_cu_count() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+    if (dev < 64) {
+        const int c = cache[dev].load(std::memory_order_relaxed);
+        if (c > 0) return c;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (dev < 64) cache[dev].store(cus, std::memory_order_relaxed);
+    return cus;
+}
+
+// Pre-cast paths (FP16 input, ForceBaseline, E3M2 operands) convert and multiply a chunk
+// of batches at a time: the chunk's operand rows (at most kPrecastChunkBytes) are then
+// still in the memory-side cache when its tiles read them, and the scratch holds one
+// chunk.  Among chunk sizes up to that cap, the one whose launches leave the fewest idle
+// workgroup slots in their last rounds (plus a small cost per launch) wins.
+constexpr size_t kPrecastChunkBytes = size_t(320) << 20;
+
+int precast_chunk_cap(int N, int K, int batch) {
+    const size_t per = (size_t)N * 2 * precast_row_samples(K);
+    const size_t cap = per > 0 ? kPrecastChunkBytes / per : (size_t)batch;
+    return (int)std::max<size_t>(1, std::min<size_t>(cap, (size_t)std::max(batch, 1)));
+}
+
+int precast_chunk(int cfg, int N, int K, int batch) {
+    const int cmax = precast_chunk_cap(N, K, batch);
+    if (cmax >= batch) return std::max(batch, 1);
+    const int64_t slots = (int64_t)std::max(device_cu_count(), 1) * tile_config_wg_per_cu(cfg);
+    const int64_t tiles = herk_tiles(cfg, N);
+    double best = 1e30;
+    int pick = cmax;
+    for (int c = cmax; c >= 1; --c) {
+        int64_t idle = 0;
+        for (int b0 = 0; b0 < batch; b0 += c) {
+            const int64_t t = tiles * std::min(c, batch - b0);
+            idle += (t + slots - 1) / slots * slots - t;
+        }
+        const double cost = (double)idle / (double)slots + 0.1 * ((batch + c - 1) / c);
+        if (cost < best) {
+            best = cost;
+            pick = c;
+        }
+    }
+    return pick;
+}
+
+// The pre-cast path of one call: per chunk of batches, convert A's rows into the
+// context's scratch (FP8 planar rows, or E3M2 rows when bf6), then run the tile over
+// them.  The scratch must hold N * precast_chunk_cap(...) rows of 2 * Kp bytes.
+hipError_t run_precast(ftdx_context* ctx, const HerkPlainArgs& a, int input, bool bf6, int cfg, bool fp16,
+                       int remap, hipStream_t st) {
+    const int Kp = precast_row_samples(a.K);
+    const int chunk = precast_chunk(cfg, a.N, a.K, a.batch);
+    const int64_t in_bytes = (int64_t)a.N * a.K * (input == FTDX_INPUT_FP16 ? 4 : 1);
+    const int64_t out_bytes = a.c_bstride * (fp16 ? 2 : 4);
+    for (int b0 = 0; b0 < a.batch; b0 += chunk) {
+        const int nb = std::min(chunk, a.batch - b0);
+        const uint8_t* src = a.A + (int64_t)b0 * in_bytes;
+        const int64_t rows = (int64_t)a.N * nb;
+        hipError_t e = input == FTDX_INPUT_FP16
+            ? launch_promote_fp16_fp8_planar(reinterpret_cast<const uint16_t*>(src), ctx->precast, rows, a.K, st)
+            : bf6 ? launch_promote_qc_bf6_planar(src, ctx->precast, rows, a.K, st)
+                  : launch_promote_qc_fp8_planar(src, ctx->precast, rows, a.K, st);
+        if (e != hipSuccess) return e;
+        HerkPlainArgs b = a;
+        b.A = ctx->precast;
+        b.C = static_cast<uint8_t*>(a.C) + (int64_t)b0 * out_bytes;
+        b.batch = nb;
+        b.precast = bf6 ? 2 : 1;
+        b.lda = (int64_t)2 * Kp;
+        b.a_bstride = (int64_t)a.N * 2 * Kp;
+        if ((e = launch_herk_int4_plain(b, cfg, fp16, remap, st)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 // Times every (tile configuration, workgroup order) on this call's buffers and
 // stores the fastest (cutlass_interface/strategy_cache.hpp:504-800: warm-up, then
 // event-timed repeats, failures skipped).  With beta != 0 the timing runs use a
@@ -184,16 +265,9 @@ int autotune(ftdx_context* ctx, const HerkPlainArgs& a0, bool out_fp16, hipStrea
     hipEvent_t t0, t1;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
-    const int Kp = precast_row_samples(a.K);
-    HerkPlainArgs a6 = a;
-    a6.A = ctx->precast;
-    a6.precast = 2;
-    a6.lda = (int64_t)2 * Kp;
-    a6.a_bstride = (int64_t)a.N * 2 * Kp;
     auto launch = [&](int cfg, int remap) -> hipError_t {
         if (!(cfg & TILE_BF6)) return launch_herk_int4_plain(a, cfg, out_fp16, remap, s);
-        hipError_t e2 = launch_promote_qc_bf6_planar(a.A, ctx->precast, (int64_t)a.N * a.batch, a.K, s);
-        return e2 != hipSuccess ? e2 : launch_herk_int4_plain(a6, tile_base(cfg), out_fp16, remap, s);
+        return run_precast(ctx, a, FTDX_INPUT_INT4, true, tile_base(cfg), out_fp16, remap, s);
     };
     // mean ms of `reps` launches, or a negative value if the launch failed
     auto time_cand = [&](int cfg, int remap, int reps) -> float {
@@ -831,7 +905,7 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     ftdx::pick_config(ctx, key, cfg, remap, cached);
     const bool explicit_cfg = ctx->forced_cfg >= 0 || ctx->direct_tile != FTDX_DIRECT_TILE_AUTO;
     const int Kp = ftdx::precast_row_samples(K);
-    const size_t rows_need = (size_t)N * batch_count * 2 * Kp;
+    const size_t rows_need = (size_t)N * ftdx::precast_chunk_cap(N, K, batch_count) * 2 * Kp;
     if (precast) {
         int rc = ftdx::grow_precast(ctx, rows_need, "ftdx_herk: scratch");
         if (rc != FTDX_OK) return rc;
@@ -856,22 +930,8 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
         if (rc != FTDX_OK) return rc;
     }
     auto run = [&](hipStream_t st) -> hipError_t {
-        ftdx::HerkPlainArgs b = a;
-        if (precast || bf6) {
-            hipError_t pe = input == FTDX_INPUT_FP16
-                ? ftdx::launch_promote_fp16_fp8_planar(static_cast<const uint16_t*>(A), ctx->precast,
-                                                       (int64_t)N * batch_count, K, st)
-                : bf6 ? ftdx::launch_promote_qc_bf6_planar(static_cast<const uint8_t*>(A), ctx->precast,
-                                                           (int64_t)N * batch_count, K, st)
-                      : ftdx::launch_promote_qc_fp8_planar(static_cast<const uint8_t*>(A), ctx->precast,
-                                                           (int64_t)N * batch_count, K, st);
-            if (pe != hipSuccess) return pe;
-            b.A = ctx->precast;
-            b.precast = bf6 ? 2 : 1;
-            b.lda = (int64_t)2 * Kp;
-            b.a_bstride = (int64_t)N * 2 * Kp;
-        }
-        return ftdx::launch_herk_int4_plain(b, cfg, fp16, remap, st);
+        if (precast || bf6) return ftdx::run_precast(ctx, a, input, bf6, cfg, fp16, remap, st);
+        return ftdx::launch_herk_int4_plain(a, cfg, fp16, remap, st);
     };
     // Graph replay of identical calls (cutlass_gemm_api.cu:1738-1807): capture the
     // launch sequence once per (shape, scalars, pointers), then replay it.

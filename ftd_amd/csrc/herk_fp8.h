@@ -19,7 +19,7 @@ enum TileConfigId {
 
 // Flag on a tile configuration id (ftdx_herk, INT4 input): the same tile fed by
 // pre-cast E3M2 operand rows.  Exact for QC data, MFMAs at twice the FP8 rate, one
-// extra pass over the input.
+// extra pass over the input (per chunk of batches, ftdx_herk).
 constexpr int TILE_BF6 = 16;
 inline bool valid_tile_config(int c) { return c >= 0 && (c & ~TILE_BF6) < NUM_TILE_CONFIGS; }
 inline int tile_base(int c) { return c & ~TILE_BF6; }

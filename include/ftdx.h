@@ -86,8 +86,9 @@ extern "C" {
 #define FTDX_TILE_128x128_K64_P2 4 /* as 0, global loads two stages ahead */
 #define FTDX_TILE_256x128_K64_P2 5 /* as 3, global loads two stages ahead */
 /* Flag on the ids above, ftdx_herk with INT4 input only: the same tile fed by operand
- * rows pre-cast to OCP E3M2 (6-bit) in one extra pass.  Every QC value is exact in
- * E3M2, so results are identical to the FP8 path; the MFMAs run at twice the FP8 rate.
+ * rows pre-cast to OCP E3M2 (6-bit) by an extra pass per chunk of channels.  Every QC
+ * value is exact in E3M2, so results are identical to the FP8 path; the MFMAs run at
+ * twice the FP8 rate.
  * Chosen explicitly (ftdx_set_launch_config) or from a tune-cache entry; the autotuner
  * times the FP8 tiles only.  Other entry points use the tile without the flag.  No
  * counterpart in the reference (its INT4 path computes in FP8). */

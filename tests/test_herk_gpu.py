@@ -251,3 +251,41 @@ def test_full_size_cfg2(oracle, dev, tile_config):
     for b in range(B):
         lhs, rhs = _weighted_checksum_check(torch, A2[b], C[b], N, K, seed=100 + b)
         assert lhs == rhs, (b, lhs, rhs)
+
+
+def test_precast_paths_in_chunks(dev):
+    """The pre-cast paths (FP16 input, ForceBaseline INT4, E3M2 operand tiles) convert and
+    multiply chunks of at most 320 MB of operand rows.  2048 inputs x 90 channels x K=1024
+    (4 MB of rows per channel) runs them in several chunks, the last one partial; every
+    path's output bytes must equal the fused INT4 HERK's, which is checked exactly
+    (weighted checksums of every channel, exact rows)."""
+    import torch
+    from ftd_amd import ftdx, selfcheck
+    N, K, B = 2048, 1024, 90
+    A = selfcheck.qc_uniform(N * K * B, 11, dev)
+    I4, F16 = ftdx.InputPrecision.INT4, ftdx.InputPrecision.FP16
+    F8, O32 = ftdx.ComputePrecision.FP8, ftdx.OutputPrecision.FP32
+    P = N * (N + 1) // 2
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_launch_config(5, 1)
+
+    def run(X, inp):
+        C = torch.empty((B, P, 2), dtype=torch.float32, device=dev)
+        assert ctx.herk(X, C, N, K, B, inp, F8, O32, tune=False) == 0
+        torch.cuda.synchronize()
+        return C
+
+    ref = run(A, I4)
+    ok = selfcheck.check_herk(torch, A, ref, N, K, B, rows=[0, 1, 1023, 2047], row_channels=[0, 63, 64, 89])
+    assert ok["ok"], ok
+    ctx.set_launch_config(5 | 16, 1)  # E3M2 operand rows
+    assert torch.equal(run(A, I4).view(torch.int32), ref.view(torch.int32))
+    ctx.set_launch_config(5, 1)
+    ctx.set_herk_mode(ftdx.HerkMode.ForceBaseline)  # FP8 operand rows
+    assert torch.equal(run(A, I4).view(torch.int32), ref.view(torch.int32))
+    ctx.set_herk_mode(ftdx.HerkMode.Auto)
+    re, im = selfcheck._decode(torch, A)
+    X16 = torch.stack([re, im], dim=-1).to(torch.float16).contiguous()
+    del re, im
+    assert torch.equal(run(X16, F16).view(torch.int32), ref.view(torch.int32))
+    ctx.close()
