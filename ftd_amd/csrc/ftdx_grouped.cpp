@@ -27,6 +27,7 @@ struct ftdx_grouped_plan {
     int out_fp16 = 0;
     int num_wg = 0;
     int count = 0;
+    bool all_vec = false;  // every problem takes 16-byte loads
     int64_t items = 0;
     double max_load = 0, mean_load = 0;
     ftdx::GroupProblemDev* d_probs = nullptr;
@@ -58,6 +59,7 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
     if (rc == -2) return fail(FTDX_ERR_INVALID_ARG, "grouped: too many tiles");
     if (rc != 0) return fail(FTDX_ERR_INVALID_ARG, "grouped: invalid problem list");
     std::vector<GroupProblemDev> probs(count);
+    bool all_vec = true;
     for (int p = 0; p < count; ++p) {
         const ftdx_herk_problem& q = problems[p];
         GroupProblemDev& d = probs[p];
@@ -69,6 +71,7 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
         d.beta = q.beta;
         d.vec = (q.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(q.A) & 15u) == 0);
         d.tiles = sch.tiles[p];
+        all_vec = all_vec && d.vec;
     }
     static_assert(sizeof(GroupItemDev) == sizeof(ScheduleItem), "schedule items are uploaded as they are");
     const std::vector<ScheduleItem>& flat = sch.items;
@@ -97,6 +100,7 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
                        hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "grouped plan: upload");
     P->num_wg = G;
+    P->all_vec = all_vec;
     P->cfg = cfg;
     P->out_fp16 = out_fp16;
     P->count = count;
@@ -105,7 +109,7 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
 
 hipError_t grouped_launch(ftdx_grouped_plan* P, hipStream_t s) {
     return launch_herk_int4_grouped(P->d_probs, P->d_items, P->d_wg_begin, P->num_wg, P->cfg,
-                                    P->out_fp16 != 0, s);
+                                    P->out_fp16 != 0, P->all_vec, s);
 }
 
 void grouped_release(ftdx_grouped_plan* P) {

@@ -153,9 +153,10 @@ struct GroupItemDev {
     int tile;
 };
 
-// Persistent kernel: workgroup g processes items[wg_begin[g] .. wg_begin[g+1]).
+// Persistent kernel: workgroup g processes items[wg_begin[g] .. wg_begin[g+1]).  all_vec:
+// every problem has vec set (the kernel then compiles only the 16-byte load path).
 hipError_t launch_herk_int4_grouped(const GroupProblemDev* probs, const GroupItemDev* items,
-                                    const int* wg_begin, int num_wg, int cfg, bool out_fp16,
+                                    const int* wg_begin, int num_wg, int cfg, bool out_fp16, bool all_vec,
                                     hipStream_t stream);
 
 }  // namespace ftdx

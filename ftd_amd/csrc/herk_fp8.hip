@@ -88,7 +88,10 @@ herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
     TileRunner<Cfg>::template run<OutT, MODE, FMT>(j, lds);
 }
 
-template <class Cfg, typename OutT>
+// VEC_ONLY: every problem of the plan takes 16-byte loads, so only that load path is
+// compiled in and the tile keeps its own prefetch depth; otherwise both load paths are
+// inlined and loads run one stage ahead (the second register set would spill).
+template <class Cfg, typename OutT, bool VEC_ONLY>
 __global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
 herk_int4_fp8_grouped_kernel(const GroupProblemDev* __restrict__ probs,
                              const GroupItemDev* __restrict__ items,
@@ -111,7 +114,9 @@ herk_int4_fp8_grouped_kernel(const GroupProblemDev* __restrict__ probs,
         j.beta = pr.beta;
         j.lda = pr.K;
         j.nseg = 1;
-        if (pr.vec)
+        if constexpr (VEC_ONLY)
+            TileRunner<Cfg>::template run<OutT, LOAD_VEC16, OUT_LOWER_PACKED, false>(j, lds);
+        else if (pr.vec)
             TileRunner<Cfg>::template run<OutT, LOAD_VEC16, OUT_LOWER_PACKED, true>(j, lds);
         else
             TileRunner<Cfg>::template run<OutT, LOAD_BYTES, OUT_LOWER_PACKED, true>(j, lds);
@@ -338,14 +343,23 @@ hipError_t launch_ggp_cfg(const HerkPlainArgs& a, int fmt, bool out_fp64, int xc
 
 template <class Cfg>
 hipError_t launch_grouped_cfg(const GroupProblemDev* probs, const GroupItemDev* items,
-                              const int* wg_begin, int num_wg, bool out_fp16, hipStream_t stream) {
+                              const int* wg_begin, int num_wg, bool out_fp16, bool all_vec, hipStream_t stream) {
     dim3 grid((unsigned)num_wg), block(Cfg::THREADS);
-    if (out_fp16)
-        hipLaunchKernelGGL((herk_int4_fp8_grouped_kernel<Cfg, _Float16>), grid, block, 0, stream,
-                           probs, items, wg_begin);
-    else
-        hipLaunchKernelGGL((herk_int4_fp8_grouped_kernel<Cfg, float>), grid, block, 0, stream,
-                           probs, items, wg_begin);
+    if (all_vec) {
+        if (out_fp16)
+            hipLaunchKernelGGL((herk_int4_fp8_grouped_kernel<Cfg, _Float16, true>), grid, block, 0, stream,
+                               probs, items, wg_begin);
+        else
+            hipLaunchKernelGGL((herk_int4_fp8_grouped_kernel<Cfg, float, true>), grid, block, 0, stream,
+                               probs, items, wg_begin);
+    } else {
+        if (out_fp16)
+            hipLaunchKernelGGL((herk_int4_fp8_grouped_kernel<Cfg, _Float16, false>), grid, block, 0, stream,
+                               probs, items, wg_begin);
+        else
+            hipLaunchKernelGGL((herk_int4_fp8_grouped_kernel<Cfg, float, false>), grid, block, 0, stream,
+                               probs, items, wg_begin);
+    }
     return hipGetLastError();
 }
 
@@ -412,11 +426,11 @@ hipError_t launch_gemm_fp8(const GemmArgs& a, int cfg, int fmt, bool out_fp16, i
 }
 
 hipError_t launch_herk_int4_grouped(const GroupProblemDev* probs, const GroupItemDev* items,
-                                    const int* wg_begin, int num_wg, int cfg, bool out_fp16,
+                                    const int* wg_begin, int num_wg, int cfg, bool out_fp16, bool all_vec,
                                     hipStream_t stream) {
     if (num_wg <= 0) return hipSuccess;
     return with_cfg(cfg, [&](auto c) {
-        return launch_grouped_cfg<decltype(c)>(probs, items, wg_begin, num_wg, out_fp16, stream);
+        return launch_grouped_cfg<decltype(c)>(probs, items, wg_begin, num_wg, out_fp16, all_vec, stream);
     });
 }
 

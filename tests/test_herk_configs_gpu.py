@@ -75,14 +75,24 @@ def _grouped_case(oracle, dev, shapes, seed):
     return probs, refs
 
 
+# "mixed": some K not a multiple of 16 (both load paths in one kernel); "vec": every
+# problem takes 16-byte loads (the 16-byte-only kernel at the tile's own prefetch depth),
+# with partial tiles and one-stage problems
+GROUPED_SHAPES = {
+    "mixed": [(256, 320), (64, 64), (130, 1000), (512, 256), (33, 17), (1, 5), (300, 64),
+              (128, 4096), (200, 2048), (96, 100)],
+    "vec": [(256, 320), (130, 1024), (512, 256), (300, 64), (128, 4096), (200, 2048), (96, 112),
+            (1, 16), (33, 48), (640, 192)],
+}
+
+
 @pytest.mark.parametrize("cfg", [-1, 0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("graph", [False, True])
-def test_grouped_uneven_bit_exact(oracle, dev, cfg, graph):
+@pytest.mark.parametrize("shapes", ["mixed", "vec"])
+def test_grouped_uneven_bit_exact(oracle, dev, cfg, graph, shapes):
     import torch
     from ftd_amd import ftdx
-    shapes = [(256, 320), (64, 64), (130, 1000), (512, 256), (33, 17), (1, 5), (300, 64),
-              (128, 4096), (200, 2048), (96, 100)]
-    probs, refs = _grouped_case(oracle, dev, shapes, 7)
+    probs, refs = _grouped_case(oracle, dev, GROUPED_SHAPES[shapes], 7)
     ctx = ftdx.CutlassComplexGemm()
     plan = ftdx.GroupedPlan(ctx, probs, tile_config=cfg, use_graph=graph)
     info = plan.info()
