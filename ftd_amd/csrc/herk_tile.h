@@ -671,13 +671,14 @@ struct TileRunner {
 
     // ---- Interior tiles of the fused QC path ----
     // Every row and column of the tile exists, K is a whole number of stages and the
-    // product runs over one K segment.  The loads then need no bounds checks, so each
-    // stage is one basic block -- global loads for a later stage, this stage's
-    // ds_read + MFMA, the next stage's unpack + ds_write -- which the scheduler can
-    // interleave.  A diagonal tile loads its column block as well (the same rows
-    // as part of its row block) to keep the body uniform; the MFMAs read the row copy.
+    // product runs over one or two K segments (a polarization pair: a stage never
+    // straddles them).  The loads then need no bounds checks, so each stage is one
+    // basic block -- global loads for a later stage, this stage's ds_read + MFMA, the
+    // next stage's unpack + ds_write -- which the scheduler can interleave.  A diagonal
+    // tile loads its column block as well (the same rows as part of its row block) to
+    // keep the body uniform; the MFMAs read the row copy.
     static __device__ __forceinline__ bool full_tile(const TileJob& j) {
-        return j.nseg <= 1 && j.K > 0 && j.K % Cfg::KC == 0 && j.row0 + Cfg::TM <= j.N &&
+        return j.nseg >= 1 && j.nseg <= 2 && j.K > 0 && j.K % Cfg::KC == 0 && j.row0 + Cfg::TM <= j.N &&
                j.col0 + Cfg::TN <= j.N && (int64_t)j.N * j.K <= 0xFFFFFFFFll;
     }
 
@@ -727,8 +728,14 @@ struct TileRunner {
         uint32_t off[NG];
         full_offsets(j, tid, off);
         // stage s's first sample; past the end the last stage is loaded again (in
-        // bounds, never stored where it is read)
-        auto src = [&](int s) { return j.A + (int64_t)(s < nst ? s : nst - 1) * Cfg::KC; };
+        // bounds, never stored where it is read).  Stages from sps on are in the second
+        // K segment, seg_stride bytes after the first (one segment: never reached).
+        const int sps = j.K / Cfg::KC;
+        const int64_t jump = j.seg_stride - (int64_t)j.K;
+        auto src = [&](int s) {
+            const int t = s < nst ? s : nst - 1;
+            return j.A + (int64_t)t * Cfg::KC + (t >= sps ? jump : 0);
+        };
         if constexpr (DEPTH == 2) {
             uint4 ga[NG], gb[NG];
             full_load(src(0), off, ga);
