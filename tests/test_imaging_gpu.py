@@ -57,6 +57,30 @@ def test_fused_herk_pol_reduce_bit_exact(oracle, dev, N, K, n_ch, n_pol, n_ti):
 
 
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("n_pol", [2, 3])
+def test_fused_pol_reduce_interior_tiles(oracle, dev, cfg, n_pol):
+    """Every tile interior (N a multiple of the tile rows, K a whole number of stages, three
+    stages per segment on the 64-sample configurations): two segments run the branch-free
+    interior loop, three the bounds-checked one; both bit-exact against the oracle."""
+    import torch
+    from ftd_amd import ftdx
+    N, n_ch, n_ti = 512, 2, 2
+    K = 384 if cfg == 1 else 192
+    B = n_ch * n_pol * n_ti
+    qc = oracle.gen_qc_uniform(40 + cfg, N * K * B)
+    A = torch.from_numpy(qc).to(dev)
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, 1)
+    print(f"herk_pol_reduce interior N={N} K={K} pol={n_pol} cfg={cfg}", flush=True)
+    fused = ftdx.herk_pol_reduce(g, A, N, K, n_ch, n_pol, n_ti)
+    torch.cuda.synchronize()
+    per = oracle.herk_int64(qc, N, K, B).reshape(n_ch, n_pol, n_ti, -1, 2).sum(axis=1)
+    ref = per.reshape(n_ch * n_ti, -1, 2).astype(np.float32)
+    assert np.array_equal(_bits(fused.cpu().numpy()), _bits(ref))
+    g.close()
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
 def test_fused_pol_reduce_every_tile_config_fp16_alpha_beta(oracle, dev, cfg):
     import torch
     from ftd_amd import ftdx
@@ -126,7 +150,8 @@ def _grid_ref(oracle, qc, N, K, n_ch, n_pol, uv, freqs, Ng, cell, freq_offset, a
     return oracle.pillbox_grid(vis, uv, freqs, N * (N + 1) // 2, Ng, n_ch, cell, freq_offset=freq_offset)
 
 
-GRID_CASES = [(96, 80, 3, 2, 0), (130, 64, 2, 2, 1), (64, 37, 2, 3, 2), (257, 128, 1, 1, 0)]
+GRID_CASES = [(96, 80, 3, 2, 0), (130, 64, 2, 2, 1), (64, 37, 2, 3, 2), (257, 128, 1, 1, 0),
+              (256, 192, 2, 2, 4)]
 
 
 @pytest.mark.parametrize("N,K,n_ch,n_pol,cfg", GRID_CASES)
