@@ -607,14 +607,16 @@ int ftdx_init_herk(ftdx_context* ctx, int N, int K, int batch_count, void* strea
     if (N < 0 || K < 0 || batch_count < 0)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_init_herk: negative size");
     // The fused path needs no scratch.  The unfused (ForceBaseline) and FP16-input
-    // paths stage pre-cast FP8 rows: allocate them now so that later calls (and
-    // graph capture) allocate nothing (cutlass_gemm_api.h:601-617).
+    // paths stage pre-cast FP8 rows, one chunk of channels at a time: allocate that
+    // now so that later calls (and graph capture) allocate nothing
+    // (cutlass_gemm_api.h:601-617).
     ctx->init_N = N;
     ctx->init_K = K;
     ctx->init_batch = batch_count;
     ftdx::ensure_tune_loaded(ctx);
     if (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE || ctx->graph) {
-        const size_t need = (size_t)N * batch_count * 2 * ftdx::precast_row_samples(K);
+        const size_t need = (size_t)N * ftdx::precast_chunk_cap(N, K, batch_count) * 2 *
+                            ftdx::precast_row_samples(K);
         int rc = ftdx::grow_precast(ctx, need, "ftdx_init_herk: scratch");
         if (rc != FTDX_OK) return rc;
     }
