@@ -211,3 +211,62 @@ def test_pol_reduce_and_grid_random(oracle, dev, case):
     vis[:, [i * (i + 1) // 2 + i for i in range(N)], 1] = 0.0
     ref = oracle.pillbox_grid(vis, uv, freqs, N * (N + 1) // 2, Ng, n_ch, cell)
     assert np.array_equal(_bits(grid.cpu().numpy()), _bits(ref))
+
+
+@pytest.mark.parametrize("case", range(CASES // 2))
+def test_grouped_random_aligned_list(oracle, dev, case):
+    """A random list whose problems all take 16-byte loads (K a multiple of 16, each A its own
+    allocation), so the plan runs the 16-byte-only grouped kernel at the tile's own prefetch
+    depth; N from 1 to 600 with some multiples of 256 (interior tiles), alpha / beta, replayed
+    twice; every problem bit-exact."""
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(15000 + case)
+    count = int(rng.integers(1, 13))
+    cfg = int(rng.integers(-1, 6))
+    probs, refs = [], []
+    for i in range(count):
+        N = int(rng.choice([256, 512])) if rng.random() < 0.3 else int(rng.integers(1, 601))
+        K = 16 * int(rng.integers(1, 65))
+        alpha = float(rng.choice([1.0, 0.5, -2.0]))
+        beta = float(rng.choice([0.0, 1.0]))
+        A_np = oracle.gen_bytes(15100 + 20 * case + i, N * K)
+        C0 = rng.integers(-40, 40, size=(N * (N + 1) // 2, 2)).astype(np.float32)
+        probs.append((torch.from_numpy(A_np).to(dev), torch.from_numpy(C0).to(dev), N, K, alpha, beta))
+        refs.append((C0, oracle.herk_f32(A_np, N, K, 1, alpha=alpha, beta=beta, C=C0[None])[0]))
+    print(f"random aligned grouped case {case}: {count} problems, cfg={cfg}: "
+          + " ".join(f"{p[2]}x{p[3]}" for p in probs), flush=True)
+    ctx = ftdx.CutlassComplexGemm()
+    plan = ftdx.GroupedPlan(ctx, probs, tile_config=cfg, use_graph=bool(case % 2))
+    for rep in range(2):
+        for (A, C, N, K, _, _), (C0, _) in zip(probs, refs):
+            C.copy_(torch.from_numpy(C0))
+        plan.run()
+        torch.cuda.synchronize()
+        for (A, C, N, K, _, _), (_, ref) in zip(probs, refs):
+            assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (rep, N, K)
+    plan.close()
+
+
+@pytest.mark.parametrize("case", range(CASES // 4))
+def test_pol_reduce_interior_random(oracle, dev, case):
+    """The polarization-summed HERK with N a multiple of 128 and K a multiple of 64, so the
+    tiles of the 64-sample configurations are interior: one or two segments run the
+    branch-free loop, three the bounds-checked one.  Bit-exact against the oracle."""
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(16000 + case)
+    N = 128 * int(rng.integers(1, 5))
+    K = 64 * int(rng.integers(1, 7))
+    n_ch, n_pol, n_ti = int(rng.integers(1, 3)), int(rng.integers(1, 4)), int(rng.integers(1, 3))
+    cfg = int(rng.integers(0, 6))
+    qc = oracle.gen_qc_uniform(16100 + case, N * K * n_ch * n_pol * n_ti)
+    A = torch.from_numpy(qc).to(dev)
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(cfg, int(case % 2))
+    print(f"random interior pol case {case}: N={N} K={K} ch={n_ch} pol={n_pol} ti={n_ti} cfg={cfg}", flush=True)
+    fused = ftdx.herk_pol_reduce(g, A, N, K, n_ch, n_pol, n_ti)
+    torch.cuda.synchronize()
+    per = oracle.herk_int64(qc, N, K, n_ch * n_pol * n_ti).reshape(n_ch, n_pol, n_ti, -1, 2).sum(axis=1)
+    assert np.array_equal(_bits(fused.cpu().numpy()), _bits(per.reshape(n_ch * n_ti, -1, 2).astype(np.float32)))
+    g.close()
