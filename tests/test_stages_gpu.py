@@ -242,3 +242,39 @@ def test_graph_replay_follows_launch_config(oracle, dev):
     # captures: (0,1) new, replay, (3,1) new, replay, (3,0) new, (0,1) replayed from the cache
     assert seen == [1, 1, 2, 2, 3, 3], seen
     g.end_herk()
+
+
+@pytest.mark.parametrize("mode", ["ForceBaseline", "ForceDirectBF6"])
+def test_graph_replay_of_chunked_precast(dev, mode):
+    """A captured call whose pre-cast path runs in several chunks (2048 inputs x 90 channels x
+    K=1024: 4 MB of rows per channel, at most 320 MB per chunk) replays the whole sequence
+    of conversion and tile launches; init_herk sizes the scratch for one chunk, so the
+    captured pointers stay valid.  Every replay equals the fused INT4 output."""
+    import torch
+    from ftd_amd import ftdx, selfcheck
+    N, K, B = 2048, 1024, 90
+    A = selfcheck.qc_uniform(N * K * B, 13, dev)
+    P = N * (N + 1) // 2
+    ref = torch.empty((B, P, 2), dtype=torch.float32, device=dev)
+    f = ftdx.CutlassComplexGemm()
+    f.set_launch_config(5, 1)
+    assert f.herk(A, ref, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                  ftdx.OutputPrecision.FP32, tune=False) == 0
+    f.close()
+    g = ftdx.CutlassComplexGemm()
+    g.set_herk_mode(getattr(ftdx.HerkMode, mode.replace("BF6", "")))
+    g.set_launch_config(5 | (ftdx.TILE_BF6 if mode.endswith("BF6") else 0), 1)
+    g.set_direct_herk_graph(True)
+    g.init_herk(N, K, B)
+    C = torch.empty_like(ref)
+    for rep in range(3):
+        C.fill_(float("nan"))
+        torch.cuda.synchronize()
+        assert g.herk(A, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                      ftdx.OutputPrecision.FP32, tune=False) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(C.view(torch.int32), ref.view(torch.int32)), (mode, rep)
+    cached, captures = g.graph_stats()
+    assert captures == 1, (cached, captures)
+    g.end_herk()
+    g.close()
