@@ -708,7 +708,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "channels shard across GPUs with no exchange, one process per GPU", file=sys.stderr)
         return 2
 

@@ -1,5 +1,5 @@
 """Build recipe: compiles libftdx.so / libftdx_api.so (hipcc, gfx950) in place,
-the C++ API test driver, and the CPU oracle used by the tests."""
+the C++ API test program, and the CPU oracle used by the tests."""
 from __future__ import annotations
 
 import os

@@ -15,7 +15,7 @@
 //   d_out[b]       reused after D2H of payload p-2 completed      (compute stream waits)
 //   staging_out[b] read by the host after D2H of its payload       (host waits)
 // The host copies between caller memory (pageable) and the pinned staging are the
-// driver's bottleneck (one thread copies at 10-15 GB/s, the DMA engines move several
+// pipeline's bottleneck (one thread copies at 10-15 GB/s, the DMA engines move several
 // times that), so large copies are split over up to 8 threads.  A caller buffer that is
 // itself page-locked (hipHostMalloc, torch pin_memory) skips its staging copy: the DMA
 // engines read or write it directly.

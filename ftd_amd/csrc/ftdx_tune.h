@@ -26,7 +26,7 @@ class TuneCache {
 public:
     bool find(const TuneKey& k, TuneEntry& e) const;
     void put(const TuneKey& k, const TuneEntry& e);
-    // Loads entries of this build's fingerprint; malformed or foreign lines are
+    // Loads entries of this library's fingerprint; malformed or foreign lines are
     // skipped and counted.  Returns the number of entries read (or -1: no file).
     int load(const std::string& path, int* skipped = nullptr);
     bool save(const std::string& path) const;

@@ -1,7 +1,7 @@
 /*
  * herk_oracle.c -- CPU oracle for the X-engine HERK.  TEST INFRASTRUCTURE ONLY:
- * loaded by tests/, the smoke check and bench.py's CPU-baseline leg as the
- * checker; never linked into or called by the product library.
+ * loaded by tests/, the smoke check and bench.py's CPU-baseline leg to check
+ * results; never linked into or called by the product library.
  *
  * Restates, in plain C:
  *   encode_int4           cutlass_interface/test_herk_int4.cu:39-45

@@ -1,7 +1,7 @@
 /*
  * herk_oracle.h -- CPU oracle for the X-engine HERK (test infrastructure only).
  *
- * This library is the checker: only tests, the smoke check and the CPU-baseline
+ * This library checks results: only tests, the smoke check and the CPU-baseline
  * leg of bench.py load it.  The product path (libftdx.so) never links or calls it.
  *
  * Conventions restated from the reference (cutlass_gemm_api side):

@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (liboracle_herk.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
-CPU-baseline leg of ``bench.py`` as the checker.  The product path
+CPU-baseline leg of ``bench.py``, to check results.  The product path
 (``ftd_amd``, ``libftdx.so``) never imports this module.
 
 The C restatement lives in ``herk_oracle.c``; each function there cites the

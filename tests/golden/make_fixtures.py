@@ -4,7 +4,7 @@ Each case's output comes from the C oracle and is cross-checked here, before it
 is written, against an independent numpy complex128 product A A^H (lower
 triangle, Im(diag) = 0).  The reference ships no golden vectors
 (its tests regenerate data from seeds), so these fixtures are the pinned
-vectors of this build.  Regenerate with:  python tests/golden/make_fixtures.py
+vectors of this library.  Regenerate with:  python tests/golden/make_fixtures.py
 """
 import hashlib
 import os
