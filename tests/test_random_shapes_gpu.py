@@ -9,12 +9,15 @@ values, and every tile configuration forced in turn.
 - MX FP6 / FP4 on FP16 voltages, both element roundings: within the MFMA's accumulation of
   the FP64 HERK of the oracle's dequantized values.
 """
+import os
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-CASES = 32
+# seeds per test family; FTDX_RANDOM_CASES widens the sweep (160: 972 cases)
+CASES = int(os.environ.get("FTDX_RANDOM_CASES", "32"))
 
 
 def _draw(seed):
