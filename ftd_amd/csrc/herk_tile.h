@@ -788,12 +788,17 @@ struct TileRunner {
 
     // Column rows: HERK reads them from A (pitch lda), the GEMM layouts from B (pitch ldb,
     // Ncol rows).  Every load instruction covers 1024 / DROW whole rows of one block.
-    template <bool GEMM>
+    // Edge tiles (rows or columns past the matrix) take the direct loads too: dma_offsets
+    // reads the block's last row in place of each missing one, and those rows feed only
+    // outputs that every epilogue leaves unstored (rows >= N, columns >= Ncol or j > i).
+    // EDGES = false keeps to interior tiles (dense E2M1 rows read their scales per block).
+    template <bool GEMM, bool EDGES = true>
     static __device__ __forceinline__ bool dma_tile(const TileJob& j) {
         const uint8_t* B = GEMM ? j.B : j.A;
         const int64_t ldb = GEMM ? j.ldb : j.lda;
         const int ncol = GEMM ? j.Ncol : j.N;
-        return j.K > 0 && j.K % Cfg::KC == 0 && j.row0 + Cfg::TM <= j.N && j.col0 + Cfg::TN <= ncol &&
+        const bool inside = j.row0 + Cfg::TM <= j.N && j.col0 + Cfg::TN <= ncol;
+        return j.K > 0 && j.K % Cfg::KC == 0 && (EDGES || inside) && j.N > 0 && ncol > 0 &&
                ((reinterpret_cast<uintptr_t>(j.A) | reinterpret_cast<uintptr_t>(B)) & 15u) == 0 &&
                j.lda % 16 == 0 && ldb % 16 == 0 && (int64_t)j.N * j.lda <= 0xFFFFFFFFll &&
                (int64_t)ncol * ldb <= 0xFFFFFFFFll;
@@ -803,18 +808,22 @@ struct TileRunner {
         return (q * NWAVES + wave) * (1024 / DROW) >= Cfg::TM;
     }
 
-    // Global byte offset (inside its block's rows, stage 0) of each load of this lane.
+    // Global byte offset (inside its block's rows, stage 0) of each load of this lane; a
+    // row past the matrix (edge tiles) reads the last row instead.
     template <bool GEMM>
     static __device__ __forceinline__ void dma_offsets(const TileJob& j, int lane, int wave,
                                                        uint32_t (&off)[DOPS]) {
         const int64_t ldb = GEMM ? j.ldb : j.lda;
+        const int ncol = GEMM ? j.Ncol : j.N;
 #pragma unroll
         for (int q = 0; q < DOPS; ++q) {
             const int byte = (q * NWAVES + wave) * 1024 + lane * 16;  // position in the stage
             const int row = byte / DROW, pc = (byte % DROW) / 16;
             const int c = pc ^ swz(row);                               // logical chunk held there
-            off[q] = row < Cfg::TM ? (uint32_t)(j.row0 + row) * (uint32_t)j.lda + (uint32_t)(c * 16)
-                                   : (uint32_t)(j.col0 + row - Cfg::TM) * (uint32_t)ldb + (uint32_t)(c * 16);
+            const int ra = j.row0 + row < j.N ? j.row0 + row : j.N - 1;
+            const int rb = j.col0 + row - Cfg::TM < ncol ? j.col0 + row - Cfg::TM : ncol - 1;
+            off[q] = row < Cfg::TM ? (uint32_t)ra * (uint32_t)j.lda + (uint32_t)(c * 16)
+                                   : (uint32_t)rb * (uint32_t)ldb + (uint32_t)(c * 16);
         }
     }
 
@@ -1196,7 +1205,7 @@ struct TileRunner {
                 looped = true;
             }
         } else if constexpr (PRE && DMA_OK) {
-            if (dma_tile<GEMM>(j)) {
+            if (dma_tile<GEMM, MODE != LOAD_MX4D>(j)) {
                 const int a_row = wm * Cfg::WM + (lane & 31);
                 const int b_row = (share_cols ? (j.col0 - j.row0) : Cfg::TM) + wn * Cfg::WN + (lane & 31);
                 if (active)
