@@ -669,27 +669,31 @@ struct TileRunner {
         }
     }
 
-    // ---- Interior tiles of the fused QC path ----
-    // Every row and column of the tile exists, K is a whole number of stages and the
-    // product runs over one or two K segments (a polarization pair: a stage never
-    // straddles them).  The loads then need no bounds checks, so each stage is one
-    // basic block -- global loads for a later stage, this stage's ds_read + MFMA, the
-    // next stage's unpack + ds_write -- which the scheduler can interleave.  A diagonal
-    // tile loads its column block as well (the same rows as part of its row block) to
-    // keep the body uniform; the MFMAs read the row copy.
+    // ---- Branch-free tiles of the fused QC path ----
+    // K is a whole number of stages and the product runs over one or two K segments (a
+    // polarization pair: a stage never straddles them).  The loads then need no bounds
+    // checks, so each stage is one basic block -- global loads for a later stage, this
+    // stage's ds_read + MFMA, the next stage's unpack + ds_write -- which the scheduler
+    // can interleave.  Edge tiles (rows past N) run it too: full_offsets reads row N - 1
+    // in place of each missing row, whose products land only in elements (i >= N or
+    // j > i) that no epilogue stores.  A diagonal tile loads its column block as well (the
+    // same rows as part of its row block) to keep the body uniform; the MFMAs read the
+    // row copy.
     static __device__ __forceinline__ bool full_tile(const TileJob& j) {
-        return j.nseg >= 1 && j.nseg <= 2 && j.K > 0 && j.K % Cfg::KC == 0 && j.row0 + Cfg::TM <= j.N &&
-               j.col0 + Cfg::TN <= j.N && (int64_t)j.N * j.K <= 0xFFFFFFFFll;
+        return j.nseg >= 1 && j.nseg <= 2 && j.K > 0 && j.K % Cfg::KC == 0 && j.N > 0 &&
+               (int64_t)j.N * j.K <= 0xFFFFFFFFll;
     }
 
-    // Byte offset of each 16-byte segment of this thread inside one problem's rows.
+    // Byte offset of each 16-byte segment of this thread inside one problem's rows (a row
+    // past N reads row N - 1).
     static __device__ __forceinline__ void full_offsets(const TileJob& j, int tid,
                                                         uint32_t (&off)[Cfg::SEGS_PER_THREAD]) {
 #pragma unroll
         for (int q = 0; q < Cfg::SEGS_PER_THREAD; ++q) {
             int lrow, seg;
             seg_coords(q, tid, lrow, seg);
-            const int grow = lrow >= Cfg::TM ? j.col0 + (lrow - Cfg::TM) : j.row0 + lrow;
+            const int want = lrow >= Cfg::TM ? j.col0 + (lrow - Cfg::TM) : j.row0 + lrow;
+            const int grow = want < j.N ? want : j.N - 1;
             off[q] = (uint32_t)grow * (uint32_t)j.K + (uint32_t)(seg * 16);
         }
     }
