@@ -50,7 +50,10 @@ int cfg_value(int cfg, F&& f, int fallback) {
     return v;
 }
 
-template <class Cfg, typename OutT, int MODE, int FMT = OUT_LOWER_PACKED>
+// TAILS: K is not a whole number of stages (16-byte loads, one K segment); those launches
+// get the branch-free loop's variant for a partial last stage, built apart so the other
+// kernels' register allocation does not carry it.
+template <class Cfg, typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool TAILS = false>
 __global__ void __launch_bounds__(Cfg::THREADS, Cfg::WAVES_PER_EU)
 herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[Cfg::LDS_BYTES];
@@ -85,7 +88,7 @@ herk_int4_fp8_plain_kernel(HerkPlainArgs p, int tiles, int xcd_remap) {
         j.Ng = p.Ng;
         j.cell = p.cell;
     }
-    TileRunner<Cfg>::template run<OutT, MODE, FMT>(j, lds);
+    TileRunner<Cfg>::template run<OutT, MODE, FMT, false, TAILS>(j, lds);
 }
 
 // VEC_ONLY: every problem of the plan takes 16-byte loads, so only that load path is
@@ -180,15 +183,22 @@ hipError_t launch_plain_cfg(const HerkPlainArgs& a, bool out_fp16, int xcd_remap
                                stream, a, tiles, xcd_remap);
         return hipGetLastError();
     }
+    const bool tails = vec && a.nseg == 1 && a.K % Cfg::KC != 0;
     if (out_fp16) {
-        if (vec)
+        if (tails)
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_VEC16, OUT_LOWER_PACKED, true>), grid,
+                               block, 0, stream, a, tiles, xcd_remap);
+        else if (vec)
             hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_VEC16>), grid, block,
                                0, stream, a, tiles, xcd_remap);
         else
             hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, _Float16, LOAD_BYTES>), grid, block,
                                0, stream, a, tiles, xcd_remap);
     } else {
-        if (vec)
+        if (tails)
+            hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_VEC16, OUT_LOWER_PACKED, true>), grid,
+                               block, 0, stream, a, tiles, xcd_remap);
+        else if (vec)
             hipLaunchKernelGGL((herk_int4_fp8_plain_kernel<Cfg, float, LOAD_VEC16>), grid, block, 0,
                                stream, a, tiles, xcd_remap);
         else

@@ -671,7 +671,8 @@ struct TileRunner {
 
     // ---- Branch-free tiles of the fused QC path ----
     // K is a whole number of stages and the product runs over one or two K segments (a
-    // polarization pair: a stage never straddles them).  The loads then need no bounds
+    // polarization pair: a stage never straddles them), or one segment has a partial last
+    // stage (K a multiple of 16, full_tile_loop's TAIL).  The loads then need no bounds
     // checks, so each stage is one basic block -- global loads for a later stage, this
     // stage's ds_read + MFMA, the next stage's unpack + ds_write -- which the scheduler
     // can interleave.  Edge tiles (rows past N) run it too: full_offsets reads row N - 1
@@ -679,9 +680,12 @@ struct TileRunner {
     // j > i) that no epilogue stores.  A diagonal tile loads its column block as well (the
     // same rows as part of its row block) to keep the body uniform; the MFMAs read the
     // row copy.
+    // TAILS: the launch's K is not a whole number of stages (kernels built for that case)
+    template <bool TAILS>
     static __device__ __forceinline__ bool full_tile(const TileJob& j) {
-        return j.nseg >= 1 && j.nseg <= 2 && j.K > 0 && j.K % Cfg::KC == 0 && j.N > 0 &&
-               (int64_t)j.N * j.K <= 0xFFFFFFFFll;
+        const bool stages = TAILS ? (j.nseg == 1 && j.K % 16 == 0 && j.K % Cfg::KC != 0)
+                                  : (j.nseg >= 1 && j.nseg <= 2 && j.K % Cfg::KC == 0);
+        return stages && j.K > 0 && j.N > 0 && (int64_t)j.N * j.K <= 0xFFFFFFFFll;
     }
 
     // Byte offset of each 16-byte segment of this thread inside one problem's rows (a row
@@ -723,7 +727,12 @@ struct TileRunner {
         }
     }
 
-    template <int FMT, bool ACTIVE, int DEPTH>
+    // TAIL (one K segment, K a multiple of 16 but not of KC): the last stage holds
+    // K - (nst - 1) * KC samples.  Its 16-byte segments past K load the same row's first
+    // segment of that stage instead (in bounds) and are zeroed before the unpack, so they
+    // add nothing; both choices are selects on a uniform condition, and every stage stays
+    // one basic block.
+    template <int FMT, bool ACTIVE, int DEPTH, bool TAIL>
     static __device__ __forceinline__ void full_tile_loop(const TileJob& j, uint8_t* __restrict__ lds,
                                                           int tid, int nst, int a_off, int b_off,
                                                           v16f (&acc_re)[Cfg::FM][Cfg::FN],
@@ -732,41 +741,78 @@ struct TileRunner {
         uint32_t off[NG];
         full_offsets(j, tid, off);
         // stage s's first sample; past the end the last stage is loaded again (in
-        // bounds, never stored where it is read).  Stages from sps on are in the second
-        // K segment, seg_stride bytes after the first (one segment: never reached).
-        const int sps = j.K / Cfg::KC;
+        // bounds, never stored where it is read).  With two K segments, stages from sps
+        // on are in the second one, seg_stride bytes after the first.
+        const int sps = j.nseg > 1 ? j.K / Cfg::KC : nst;
         const int64_t jump = j.seg_stride - (int64_t)j.K;
         auto src = [&](int s) {
             const int t = s < nst ? s : nst - 1;
             return j.A + (int64_t)t * Cfg::KC + (t >= sps ? jump : 0);
         };
+        // A thread's segments all sit at the same sample of their rows (THREADS is a
+        // multiple of the segments per row): (tid % SEGS_PER_ROW) * 16.  In the last stage
+        // a segment past K moves back by that much and is zeroed.
+        static_assert(Cfg::THREADS % Cfg::SEGS_PER_ROW == 0, "segment position must not depend on q");
+        uint32_t tback = 0u, tkeep = 0xFFFFFFFFu;
+        if constexpr (TAIL) {
+            const int kt = j.K - (nst - 1) * Cfg::KC;  // samples of the last stage
+            const int seg = tid % Cfg::SEGS_PER_ROW;
+            const bool v = seg * 16 < kt;
+            tback = v ? 0u : (uint32_t)(seg * 16);
+            tkeep = v ? 0xFFFFFFFFu : 0u;
+        }
+        auto load = [&](int s, uint4 (&g)[NG]) {
+            if constexpr (TAIL) {
+                const uint32_t back = s >= nst - 1 ? tback : 0u;
+                uint32_t o[NG];
+#pragma unroll
+                for (int q = 0; q < NG; ++q) o[q] = off[q] - back;
+                full_load(src(s), o, g);
+            } else {
+                full_load(src(s), off, g);
+            }
+        };
+        // unpack + write stage s (held in g)
+        auto store = [&](uint8_t* st, uint4 (&g)[NG], int s) {
+            if constexpr (TAIL) {
+                const uint32_t m = s >= nst - 1 ? tkeep : 0xFFFFFFFFu;
+#pragma unroll
+                for (int q = 0; q < NG; ++q) {
+                    g[q].x &= m;
+                    g[q].y &= m;
+                    g[q].z &= m;
+                    g[q].w &= m;
+                }
+            }
+            full_store(st, tid, g);
+        };
         if constexpr (DEPTH == 2) {
             uint4 ga[NG], gb[NG];
-            full_load(src(0), off, ga);
-            full_store(lds, tid, ga);
-            full_load(src(1), off, ga);
+            load(0, ga);
+            store(lds, ga, 0);
+            load(1, ga);
             __syncthreads();
             for (int s = 0; s < nst; s += 2) {
-                full_load(src(s + 2), off, gb);
+                load(s + 2, gb);
                 if constexpr (ACTIVE) mfma_stage<FMT>(lds, a_off, b_off, acc_re, acc_im);
-                full_store(lds + Cfg::STAGE, tid, ga);
+                store(lds + Cfg::STAGE, ga, s + 1);
                 __syncthreads();
                 if (s + 1 >= nst) break;
-                full_load(src(s + 3), off, ga);
+                load(s + 3, ga);
                 if constexpr (ACTIVE) mfma_stage<FMT>(lds + Cfg::STAGE, a_off, b_off, acc_re, acc_im);
-                full_store(lds, tid, gb);
+                store(lds, gb, s + 2);
                 __syncthreads();
             }
         } else {
             uint4 g[NG];
-            full_load(src(0), off, g);
-            full_store(lds, tid, g);
+            load(0, g);
+            store(lds, g, 0);
             __syncthreads();
             for (int s = 0; s < nst; ++s) {
-                full_load(src(s + 1), off, g);
+                load(s + 1, g);
                 if constexpr (ACTIVE)
                     mfma_stage<FMT>(lds + (s & 1) * Cfg::STAGE, a_off, b_off, acc_re, acc_im);
-                full_store(lds + ((s + 1) & 1) * Cfg::STAGE, tid, g);
+                store(lds + ((s + 1) & 1) * Cfg::STAGE, g, s + 1);
                 __syncthreads();
             }
         }
@@ -1124,7 +1170,7 @@ struct TileRunner {
 
     // SHALLOW forces prefetch depth 1 (the persistent grouped kernel keeps more state
     // live and would spill the second register set).
-    template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false>
+    template <typename OutT, int MODE, int FMT = OUT_LOWER_PACKED, bool SHALLOW = false, bool TAILS = false>
     static __device__ __forceinline__ void run(const TileJob& j, uint8_t* __restrict__ lds) {
         constexpr bool GEMM = gemm_layout(FMT);
         static_assert(!GEMM || MODE == LOAD_FP8 || MODE == LOAD_MX6 || MODE == LOAD_MX4,
@@ -1190,11 +1236,11 @@ struct TileRunner {
         constexpr int DEPTH = (MODE == LOAD_VEC16 && !SHALLOW) ? Cfg::DEPTH : 1;
         bool looped = false;
         if constexpr (MODE == LOAD_VEC16 && !GEMM) {
-            if (full_tile(j)) {
+            if (full_tile<TAILS>(j)) {
                 if (active)
-                    full_tile_loop<FMT, true, DEPTH>(j, lds, tid, nst, a_off, b_off, acc_re, acc_im);
+                    full_tile_loop<FMT, true, DEPTH, TAILS>(j, lds, tid, nst, a_off, b_off, acc_re, acc_im);
                 else
-                    full_tile_loop<FMT, false, DEPTH>(j, lds, tid, nst, a_off, b_off, acc_re, acc_im);
+                    full_tile_loop<FMT, false, DEPTH, TAILS>(j, lds, tid, nst, a_off, b_off, acc_re, acc_im);
                 looped = true;
             }
         }

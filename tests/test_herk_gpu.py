@@ -289,3 +289,31 @@ def test_precast_paths_in_chunks(dev):
     del re, im
     assert torch.equal(run(X16, F16).view(torch.int32), ref.view(torch.int32))
     ctx.close()
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("K", [80, 112, 160, 1040])
+def test_partial_last_stage(oracle, dev, cfg, K):
+    """K a multiple of 16 but not of the stage (16, 32 or 48 samples left over; 16 to 112 on
+    the 128-sample configuration): the branch-free loop's partial-last-stage variant, on
+    edge tiles (N = 300), FP32 with alpha / beta and FP16, bit-exact against the oracle."""
+    import torch
+    from ftd_amd import ftdx
+    N, B = 300, 2
+    A_np = oracle.gen_bytes(3000 + 7 * cfg + K, N * K * B)
+    A = torch.from_numpy(A_np).to(dev)
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_launch_config(cfg, cfg & 1)
+    P = N * (N + 1) // 2
+    C0 = np.random.default_rng(K + cfg).integers(-30, 30, (B, P, 2)).astype(np.float32)
+    C = torch.from_numpy(C0).to(dev)
+    assert ctx.herk(A, C, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                    ftdx.OutputPrecision.FP32, alpha=0.5, beta=2.0, tune=False) == 0
+    torch.cuda.synchronize()
+    _assert_bits_equal(C.cpu().numpy(), oracle.herk_f32(A_np, N, K, B, alpha=0.5, beta=2.0, C=C0))
+    C16 = torch.empty((B, P, 2), dtype=torch.float16, device=dev)
+    assert ctx.herk(A, C16, N, K, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                    ftdx.OutputPrecision.FP16, tune=False) == 0
+    torch.cuda.synchronize()
+    _assert_bits_equal(C16.cpu().numpy(), oracle.herk_f16(A_np, N, K, B))
+    ctx.close()
