@@ -181,14 +181,22 @@ int device_cu_count() {
 // workgroup slots in their last rounds (plus a small cost per launch) wins.
 constexpr size_t kPrecastChunkBytes = size_t(320) << 20;
 
-int precast_chunk_cap(int N, int K, int batch) {
-    const size_t per = (size_t)N * 2 * precast_row_samples(K);
+// batches of `per` scratch bytes each that fit one chunk
+int chunk_cap_bytes(size_t per, int batch) {
     const size_t cap = per > 0 ? kPrecastChunkBytes / per : (size_t)batch;
     return (int)std::max<size_t>(1, std::min<size_t>(cap, (size_t)std::max(batch, 1)));
 }
 
-int precast_chunk(int cfg, int N, int K, int batch) {
-    const int cmax = precast_chunk_cap(N, K, batch);
+int precast_chunk_cap(int N, int K, int batch) {
+    return chunk_cap_bytes((size_t)N * 2 * precast_row_samples(K), batch);
+}
+
+// padded QC rows (K rounded up to 16 bytes) of the fused path with unaligned rows
+int padded_chunk_cap(int N, int K, int batch) {
+    return chunk_cap_bytes((size_t)N * precast_row_samples(K), batch);
+}
+
+int pick_chunk(int cfg, int N, int batch, int cmax) {
     if (cmax >= batch) return std::max(batch, 1);
     const int64_t slots = (int64_t)std::max(device_cu_count(), 1) * tile_config_wg_per_cu(cfg);
     const int64_t tiles = herk_tiles(cfg, N);
@@ -215,7 +223,7 @@ int precast_chunk(int cfg, int N, int K, int batch) {
 hipError_t run_precast(ftdx_context* ctx, const HerkPlainArgs& a, int input, bool bf6, int cfg, bool fp16,
                        int remap, hipStream_t st) {
     const int Kp = precast_row_samples(a.K);
-    const int chunk = precast_chunk(cfg, a.N, a.K, a.batch);
+    const int chunk = pick_chunk(cfg, a.N, a.batch, precast_chunk_cap(a.N, a.K, a.batch));
     const int64_t in_bytes = (int64_t)a.N * a.K * (input == FTDX_INPUT_FP16 ? 4 : 1);
     const int64_t out_bytes = a.c_bstride * (fp16 ? 2 : 4);
     for (int b0 = 0; b0 < a.batch; b0 += chunk) {
@@ -234,6 +242,29 @@ hipError_t run_precast(ftdx_context* ctx, const HerkPlainArgs& a, int input, boo
         b.precast = bf6 ? 2 : 1;
         b.lda = (int64_t)2 * Kp;
         b.a_bstride = (int64_t)a.N * 2 * Kp;
+        if ((e = launch_herk_int4_plain(b, cfg, fp16, remap, st)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// The fused path for QC rows that 16-byte loads cannot take (K not a multiple of 16, or
+// A not 16-byte aligned): per chunk of batches, copy the rows zero-padded to K rounded up
+// to 16 into the context's scratch (zero samples add nothing), then run the fused tile on
+// them.  The scratch must hold N * padded_chunk_cap(...) such rows.
+hipError_t run_padded(ftdx_context* ctx, const HerkPlainArgs& a, int cfg, bool fp16, int remap, hipStream_t st) {
+    const int Kq = precast_row_samples(a.K);
+    const int chunk = pick_chunk(cfg, a.N, a.batch, padded_chunk_cap(a.N, a.K, a.batch));
+    const int64_t out_bytes = a.c_bstride * (fp16 ? 2 : 4);
+    for (int b0 = 0; b0 < a.batch; b0 += chunk) {
+        const int nb = std::min(chunk, a.batch - b0);
+        hipError_t e = launch_pad_qc_rows(a.A + (int64_t)b0 * a.a_bstride, ctx->precast, (int64_t)a.N * nb, a.K, st);
+        if (e != hipSuccess) return e;
+        HerkPlainArgs b = a;
+        b.A = ctx->precast;
+        b.C = static_cast<uint8_t*>(a.C) + (int64_t)b0 * out_bytes;
+        b.batch = nb;
+        b.K = Kq;
+        b.a_bstride = (int64_t)a.N * Kq;
         if ((e = launch_herk_int4_plain(b, cfg, fp16, remap, st)) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -606,17 +637,24 @@ int ftdx_init_herk(ftdx_context* ctx, int N, int K, int batch_count, void* strea
     if (!ctx) return fail(FTDX_ERR_INVALID_ARG, "ftdx_init_herk: NULL context");
     if (N < 0 || K < 0 || batch_count < 0)
         return fail(FTDX_ERR_INVALID_ARG, "ftdx_init_herk: negative size");
-    // The fused path needs no scratch.  The unfused (ForceBaseline) and FP16-input
-    // paths stage pre-cast FP8 rows, one chunk of channels at a time: allocate that
-    // now so that later calls (and graph capture) allocate nothing
-    // (cutlass_gemm_api.h:601-617).
+    // The fused path on 16-byte-aligned rows of K % 16 == 0 needs no scratch.  The
+    // unfused (ForceBaseline) and FP16-input paths stage pre-cast FP8 rows, and the fused
+    // path on rows of K % 16 != 0 zero-padded copies, one chunk of channels at a time:
+    // allocate that now so that later calls (and graph capture) allocate nothing
+    // (cutlass_gemm_api.h:601-617).  Rows at an address that is not 16-byte aligned are
+    // padded too; their scratch is allocated by the first such call unless graph mode
+    // or ForceBaseline already sized it.
     ctx->init_N = N;
     ctx->init_K = K;
     ctx->init_batch = batch_count;
     ftdx::ensure_tune_loaded(ctx);
-    if (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE || ctx->graph) {
-        const size_t need = (size_t)N * ftdx::precast_chunk_cap(N, K, batch_count) * 2 *
-                            ftdx::precast_row_samples(K);
+    const int Kp = ftdx::precast_row_samples(K);
+    size_t need = 0;
+    if (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE || ctx->graph)
+        need = (size_t)N * ftdx::precast_chunk_cap(N, K, batch_count) * 2 * Kp;
+    if (K % 16 != 0 || ctx->graph)
+        need = std::max(need, (size_t)N * ftdx::padded_chunk_cap(N, K, batch_count) * Kp);
+    if (need > 0) {
         int rc = ftdx::grow_precast(ctx, need, "ftdx_init_herk: scratch");
         if (rc != FTDX_OK) return rc;
     }
@@ -899,8 +937,11 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     // Pre-cast FP8 (unfused) path: FP16 input always; INT4 input under ForceBaseline.
     // E3M2 compute always converts first (its own tune-cache entries: vec = 3).
     const bool precast = !e3m2 && ((input == FTDX_INPUT_FP16) || (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE));
-    const int vec = e3m2 ? 3
-                  : precast ? 2 : ((K % 16 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15u) == 0));
+    // Fused path on QC rows that 16-byte loads cannot take (K % 16 != 0 or A unaligned):
+    // the rows are copied zero-padded first (tune-cache entries: vec = 4).
+    const bool aligned16 = (K % 16 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15u) == 0);
+    const bool padded = !e3m2 && !precast && !aligned16;
+    const int vec = e3m2 ? 3 : precast ? 2 : padded ? 4 : 1;
     ftdx::TuneKey key{N, K, batch_count, fp16 ? 1 : 0, vec};
     int cfg, remap;
     bool cached;
@@ -916,9 +957,24 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
         int rc = ftdx::grow_precast(ctx, rows_need, "ftdx_herk: scratch");
         if (rc != FTDX_OK) return rc;
     }
+    const size_t pad_need = (size_t)N * ftdx::padded_chunk_cap(N, K, batch_count) * Kp;
+    if (padded) {
+        int rc = ftdx::grow_precast(ctx, pad_need, "ftdx_herk: scratch");
+        if (rc != FTDX_OK) return rc;
+    }
     if (tune && !cached && !explicit_cfg && !precast) {
         ftdx::TuneEntry best;
-        if (ftdx::autotune(ctx, a, fp16, s, key, best, e3m2 ? ftdx::TILE_BF6 : 0) == FTDX_OK) {
+        ftdx::HerkPlainArgs t = a;
+        if (padded) {
+            // time the tiles on the first chunk of padded rows
+            t.batch = ftdx::padded_chunk_cap(N, K, batch_count);
+            if ((e = ftdx::launch_pad_qc_rows(a.A, ctx->precast, (int64_t)N * t.batch, K, s)) != hipSuccess)
+                return hip_fail(e, "ftdx_herk: pad rows");
+            t.A = ctx->precast;
+            t.K = Kp;
+            t.a_bstride = (int64_t)N * Kp;
+        }
+        if (ftdx::autotune(ctx, t, fp16, s, key, best, e3m2 ? ftdx::TILE_BF6 : 0) == FTDX_OK) {
             cfg = best.cfg;
             if (ctx->forced_remap < 0) remap = best.xcd_remap;
         }
@@ -933,12 +989,13 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     }
     auto run = [&](hipStream_t st) -> hipError_t {
         if (precast || bf6) return ftdx::run_precast(ctx, a, input, bf6, cfg, fp16, remap, st);
+        if (padded) return ftdx::run_padded(ctx, a, cfg, fp16, remap, st);
         return ftdx::launch_herk_int4_plain(a, cfg, fp16, remap, st);
     };
     // Graph replay of identical calls (cutlass_gemm_api.cu:1738-1807): capture the
     // launch sequence once per (shape, scalars, pointers), then replay it.
     if (ctx->graph && ctx->herk_mode != FTDX_HERK_AUTO) {
-        const int mode = bf6 ? 2 : precast ? 1 : 0;
+        const int mode = bf6 ? 2 : precast ? 1 : padded ? 3 : 0;
         ftdx_context::GraphEntry* hit = nullptr;
         for (auto& gph : ctx->graphs)
             if (gph.used && gph.N == N && gph.K == K && gph.batch == batch_count &&

@@ -8,6 +8,8 @@ namespace ftdx {
 // Samples per pre-cast operand row (K rounded up to 16); row pitch = 2 * that bytes.
 int precast_row_samples(int K);
 
+// QC rows [rows][K] (any alignment) -> zero-padded rows [rows][K rounded up to 16].
+hipError_t launch_pad_qc_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K, hipStream_t s);
 hipError_t launch_promote_qc_fp8_interleaved(const uint8_t* in, uint8_t* out, int64_t n,
                                              hipStream_t s);
 hipError_t launch_promote_qc_fp8_planar(const uint8_t* in, uint8_t* out, int64_t rows, int K,

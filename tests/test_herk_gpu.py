@@ -291,6 +291,62 @@ def test_precast_paths_in_chunks(dev):
     ctx.close()
 
 
+def test_padded_rows_in_chunks(dev):
+    """QC rows that 16-byte loads cannot take (K % 16 != 0, or A not 16-byte aligned) are
+    copied zero-padded to K rounded up to 16, at most 320 MB of rows per chunk, and the
+    fused tile runs on the copy.  2048 inputs x 170 channels x K=1000 (2 MB of padded rows
+    per channel) runs in several chunks.  Zero samples add nothing, so the output bytes
+    must equal those of the same rows padded on the host and run at K=1008 (16-byte loads,
+    no copy); the weighted checksums and exact rows check that output.  The same holds at
+    an unaligned address, and for three replays of a captured call after init_herk."""
+    import torch
+    from ftd_amd import ftdx, selfcheck
+    N, K, B = 2048, 1000, 170
+    Kq = 1008
+    A = selfcheck.qc_uniform(N * K * B, 17, dev)
+    Ap = torch.zeros((B, N, Kq), dtype=torch.uint8, device=dev)
+    Ap[:, :, :K] = A.view(B, N, K)
+    I4, F8, O32 = ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8, ftdx.OutputPrecision.FP32
+    P = N * (N + 1) // 2
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_launch_config(5, 1)
+
+    def run(c, X, k):
+        C = torch.empty((B, P, 2), dtype=torch.float32, device=dev)
+        assert c.herk(X, C, N, k, B, I4, F8, O32, tune=False) == 0
+        torch.cuda.synchronize()
+        return C
+
+    ref = run(ctx, Ap.view(-1), Kq)
+    got = run(ctx, A, K)
+    assert torch.equal(got.view(torch.int32), ref.view(torch.int32))
+    ok = selfcheck.check_herk(torch, A, got, N, K, B, rows=[0, 1, 1023, 2047], row_channels=[0, 154, 155, 169])
+    assert ok["ok"], ok
+    del got
+    buf = torch.zeros(Ap.numel() + 64, dtype=torch.uint8, device=dev)
+    Au = buf[1:1 + Ap.numel()]
+    Au.copy_(Ap.view(-1))
+    assert torch.equal(run(ctx, Au, Kq).view(torch.int32), ref.view(torch.int32))
+    del buf, Au
+    ctx.close()
+    g = ftdx.CutlassComplexGemm()
+    g.set_herk_mode(ftdx.HerkMode.ForceDirect)
+    g.set_launch_config(5, 1)
+    g.set_direct_herk_graph(True)
+    g.init_herk(N, K, B)
+    C = torch.empty_like(ref)
+    for rep in range(3):
+        C.fill_(float("nan"))
+        torch.cuda.synchronize()
+        assert g.herk(A, C, N, K, B, I4, F8, O32, tune=False) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(C.view(torch.int32), ref.view(torch.int32)), rep
+    cached, captures = g.graph_stats()
+    assert captures == 1, (cached, captures)
+    g.end_herk()
+    g.close()
+
+
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("K", [80, 112, 160, 1040])
 def test_partial_last_stage(oracle, dev, cfg, K):
