@@ -460,6 +460,51 @@ def stage_study(torch, ftdx, dev, N: int, K: int, B: int, seed: int, reps: int, 
     return res
 
 
+def ragged_k_study(torch, ftdx, dev, N: int, K: int, B: int, seed: int, reps: int, cfg: int, remap: int) -> dict:
+    """HERK on rows that 16-byte loads cannot take: K - 8 and K - 1 samples per row (the
+    library copies the rows zero-padded to K rounded up to 16, per chunk of channels, then
+    runs the fused tile).  Each is timed against the same rows padded beforehand and run
+    at the padded K (no copy), and their output bytes are compared."""
+    res = {"inputs": N, "channels": B, "tile_config": cfg, "cases": []}
+    ctx = ftdx.CutlassComplexGemm()
+    ctx.set_strategy_tune_verbosity(0)
+    ctx.set_launch_config(cfg, remap)
+    P = N * (N + 1) // 2
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    for Kr in (K - 8, K - 1):
+        Kq = (Kr + 15) // 16 * 16
+        A = synth_qc(torch, N * Kr * B, seed, dev)
+        Ap = torch.zeros((B, N, Kq), dtype=torch.uint8, device=dev)
+        Ap[:, :, :Kr] = A.view(B, N, Kr)
+        C = torch.empty((B, P, 2), dtype=torch.float32, device=dev)
+        R = torch.empty_like(C)
+
+        def run(X, k, out):
+            return ctx.herk(X, out, N, k, B, ftdx.InputPrecision.INT4, ftdx.ComputePrecision.FP8,
+                            ftdx.OutputPrecision.FP32, tune=False)
+
+        ms = timed(lambda: run(A, Kr, C))
+        ms_pre = timed(lambda: run(Ap, Kq, R))
+        res["cases"].append({"K": Kr, "ms": round(ms, 4), "K_padded": Kq, "prepadded_ms": round(ms_pre, 4),
+                             "ratio": round(ms / ms_pre, 3),
+                             "identical_output": bool(torch.equal(C.view(torch.int32), R.view(torch.int32)))})
+        del A, Ap, C, R
+        torch.cuda.empty_cache()
+    ctx.close()
+    return res
+
+
 def pol_reduce_study(torch, ftdx, dev, N: int, K: int, n_ch: int, n_pol: int, n_ti: int, seed: int,
                      reps: int) -> dict:
     """HERK + polarization sum: one fused pass (ftdx_herk_pol_reduce) against the HERK of
@@ -699,6 +744,7 @@ def main() -> int:
                     help="also time the voltage beamformer pipeline (a caller outside the correlator path)")
     ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--no-bf6", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true")
     ap.add_argument("--batched-problems", type=int, default=512)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed steps, one plain device copy of this many MiB "
@@ -856,6 +902,14 @@ def main() -> int:
             + f"; HERK fused {stages['herk_fused_ms']:.3f} ms vs unfused {stages['herk_unfused_ms']:.3f} ms")
         torch.cuda.empty_cache()
 
+    ragged = None
+    if not args.no_ragged:
+        log(f"bench: HERK at K={K - 8} and K={K - 1} (rows padded per chunk) vs pre-padded rows ...")
+        ragged = ragged_k_study(torch, ftdx, dev, N, K, B, args.seed, 5, cfg, remap)
+        log("bench: " + "; ".join(f"K={c['K']} {c['ms']:.3f} ms vs pre-padded K={c['K_padded']} "
+                                  f"{c['prepadded_ms']:.3f} ms, identical={c['identical_output']}"
+                                  for c in ragged["cases"]))
+
     volt = None
     if args.voltage:
         log("bench: voltage pipeline 1664 ant x 4000 beams x 128 ch, 256 times, n_tps 4 ...")
@@ -949,10 +1003,14 @@ def main() -> int:
         "herk_pol_grid_fusion": grid,
         "voltage_pipeline": volt,
         "stages_vs_hbm_peak": stages,
+        "ragged_k": ragged,
         "e3m2_operands": bf6,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if ragged and not all(c["identical_output"] for c in ragged["cases"]):
+        log("bench: ragged-K output differs from the pre-padded rows' output")
+        return 3
     if not parity["ok"] or (batched and not batched["grouped_parity"]["ok"]):
         log("bench: PARITY FAILURE in the timed output")
         return 3
