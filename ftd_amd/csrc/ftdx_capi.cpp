@@ -191,15 +191,13 @@ int precast_chunk_cap(int N, int K, int batch) {
     return chunk_cap_bytes((size_t)N * 2 * precast_row_samples(K), batch);
 }
 
-// padded QC rows (K rounded up to 16 bytes) of the fused path with unaligned rows
-int padded_chunk_cap(int N, int K, int batch) {
-    return chunk_cap_bytes((size_t)N * precast_row_samples(K), batch);
-}
 
-int pick_chunk(int cfg, int N, int batch, int cmax) {
+// units_out: output batches per unit of `batch` (a unit is one group of polarization
+// segments on the polarization-sum path)
+int pick_chunk(int cfg, int N, int batch, int cmax, int units_out = 1) {
     if (cmax >= batch) return std::max(batch, 1);
     const int64_t slots = (int64_t)std::max(device_cu_count(), 1) * tile_config_wg_per_cu(cfg);
-    const int64_t tiles = herk_tiles(cfg, N);
+    const int64_t tiles = herk_tiles(cfg, N) * units_out;
     double best = 1e30;
     int pick = cmax;
     for (int c = cmax; c >= 1; --c) {
@@ -250,22 +248,48 @@ hipError_t run_precast(ftdx_context* ctx, const HerkPlainArgs& a, int input, boo
 // The fused path for QC rows that 16-byte loads cannot take (K not a multiple of 16, or
 // A not 16-byte aligned): per chunk of batches, copy the rows zero-padded to K rounded up
 // to 16 into the context's scratch (zero samples add nothing), then run the fused tile on
-// them.  The scratch must hold N * padded_chunk_cap(...) such rows.
-hipError_t run_padded(ftdx_context* ctx, const HerkPlainArgs& a, int cfg, bool fp16, int remap, hipStream_t st) {
+// them.  With K segments (polarization sum) a chunk holds whole groups of nseg * seg_group
+// input batches, the seg_group output batches they sum into.  launch(b) runs one chunk;
+// out_elem: bytes per scalar of C.  The scratch must hold padded_scratch_bytes(a).
+int padded_chunk_units(const HerkPlainArgs& a) {
+    const int grp = std::max(a.seg_group, 1);
+    return chunk_cap_bytes((size_t)a.N * precast_row_samples(a.K) * a.nseg * grp, a.batch / grp);
+}
+
+size_t padded_scratch_bytes(const HerkPlainArgs& a) {
+    return (size_t)a.N * precast_row_samples(a.K) * a.nseg * std::max(a.seg_group, 1) * padded_chunk_units(a);
+}
+
+// QC rows the fused tile reads with 16-byte loads
+bool rows_take_vector_loads(const void* A, int K) {
+    return K % 16 == 0 && (reinterpret_cast<uintptr_t>(A) & 15u) == 0;
+}
+
+// the plain HERK's case: one input batch per output batch
+int padded_chunk_cap(int N, int K, int batch) {
+    return chunk_cap_bytes((size_t)N * precast_row_samples(K), batch);
+}
+
+template <class Launch>
+hipError_t run_padded(ftdx_context* ctx, const HerkPlainArgs& a, int cfg, int out_elem, hipStream_t st,
+                      Launch&& launch) {
     const int Kq = precast_row_samples(a.K);
-    const int chunk = pick_chunk(cfg, a.N, a.batch, padded_chunk_cap(a.N, a.K, a.batch));
-    const int64_t out_bytes = a.c_bstride * (fp16 ? 2 : 4);
-    for (int b0 = 0; b0 < a.batch; b0 += chunk) {
-        const int nb = std::min(chunk, a.batch - b0);
-        hipError_t e = launch_pad_qc_rows(a.A + (int64_t)b0 * a.a_bstride, ctx->precast, (int64_t)a.N * nb, a.K, st);
+    const int grp = std::max(a.seg_group, 1), in_per = a.nseg * grp;
+    const int units = a.batch / grp;
+    const int chunk = pick_chunk(cfg, a.N, units, padded_chunk_units(a), grp);
+    for (int u0 = 0; u0 < units; u0 += chunk) {
+        const int nu = std::min(chunk, units - u0);
+        hipError_t e = launch_pad_qc_rows(a.A + (int64_t)u0 * in_per * a.a_bstride, ctx->precast,
+                                          (int64_t)a.N * nu * in_per, a.K, st);
         if (e != hipSuccess) return e;
         HerkPlainArgs b = a;
         b.A = ctx->precast;
-        b.C = static_cast<uint8_t*>(a.C) + (int64_t)b0 * out_bytes;
-        b.batch = nb;
+        b.C = static_cast<uint8_t*>(a.C) + (int64_t)u0 * grp * a.c_bstride * out_elem;
+        b.batch = nu * grp;
         b.K = Kq;
         b.a_bstride = (int64_t)a.N * Kq;
-        if ((e = launch_herk_int4_plain(b, cfg, fp16, remap, st)) != hipSuccess) return e;
+        b.freq_offset = a.freq_offset + u0 * grp;
+        if ((e = launch(b)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -939,8 +963,7 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     const bool precast = !e3m2 && ((input == FTDX_INPUT_FP16) || (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE));
     // Fused path on QC rows that 16-byte loads cannot take (K % 16 != 0 or A unaligned):
     // the rows are copied zero-padded first (tune-cache entries: vec = 4).
-    const bool aligned16 = (K % 16 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15u) == 0);
-    const bool padded = !e3m2 && !precast && !aligned16;
+    const bool padded = !e3m2 && !precast && !ftdx::rows_take_vector_loads(A, K);
     const int vec = e3m2 ? 3 : precast ? 2 : padded ? 4 : 1;
     ftdx::TuneKey key{N, K, batch_count, fp16 ? 1 : 0, vec};
     int cfg, remap;
@@ -989,7 +1012,10 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
     }
     auto run = [&](hipStream_t st) -> hipError_t {
         if (precast || bf6) return ftdx::run_precast(ctx, a, input, bf6, cfg, fp16, remap, st);
-        if (padded) return ftdx::run_padded(ctx, a, cfg, fp16, remap, st);
+        if (padded)
+            return ftdx::run_padded(ctx, a, cfg, fp16 ? 2 : 4, st, [&](const ftdx::HerkPlainArgs& b) {
+                return ftdx::launch_herk_int4_plain(b, cfg, fp16, remap, st);
+            });
         return ftdx::launch_herk_int4_plain(a, cfg, fp16, remap, st);
     };
     // Graph replay of identical calls (cutlass_gemm_api.cu:1738-1807): capture the
@@ -1195,8 +1221,19 @@ int ftdx_herk_pol_reduce(ftdx_context* ctx, const void* A, void* C, int N, int K
                                                 ? ftdx::direct_tile_to_cfg(ctx->direct_tile)
                                                 : ftdx::kDefaultCfg);
     const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
-    hipError_t e = ftdx::launch_herk_int4_plain(a, cfg, output == FTDX_OUTPUT_FP16, remap,
-                                                static_cast<hipStream_t>(stream));
+    const bool fp16 = output == FTDX_OUTPUT_FP16;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (K > 0 && !ftdx::rows_take_vector_loads(A, K)) {
+        // rows padded per chunk of polarization groups, as ftdx_herk does
+        int rc = ftdx::grow_precast(ctx, ftdx::padded_scratch_bytes(a), "ftdx_herk_pol_reduce: scratch");
+        if (rc != FTDX_OK) return rc;
+        e = ftdx::run_padded(ctx, a, cfg, fp16 ? 2 : 4, s, [&](const ftdx::HerkPlainArgs& b) {
+            return ftdx::launch_herk_int4_plain(b, cfg, fp16, remap, s);
+        });
+    } else {
+        e = ftdx::launch_herk_int4_plain(a, cfg, fp16, remap, s);
+    }
     return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_herk_pol_reduce");
 }
 
@@ -1234,7 +1271,18 @@ int ftdx_herk_pol_grid(ftdx_context* ctx, const void* A, int N, int K, int n_cha
                                                 ? ftdx::direct_tile_to_cfg(ctx->direct_tile)
                                                 : ftdx::kDefaultCfg);
     const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
-    hipError_t e = ftdx::launch_herk_grid(a, cfg, remap, static_cast<hipStream_t>(stream));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (K > 0 && !ftdx::rows_take_vector_loads(A, K)) {
+        // rows padded per chunk of channels, as ftdx_herk does
+        int rc = ftdx::grow_precast(ctx, ftdx::padded_scratch_bytes(a), "ftdx_herk_pol_grid: scratch");
+        if (rc != FTDX_OK) return rc;
+        e = ftdx::run_padded(ctx, a, cfg, 4, s, [&](const ftdx::HerkPlainArgs& b) {
+            return ftdx::launch_herk_grid(b, cfg, remap, s);
+        });
+    } else {
+        e = ftdx::launch_herk_grid(a, cfg, remap, s);
+    }
     return e == hipSuccess ? FTDX_OK : hip_fail(e, "ftdx_herk_pol_grid");
 }
 

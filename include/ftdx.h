@@ -122,12 +122,20 @@ int ftdx_destroy(ftdx_context* ctx);
 
 /* Replaces CutlassComplexGemm::herk (cutlass_interface/cutlass_gemm_api.h:750-760;
  * cutlass_gemm_api.cu:1707-1878).  input/compute/output/format take the FTDX_*
- * values above; `tune` != 0 consults (and on a miss fills) the tile-config cache. */
+ * values above; `tune` != 0 consults (and on a miss fills) the tile-config cache.
+ * INT4 rows of K % 16 != 0, or at an address that is not 16-byte aligned, are copied
+ * zero-padded to K rounded up to 16 into the context's scratch (at most 320 MB per chunk
+ * of channels) before the fused tile runs on them; the FP16-input, ForceBaseline and E3M2
+ * paths stage converted rows there.  The scratch grows on first need unless
+ * ftdx_init_herk sized it. */
 int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch_count,
               int input, int compute, int output, int format, float alpha, float beta,
               void* stream, int tune);
 
-/* Replaces CutlassComplexGemm::init_herk / end_herk (cutlass_gemm_api.h:601-617). */
+/* Replaces CutlassComplexGemm::init_herk / end_herk (cutlass_gemm_api.h:601-617).
+ * init_herk allocates the scratch that calls of this shape need (staged rows for the
+ * pre-cast paths and graph mode, padded rows when K % 16 != 0), so that they allocate
+ * nothing later; end_herk releases it. */
 int ftdx_init_herk(ftdx_context* ctx, int N, int K, int batch_count, void* stream);
 int ftdx_end_herk(ftdx_context* ctx);
 
@@ -384,7 +392,9 @@ int ftdx_pol_reduce_triangle(const void* tri_in, void* tri_out, int N, int n_ch,
  * [n_ch][n_ti] packed lower triangles of sum_pol X X^H (= the HERK over the n_pol*K
  * samples of both polarizations), with the HERK's alpha / beta / Im(diag) = 0 rules.
  * Equal to ftdx_herk followed by ftdx_pol_reduce_triangle for integer voltages (exact
- * sums), without writing the per-polarization triangles. */
+ * sums), without writing the per-polarization triangles.  Rows of K % 16 != 0 or at an
+ * address that is not 16-byte aligned are first copied zero-padded into the context's
+ * scratch (grown on first need, as ftdx_herk does). */
 int ftdx_herk_pol_reduce(ftdx_context* ctx, const void* A, void* C, int N, int K, int n_ch, int n_pol,
                          int n_time_inner, int output, float alpha, float beta, void* stream);
 /* Pillbox UV gridding (PillboxGridScatter, include/kernels/visibility_bf.cuh:259-370;
@@ -406,7 +416,9 @@ int ftdx_pillbox_grid(const void* vis_tri, int vis_output, const float* baseline
  * ftdx_pillbox_grid at frequency freq_hz[freq_offset + f].  Accumulates: the caller
  * zeroes the grid.  Equal to ftdx_herk_pol_reduce (n_time_inner = 1) followed by
  * ftdx_pillbox_grid (Nf = n_chan) whenever the FP32 cell sums are exact (integer
- * visibilities, |partial sums| < 2^24); otherwise equal up to the order of the adds. */
+ * visibilities, |partial sums| < 2^24); otherwise equal up to the order of the adds.
+ * Rows of K % 16 != 0 or at an unaligned address are padded first, as in
+ * ftdx_herk_pol_reduce. */
 int ftdx_herk_pol_grid(ftdx_context* ctx, const void* A, int N, int K, int n_chan, int n_pol,
                        const float* baseline_uv_m, const double* freq_hz, float* uv_grid, int Ng,
                        int freq_offset, float cell_size_rad, float alpha, void* stream);

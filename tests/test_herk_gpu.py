@@ -83,7 +83,7 @@ def test_negative_zero_and_raw_bytes(oracle, dev):
 
 
 def test_unaligned_input_pointer(oracle, dev):
-    """A not 16-byte aligned: the byte-load path must give the same bytes."""
+    """A not 16-byte aligned: the rows are copied to aligned scratch first; same bytes."""
     N, K, B = 70, 64, 2
     A = oracle.gen_qc_uniform(5, N * K * B)
     _assert_bits_equal(_herk(dev, A, N, K, B, offset=3), oracle.herk_f32(A, N, K, B))

@@ -38,7 +38,7 @@ CASES = [(64, 128, 2, 2, 1), (96, 80, 3, 2, 2), (130, 50, 1, 3, 2), (33, 17, 2, 
 
 @pytest.mark.parametrize("N,K,n_ch,n_pol,n_ti", CASES)
 def test_fused_herk_pol_reduce_bit_exact(oracle, dev, N, K, n_ch, n_pol, n_ti):
-    """K % 16 == 0 (16-byte loads) and ragged K (byte loads) across segments."""
+    """K % 16 == 0 (16-byte loads) and ragged K (rows copied zero-padded) across segments."""
     import torch
     from ftd_amd import ftdx
     B = n_ch * n_pol * n_ti
@@ -209,3 +209,57 @@ def test_fused_herk_pol_grid_accumulates_and_validates(oracle, dev):
         ftdx.herk_pol_grid(g, A, N, K, n_ch, 0, uv, f, grid, Ng, cell)
     with pytest.raises(ftdx.FtdxError):
         ftdx.herk_pol_grid(g, A, N, K, n_ch, n_pol, None, f, grid, Ng, cell)
+
+
+def test_ragged_k_rows_padded_in_chunks(dev):
+    """Rows of K % 16 != 0 are copied zero-padded to K rounded up to 16, at most 320 MB per
+    chunk of whole polarization groups (channels), before the fused polarization-sum and
+    gridding HERKs.  Several chunks here: each output must match the same rows padded
+    beforehand and run at the padded K.  The polarization sum is compared bit for bit.  The
+    grid is compared to 1e-5 of its largest cell: its atomic order differs between runs, and
+    the central cells sum past 2^24 (7e7, where one FP32 step is 8), so their sums round
+    differently by some tens of steps.  A chunk gridded at the wrong frequencies or into the
+    wrong grids would differ by about 1e-2 of the largest cell."""
+    import torch
+    from ftd_amd import ftdx, selfcheck
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(5, 1)
+    # polarization sum: 2048 inputs x 40 channels x 2 pol x 2 time-inner, K=1000 (8 MB per channel)
+    N, K, Kq, n_ch, n_pol, n_ti = 2048, 1000, 1008, 40, 2, 2
+    B = n_ch * n_pol * n_ti
+    A = selfcheck.qc_uniform(N * K * B, 23, dev)
+    Ap = torch.zeros((B, N, Kq), dtype=torch.uint8, device=dev)
+    Ap[:, :, :K] = A.view(B, N, K)
+    print(f"pol_reduce ragged N={N} K={K} ch={n_ch}", flush=True)
+    got = ftdx.herk_pol_reduce(g, A, N, K, n_ch, n_pol, n_ti)
+    ref = ftdx.herk_pol_reduce(g, Ap.view(-1), N, Kq, n_ch, n_pol, n_ti)
+    torch.cuda.synchronize()
+    assert torch.equal(got.view(torch.int32), ref.view(torch.int32))
+    del A, Ap, got, ref
+    torch.cuda.empty_cache()
+    # gridding: 1024 inputs x 160 channels x 2 pol, K=1000 (2 MB per channel), values in {-1, 0, 1}
+    N, K, Kq, n_ch, n_pol, Ng, cell, fo = 1024, 1000, 1008, 160, 2, 64, 2e-3, 3
+    B = n_ch * n_pol
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(29)
+    re = torch.randint(-1, 2, (B * N * K,), generator=gen, device=dev, dtype=torch.int16)
+    im = torch.randint(-1, 2, (B * N * K,), generator=gen, device=dev, dtype=torch.int16)
+    A = ((re & 15) | ((im & 15) << 4)).to(torch.uint8)
+    del re, im
+    Ap = torch.zeros((B, N, Kq), dtype=torch.uint8, device=dev)
+    Ap[:, :, :K] = A.view(B, N, K)
+    rng = np.random.default_rng(31)
+    pos = rng.standard_normal((N, 2)) * 0.015
+    ii, jj = np.tril_indices(N)
+    uv = torch.from_numpy((pos[ii] - pos[jj]).astype(np.float32)).to(dev)
+    freqs = torch.from_numpy(np.linspace(0.8e9, 1.6e9, n_ch + fo)).to(dev)
+    print(f"pol_grid ragged N={N} K={K} ch={n_ch}", flush=True)
+    grid = torch.zeros((n_ch, Ng, Ng, 2), dtype=torch.float32, device=dev)
+    ftdx.herk_pol_grid(g, A, N, K, n_ch, n_pol, uv, freqs, grid, Ng, cell, freq_offset=fo)
+    gref = torch.zeros_like(grid)
+    ftdx.herk_pol_grid(g, Ap.view(-1), N, Kq, n_ch, n_pol, uv, freqs, gref, Ng, cell, freq_offset=fo)
+    torch.cuda.synchronize()
+    scale = gref.abs().max().item()
+    assert scale > 0
+    assert (grid - gref).abs().max().item() <= 1e-5 * scale
+    g.close()
