@@ -9,6 +9,12 @@
 // and assigned longest-first to the least-loaded persistent workgroup.  Ties go
 // to workgroups of the same XCD first, so items that share input rows run on
 // one XCD and meet in its L2.
+//
+// Problems whose rows 16-byte loads cannot take (K % 16 != 0, or A not 16-byte aligned)
+// are run on zero-padded copies of their rows (K rounded up to 16) in a buffer the plan
+// owns, refreshed by one pad launch before the HERK launch on every run (captured in the
+// same graph), so the HERK kernel is built with the 16-byte load path only.  If that
+// buffer cannot be allocated, the plan runs the kernel that holds both load paths.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +26,7 @@
 #include "ftdx_host.h"
 #include "ftdx_internal.h"
 #include "herk_fp8.h"
+#include "stages.h"
 
 struct ftdx_grouped_plan {
     int device = 0;
@@ -33,6 +40,10 @@ struct ftdx_grouped_plan {
     ftdx::GroupProblemDev* d_probs = nullptr;
     ftdx::GroupItemDev* d_items = nullptr;
     int* d_wg_begin = nullptr;
+    uint8_t* d_pad = nullptr;             // padded rows of the problems that need them
+    ftdx::PadRowsJob* d_pad_jobs = nullptr;
+    ftdx::PadBlock* d_pad_blocks = nullptr;
+    int pad_blocks = 0;
     bool use_graph = false;
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -60,6 +71,7 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
     if (rc != 0) return fail(FTDX_ERR_INVALID_ARG, "grouped: invalid problem list");
     std::vector<GroupProblemDev> probs(count);
     bool all_vec = true;
+    size_t pad_bytes = 0;
     for (int p = 0; p < count; ++p) {
         const ftdx_herk_problem& q = problems[p];
         GroupProblemDev& d = probs[p];
@@ -69,9 +81,45 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
         d.K = q.K;
         d.alpha = q.alpha;
         d.beta = q.beta;
-        d.vec = (q.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(q.A) & 15u) == 0);
+        // (N = 0 or K = 0: no loads at all)
+        d.vec = q.N <= 0 || q.K <= 0 || ((q.K % 16 == 0) && ((reinterpret_cast<uintptr_t>(q.A) & 15u) == 0));
         d.tiles = sch.tiles[p];
         all_vec = all_vec && d.vec;
+        if (!d.vec) pad_bytes += (size_t)q.N * precast_row_samples(q.K);
+    }
+    hipError_t e;
+    if (pad_bytes > 0 && hipMalloc(&P->d_pad, pad_bytes) == hipSuccess) {
+        // every problem that needs it reads its padded copy; the plan then takes 16-byte loads only
+        std::vector<PadRowsJob> jobs;
+        std::vector<PadBlock> blocks;
+        size_t off = 0;
+        for (int p = 0; p < count; ++p) {
+            GroupProblemDev& d = probs[p];
+            if (d.vec) continue;
+            const int Kq = precast_row_samples(d.K);
+            const int64_t segs = (int64_t)d.N * (Kq / 16);
+            for (int64_t g = 0; g < segs; g += kPadBlockSegments) blocks.push_back({(int)jobs.size(), g});
+            jobs.push_back({d.A, P->d_pad + off, (int64_t)d.N, d.K, Kq});
+            d.A = P->d_pad + off;
+            d.K = Kq;
+            d.vec = 1;
+            off += (size_t)d.N * Kq;
+        }
+        if ((e = hipMalloc(&P->d_pad_jobs, sizeof(PadRowsJob) * jobs.size())) != hipSuccess)
+            return hip_fail(e, "grouped plan: hipMalloc");
+        if ((e = hipMemcpy(P->d_pad_jobs, jobs.data(), sizeof(PadRowsJob) * jobs.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, "grouped plan: upload");
+        if (blocks.size() > 0x7FFFFFFF) return fail(FTDX_ERR_INVALID_ARG, "grouped plan: too many rows to pad");
+        if ((e = hipMalloc(&P->d_pad_blocks, sizeof(PadBlock) * blocks.size())) != hipSuccess)
+            return hip_fail(e, "grouped plan: hipMalloc");
+        if ((e = hipMemcpy(P->d_pad_blocks, blocks.data(), sizeof(PadBlock) * blocks.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, "grouped plan: upload");
+        P->pad_blocks = (int)blocks.size();
+        all_vec = true;
+    } else if (pad_bytes > 0) {
+        (void)hipGetLastError();  // out of memory for the copies: both load paths instead
     }
     static_assert(sizeof(GroupItemDev) == sizeof(ScheduleItem), "schedule items are uploaded as they are");
     const std::vector<ScheduleItem>& flat = sch.items;
@@ -81,7 +129,6 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
     P->max_load = sch.max_load;
     P->mean_load = sch.mean_load;
 
-    hipError_t e;
     if ((e = hipMalloc(&P->d_probs, sizeof(GroupProblemDev) * std::max(count, 1))) != hipSuccess)
         return hip_fail(e, "grouped plan: hipMalloc");
     if ((e = hipMalloc(&P->d_items, sizeof(GroupItemDev) * std::max<size_t>(flat.size(), 1))) !=
@@ -108,6 +155,10 @@ int grouped_build(ftdx_grouped_plan* P, const ftdx_herk_problem* problems, int c
 }
 
 hipError_t grouped_launch(ftdx_grouped_plan* P, hipStream_t s) {
+    if (P->pad_blocks > 0) {
+        hipError_t e = launch_pad_qc_rows_multi(P->d_pad_jobs, P->d_pad_blocks, P->pad_blocks, s);
+        if (e != hipSuccess) return e;
+    }
     return launch_herk_int4_grouped(P->d_probs, P->d_items, P->d_wg_begin, P->num_wg, P->cfg,
                                     P->out_fp16 != 0, P->all_vec, s);
 }
@@ -119,6 +170,13 @@ void grouped_release(ftdx_grouped_plan* P) {
     if (P->d_probs) (void)hipFree(P->d_probs);
     if (P->d_items) (void)hipFree(P->d_items);
     if (P->d_wg_begin) (void)hipFree(P->d_wg_begin);
+    if (P->d_pad) (void)hipFree(P->d_pad);
+    if (P->d_pad_jobs) (void)hipFree(P->d_pad_jobs);
+    if (P->d_pad_blocks) (void)hipFree(P->d_pad_blocks);
+    P->d_pad = nullptr;
+    P->d_pad_jobs = nullptr;
+    P->d_pad_blocks = nullptr;
+    P->pad_blocks = 0;
     P->exec = nullptr;
     P->cap_stream = nullptr;
     P->d_probs = nullptr;

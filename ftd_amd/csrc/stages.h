@@ -10,6 +10,25 @@ int precast_row_samples(int K);
 
 // QC rows [rows][K] (any alignment) -> zero-padded rows [rows][K rounded up to 16].
 hipError_t launch_pad_qc_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K, hipStream_t s);
+
+// One row block of a multi-block pad launch: src [rows][K] -> dst [rows][Kp] (Kp = K
+// rounded up to 16, dst 16-byte aligned).
+struct PadRowsJob {
+    const uint8_t* src;
+    uint8_t* dst;
+    int64_t rows;
+    int K, Kp;
+};
+// One workgroup of that launch: kPadBlockSegments 16-byte output segments of job `job`
+// from segment `seg` (fewer at the job's end).  A job of rows * Kp / 16 segments takes
+// ceil(that / kPadBlockSegments) workgroups.
+constexpr int kPadBlockSegments = 2048;
+struct PadBlock {
+    int job;
+    int64_t seg;
+};
+// jobs, blocks: device arrays; one workgroup per entry of blocks (nblocks < 2^31).
+hipError_t launch_pad_qc_rows_multi(const PadRowsJob* jobs, const PadBlock* blocks, int nblocks, hipStream_t s);
 hipError_t launch_promote_qc_fp8_interleaved(const uint8_t* in, uint8_t* out, int64_t n,
                                              hipStream_t s);
 hipError_t launch_promote_qc_fp8_planar(const uint8_t* in, uint8_t* out, int64_t rows, int K,

@@ -8,8 +8,9 @@
 //
 //   promote_qc_fp8_interleaved   QC -> [fp8 Re, fp8 Im] bytes     (cutlass_gemm_api.cu:1566-1623)
 //   promote_qc_fp8_planar        QC -> pre-cast operand rows (dword-planar FP8)
-//   pad_qc_rows                  QC rows of any K and alignment -> zero-padded rows of K
-//                                rounded up to 16 (the fused HERK's 16-byte loads)
+//   pad_qc_rows[_multi]          QC rows of any K and alignment -> zero-padded rows of K
+//                                rounded up to 16 (the fused HERK's 16-byte loads); the
+//                                multi form copies many row blocks in one launch
 //   promote_fp16_fp8_planar      FP16 [re, im] -> pre-cast rows, clamp +-448 then
 //                                nearest-even E4M3 (cast_kernels_common.hpp:30-48)
 //   promote_qc_fp16_planar       QC -> FP16 Re plane + Im plane   (cutlass_gemm_api.cu:1639-1698)
@@ -640,50 +641,71 @@ __global__ void __launch_bounds__(ST_THREADS) interleave_fp32_kernel(const float
 
 // QC rows of K bytes at any alignment -> rows of Kp = K rounded up to 16 bytes, zero
 // padded, so the fused HERK can take its 16-byte loads (zero samples add nothing to the
-// product).  One thread per 16 output bytes: the (one or two) aligned 16-byte words that
-// hold its input bytes are loaded and funnel-shifted into place, and bytes past the row's
-// end are cleared.  Only words that lie wholly inside [in, in + rows * K) are loaded; a
-// segment whose words reach outside (the first or last few of the buffer) gathers bytes.
+// product).  One thread per 16 output bytes (segment t of rows * Kp / 16): the (one or
+// two) aligned 16-byte words that hold its input bytes are loaded and funnel-shifted into
+// place, and bytes past the row's end are cleared.  Only words that lie wholly inside
+// [in, in + rows * K) are loaded; a segment whose words reach outside (the first or last
+// few of the buffer) gathers bytes.
+__device__ __forceinline__ void pad_segment(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                            int64_t rows, int K, int Kp, int64_t t) {
+    const int segs = Kp / 16;
+    const uintptr_t lo = reinterpret_cast<uintptr_t>(in), hi = lo + (uintptr_t)(rows * K);
+    const int64_t r = t / segs;
+    const int k0 = (int)(t - r * segs) * 16;
+    const int v = K - k0 < 16 ? K - k0 : 16;  // input bytes of this segment, >= 1
+    const uintptr_t addr = lo + (uintptr_t)(r * K + k0);
+    const uint32_t off = (uint32_t)(addr & 15u);
+    const uintptr_t base = addr - off;
+    const bool two = off + (uint32_t)v > 16u;
+    uint32_t w[4];
+    if (base >= lo && base + (two ? 32u : 16u) <= hi) {
+        const uint4 q0 = *reinterpret_cast<const uint4*>(base);
+        const uint4 q1 = two ? *reinterpret_cast<const uint4*>(base + 16) : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t d[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+        const uint32_t sw = off >> 2, sb = 8u * (off & 3u);
+        uint32_t e[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)  // e[j] = d[j + sw], with constant register indices
+            e[j] = sw == 0 ? d[j] : sw == 1 ? d[j + 1] : sw == 2 ? d[j + 2] : d[j + 3];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint64_t pair = ((uint64_t)e[i + 1] << 32) | e[i];
+            const int lim = v - 4 * i;  // bytes of this dword inside the row
+            const uint32_t keep = lim >= 4 ? 0xFFFFFFFFu : lim <= 0 ? 0u : (1u << (8 * lim)) - 1u;
+            w[i] = (uint32_t)(pair >> sb) & keep;
+        }
+    } else {
+        const uint8_t* src = in + r * K + k0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[i] = 0u;
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+            if (b < v) w[b >> 2] |= (uint32_t)src[b] << (8 * (b & 3));
+    }
+    *reinterpret_cast<uint4*>(out + r * Kp + k0) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 __global__ void __launch_bounds__(ST_THREADS)
 pad_qc_rows_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, int64_t rows, int K, int Kp) {
-    const int segs = Kp / 16;
-    const int64_t total = rows * segs;
+    const int64_t total = rows * (Kp / 16);
     const int64_t stride = (int64_t)gridDim.x * ST_THREADS;
-    const uintptr_t lo = reinterpret_cast<uintptr_t>(in), hi = lo + (uintptr_t)(rows * K);
-    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride) {
-        const int64_t r = t / segs;
-        const int k0 = (int)(t - r * segs) * 16;
-        const int v = K - k0 < 16 ? K - k0 : 16;  // input bytes of this segment, >= 1
-        const uintptr_t addr = lo + (uintptr_t)(r * K + k0);
-        const uint32_t off = (uint32_t)(addr & 15u);
-        const uintptr_t base = addr - off;
-        const bool two = off + (uint32_t)v > 16u;
-        uint32_t w[4];
-        if (base >= lo && base + (two ? 32u : 16u) <= hi) {
-            const uint4 q0 = *reinterpret_cast<const uint4*>(base);
-            const uint4 q1 = two ? *reinterpret_cast<const uint4*>(base + 16) : make_uint4(0u, 0u, 0u, 0u);
-            const uint32_t d[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-            const uint32_t sw = off >> 2, sb = 8u * (off & 3u);
-            uint32_t e[5];
-#pragma unroll
-            for (int j = 0; j < 5; ++j)  // e[j] = d[j + sw], with constant register indices
-                e[j] = sw == 0 ? d[j] : sw == 1 ? d[j + 1] : sw == 2 ? d[j + 2] : d[j + 3];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint64_t pair = ((uint64_t)e[i + 1] << 32) | e[i];
-                const int lim = v - 4 * i;  // bytes of this dword inside the row
-                const uint32_t keep = lim >= 4 ? 0xFFFFFFFFu : lim <= 0 ? 0u : (1u << (8 * lim)) - 1u;
-                w[i] = (uint32_t)(pair >> sb) & keep;
-            }
-        } else {
-            const uint8_t* src = in + r * K + k0;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) w[i] = 0u;
-#pragma unroll
-            for (int b = 0; b < 16; ++b)
-                if (b < v) w[b >> 2] |= (uint32_t)src[b] << (8 * (b & 3));
-        }
-        *reinterpret_cast<uint4*>(out + r * Kp + k0) = make_uint4(w[0], w[1], w[2], w[3]);
+    for (int64_t t = (int64_t)blockIdx.x * ST_THREADS + threadIdx.x; t < total; t += stride)
+        pad_segment(in, out, rows, K, Kp, t);
+}
+
+// Several row blocks in one launch (the grouped HERK's problems): workgroup w copies
+// segments [blocks[w].seg, + kPadBlockSegments) of job blocks[w].job, consecutive threads
+// on consecutive segments.
+static_assert(kPadBlockSegments % ST_THREADS == 0, "whole segment rounds per workgroup");
+__global__ void __launch_bounds__(ST_THREADS)
+pad_qc_rows_multi_kernel(const PadRowsJob* __restrict__ jobs, const PadBlock* __restrict__ blocks) {
+    const PadBlock bk = blocks[blockIdx.x];
+    const PadRowsJob jb = jobs[bk.job];
+    const int64_t total = jb.rows * (jb.Kp / 16);
+#pragma unroll 2
+    for (int i = 0; i < kPadBlockSegments / ST_THREADS; ++i) {
+        const int64_t t = bk.seg + i * ST_THREADS + threadIdx.x;
+        if (t < total) pad_segment(jb.src, jb.dst, jb.rows, jb.K, jb.Kp, t);
     }
 }
 
@@ -720,6 +742,12 @@ hipError_t launch_interleave_fp32(const float* re, const float* im, float* out, 
 }
 
 int precast_row_samples(int K) { return (K + 15) / 16 * 16; }
+
+hipError_t launch_pad_qc_rows_multi(const PadRowsJob* jobs, const PadBlock* blocks, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pad_qc_rows_multi_kernel, dim3((unsigned)nblocks), dim3(ST_THREADS), 0, s, jobs, blocks);
+    return hipGetLastError();
+}
 
 hipError_t launch_pad_qc_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K, hipStream_t s) {
     if (rows <= 0 || K <= 0) return hipSuccess;
