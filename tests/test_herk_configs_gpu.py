@@ -130,6 +130,47 @@ def test_grouped_alpha_beta_and_unaligned(oracle, dev):
         assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (N, K, a, b)
 
 
+@pytest.mark.parametrize("tile_config", [3, 5])
+def test_grouped_ragged_rows_padded_by_plan(oracle, dev, tile_config):
+    """Problems whose rows 16-byte loads cannot take (K % 16 != 0 or A unaligned, here most
+    of them) run on zero-padded copies refreshed by the plan's pad launch: several pad
+    workgroups per problem (N * K up to 600 x 700), empty problems between them, and the
+    input changed between replays (the copy is taken on every run).  Bit-exact against the
+    oracle on every replay."""
+    import torch
+    from ftd_amd import ftdx
+    rng = np.random.default_rng(40 + tile_config)
+    shapes = [(int(rng.integers(1, 601)), int(rng.integers(1, 701))) for _ in range(14)]
+    shapes += [(0, 30), (50, 0), (600, 700), (257, 16), (33, 1)]
+    probs, host = [], []
+    for i, (N, K) in enumerate(shapes):
+        off = int(rng.integers(0, 16))
+        buf = torch.zeros(N * K + 32, dtype=torch.uint8, device=dev)
+        A = buf[off:off + N * K]
+        C = torch.full((max(N * (N + 1) // 2, 1), 2), float("nan"), dtype=torch.float32, device=dev)
+        probs.append((A, C, N, K))
+        host.append(None)
+    ctx = ftdx.CutlassComplexGemm()
+    plan = ftdx.GroupedPlan(ctx, probs, tile_config=tile_config, use_graph=True)
+    for rep in range(2):
+        for i, (A, C, N, K) in enumerate(probs):
+            host[i] = oracle.gen_qc_uniform(7000 + 31 * rep + i, N * K)
+            if N * K:
+                A.copy_(torch.from_numpy(host[i]))
+            C.fill_(float("nan"))
+        torch.cuda.synchronize()
+        print(f"grouped ragged cfg={tile_config} replay {rep}", flush=True)
+        plan.run()
+        torch.cuda.synchronize()
+        for (A, C, N, K), A_np in zip(probs, host):
+            if N == 0:
+                continue
+            ref = oracle.herk_f32(A_np, N, K, 1)[0]
+            assert np.array_equal(_bits(C.cpu().numpy()), _bits(ref)), (rep, N, K)
+    plan.close()
+    ctx.close()
+
+
 def test_grouped_matches_plain_loop_bytes(oracle, dev):
     """A reduced batched-uneven set: grouped output bytes == per-problem plain loop."""
     import torch
