@@ -183,7 +183,10 @@ int ftdx_herk_query_config(ftdx_context* ctx, int N, int K, int batch_count, int
  * tile_config = FTDX_TILE_AUTO times the configurations on the problems' own
  * buffers when every beta is 0 (C is overwritten), else uses the default.
  * All outputs share one precision.  The plan keeps the pointers: buffers must
- * outlive it. */
+ * outlive it.  Problems whose rows 16-byte loads cannot take (K % 16 != 0, or A
+ * not 16-byte aligned) run on zero-padded copies (K rounded up to 16) in device
+ * memory the plan owns until it is destroyed; every run refreshes them from A
+ * first, so A may change between runs. */
 int ftdx_herk_grouped_plan_create(ftdx_context* ctx, const ftdx_herk_problem* problems,
                                   int count, int output, int tile_config, int use_graph,
                                   ftdx_grouped_plan** plan);
