@@ -30,7 +30,7 @@ METRIC = "HERK TFLOP/s (4M-equiv) & % FP8 MFMA peak, 4096 inputs; batched vs pla
 # HBM bytes per launch of the headline kernel (4096 inputs, 96 channels, K = 2048, FP32
 # output), per tile configuration, from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the
 # factors a 4096 MiB device copy gives those counters); summary and commands under profiles/.
-MEASURED_TRAFFIC_BYTES = {5: 11.571e9, 4: 15.518e9}
+MEASURED_TRAFFIC_BYTES = {5: 11.631e9, 4: 15.518e9}
 
 
 def log(msg: str) -> None:
