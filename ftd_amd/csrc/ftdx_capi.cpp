@@ -191,7 +191,6 @@ int precast_chunk_cap(int N, int K, int batch) {
     return chunk_cap_bytes((size_t)N * 2 * precast_row_samples(K), batch);
 }
 
-
 // units_out: output batches per unit of `batch` (a unit is one group of polarization
 // segments on the polarization-sum path)
 int pick_chunk(int cfg, int N, int batch, int cmax, int units_out = 1) {
