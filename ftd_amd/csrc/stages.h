@@ -11,8 +11,8 @@ int precast_row_samples(int K);
 // QC rows [rows][K] (any alignment) -> zero-padded rows [rows][K rounded up to 16].
 hipError_t launch_pad_qc_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K, hipStream_t s);
 
-// One row block of a multi-block pad launch: src [rows][K] -> dst [rows][Kp] (Kp = K
-// rounded up to 16, dst 16-byte aligned).
+// One row block of a multi-block pad launch: src [rows][K] -> dst [rows][Kp] (Kp >= K, a
+// multiple of 16; dst 16-byte aligned).
 struct PadRowsJob {
     const uint8_t* src;
     uint8_t* dst;

@@ -639,7 +639,7 @@ __global__ void __launch_bounds__(ST_THREADS) interleave_fp32_kernel(const float
         out[i] = make_float2(re[i], im[i]);
 }
 
-// QC rows of K bytes at any alignment -> rows of Kp = K rounded up to 16 bytes, zero
+// QC rows of K bytes at any alignment -> rows of Kp >= K bytes (a multiple of 16), zero
 // padded, so the fused HERK can take its 16-byte loads (zero samples add nothing to the
 // product).  One thread per 16 output bytes (segment t of rows * Kp / 16): the (one or
 // two) aligned 16-byte words that hold its input bytes are loaded and funnel-shifted into
@@ -652,7 +652,11 @@ __device__ __forceinline__ void pad_segment(const uint8_t* __restrict__ in, uint
     const uintptr_t lo = reinterpret_cast<uintptr_t>(in), hi = lo + (uintptr_t)(rows * K);
     const int64_t r = t / segs;
     const int k0 = (int)(t - r * segs) * 16;
-    const int v = K - k0 < 16 ? K - k0 : 16;  // input bytes of this segment, >= 1
+    const int v = K - k0 < 16 ? K - k0 : 16;  // input bytes of this segment
+    if (v <= 0) {  // wholly past the row's end
+        *reinterpret_cast<uint4*>(out + r * Kp + k0) = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
     const uintptr_t addr = lo + (uintptr_t)(r * K + k0);
     const uint32_t off = (uint32_t)(addr & 15u);
     const uintptr_t base = addr - off;
