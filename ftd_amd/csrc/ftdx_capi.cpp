@@ -244,19 +244,34 @@ hipError_t run_precast(ftdx_context* ctx, const HerkPlainArgs& a, int input, boo
     return hipSuccess;
 }
 
+// Samples per padded row for tile configuration cfg: K rounded up to 16, or to the tile's
+// stage when that adds at most 1/16 more samples (the copy then runs the loop without a
+// partial last stage, which costs more than those zero samples).
+int padded_row_samples(int K, int cfg) {
+    const int k16 = precast_row_samples(K);
+    const int st = cfg == TILE_128x128_K128 ? 128 : 64;
+    const int kst = (K + st - 1) / st * st;
+    return kst - k16 <= k16 / 16 ? kst : k16;
+}
+
+// the longest padded row of any configuration (scratch sizing)
+int padded_row_cap(int K) {
+    return std::max(padded_row_samples(K, TILE_128x128_K64), padded_row_samples(K, TILE_128x128_K128));
+}
+
 // The fused path for QC rows that 16-byte loads cannot take (K not a multiple of 16, or
-// A not 16-byte aligned): per chunk of batches, copy the rows zero-padded to K rounded up
-// to 16 into the context's scratch (zero samples add nothing), then run the fused tile on
-// them.  With K segments (polarization sum) a chunk holds whole groups of nseg * seg_group
+// A not 16-byte aligned): per chunk of batches, copy the rows zero-padded to
+// padded_row_samples(K, cfg) into the context's scratch (zero samples add nothing), then
+// run the fused tile on them.  With K segments (polarization sum) a chunk holds whole groups of nseg * seg_group
 // input batches, the seg_group output batches they sum into.  launch(b) runs one chunk;
 // out_elem: bytes per scalar of C.  The scratch must hold padded_scratch_bytes(a).
 int padded_chunk_units(const HerkPlainArgs& a) {
     const int grp = std::max(a.seg_group, 1);
-    return chunk_cap_bytes((size_t)a.N * precast_row_samples(a.K) * a.nseg * grp, a.batch / grp);
+    return chunk_cap_bytes((size_t)a.N * padded_row_cap(a.K) * a.nseg * grp, a.batch / grp);
 }
 
 size_t padded_scratch_bytes(const HerkPlainArgs& a) {
-    return (size_t)a.N * precast_row_samples(a.K) * a.nseg * std::max(a.seg_group, 1) * padded_chunk_units(a);
+    return (size_t)a.N * padded_row_cap(a.K) * a.nseg * std::max(a.seg_group, 1) * padded_chunk_units(a);
 }
 
 // QC rows the fused tile reads with 16-byte loads
@@ -266,20 +281,20 @@ bool rows_take_vector_loads(const void* A, int K) {
 
 // the plain HERK's case: one input batch per output batch
 int padded_chunk_cap(int N, int K, int batch) {
-    return chunk_cap_bytes((size_t)N * precast_row_samples(K), batch);
+    return chunk_cap_bytes((size_t)N * padded_row_cap(K), batch);
 }
 
 template <class Launch>
 hipError_t run_padded(ftdx_context* ctx, const HerkPlainArgs& a, int cfg, int out_elem, hipStream_t st,
                       Launch&& launch) {
-    const int Kq = precast_row_samples(a.K);
+    const int Kq = padded_row_samples(a.K, cfg);
     const int grp = std::max(a.seg_group, 1), in_per = a.nseg * grp;
     const int units = a.batch / grp;
     const int chunk = pick_chunk(cfg, a.N, units, padded_chunk_units(a), grp);
     for (int u0 = 0; u0 < units; u0 += chunk) {
         const int nu = std::min(chunk, units - u0);
         hipError_t e = launch_pad_qc_rows(a.A + (int64_t)u0 * in_per * a.a_bstride, ctx->precast,
-                                          (int64_t)a.N * nu * in_per, a.K, st);
+                                          (int64_t)a.N * nu * in_per, a.K, Kq, st);
         if (e != hipSuccess) return e;
         HerkPlainArgs b = a;
         b.A = ctx->precast;
@@ -676,7 +691,7 @@ int ftdx_init_herk(ftdx_context* ctx, int N, int K, int batch_count, void* strea
     if (ctx->herk_mode == FTDX_HERK_FORCE_BASELINE || ctx->graph)
         need = (size_t)N * ftdx::precast_chunk_cap(N, K, batch_count) * 2 * Kp;
     if (K % 16 != 0 || ctx->graph)
-        need = std::max(need, (size_t)N * ftdx::padded_chunk_cap(N, K, batch_count) * Kp);
+        need = std::max(need, (size_t)N * ftdx::padded_chunk_cap(N, K, batch_count) * ftdx::padded_row_cap(K));
     if (need > 0) {
         int rc = ftdx::grow_precast(ctx, need, "ftdx_init_herk: scratch");
         if (rc != FTDX_OK) return rc;
@@ -979,7 +994,7 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
         int rc = ftdx::grow_precast(ctx, rows_need, "ftdx_herk: scratch");
         if (rc != FTDX_OK) return rc;
     }
-    const size_t pad_need = (size_t)N * ftdx::padded_chunk_cap(N, K, batch_count) * Kp;
+    const size_t pad_need = (size_t)N * ftdx::padded_chunk_cap(N, K, batch_count) * ftdx::padded_row_cap(K);
     if (padded) {
         int rc = ftdx::grow_precast(ctx, pad_need, "ftdx_herk: scratch");
         if (rc != FTDX_OK) return rc;
@@ -988,13 +1003,14 @@ int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch
         ftdx::TuneEntry best;
         ftdx::HerkPlainArgs t = a;
         if (padded) {
-            // time the tiles on the first chunk of padded rows
+            // time the tiles on the first chunk of padded rows (the 64-sample stages' padding)
+            const int Kt = ftdx::padded_row_samples(K, ftdx::TILE_128x128_K64);
             t.batch = ftdx::padded_chunk_cap(N, K, batch_count);
-            if ((e = ftdx::launch_pad_qc_rows(a.A, ctx->precast, (int64_t)N * t.batch, K, s)) != hipSuccess)
+            if ((e = ftdx::launch_pad_qc_rows(a.A, ctx->precast, (int64_t)N * t.batch, K, Kt, s)) != hipSuccess)
                 return hip_fail(e, "ftdx_herk: pad rows");
             t.A = ctx->precast;
-            t.K = Kp;
-            t.a_bstride = (int64_t)N * Kp;
+            t.K = Kt;
+            t.a_bstride = (int64_t)N * Kt;
         }
         if (ftdx::autotune(ctx, t, fp16, s, key, best, e3m2 ? ftdx::TILE_BF6 : 0) == FTDX_OK) {
             cfg = best.cfg;

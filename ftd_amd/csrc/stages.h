@@ -8,8 +8,9 @@ namespace ftdx {
 // Samples per pre-cast operand row (K rounded up to 16); row pitch = 2 * that bytes.
 int precast_row_samples(int K);
 
-// QC rows [rows][K] (any alignment) -> zero-padded rows [rows][K rounded up to 16].
-hipError_t launch_pad_qc_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K, hipStream_t s);
+// QC rows [rows][K] (any alignment) -> zero-padded rows [rows][Kp] (Kp >= K, a multiple of
+// 16; out 16-byte aligned).
+hipError_t launch_pad_qc_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K, int Kp, hipStream_t s);
 
 // One row block of a multi-block pad launch: src [rows][K] -> dst [rows][Kp] (Kp >= K, a
 // multiple of 16; dst 16-byte aligned).

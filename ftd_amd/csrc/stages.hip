@@ -753,9 +753,9 @@ hipError_t launch_pad_qc_rows_multi(const PadRowsJob* jobs, const PadBlock* bloc
     return hipGetLastError();
 }
 
-hipError_t launch_pad_qc_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K, hipStream_t s) {
+hipError_t launch_pad_qc_rows(const uint8_t* in, uint8_t* out, int64_t rows, int K, int Kp, hipStream_t s) {
     if (rows <= 0 || K <= 0) return hipSuccess;
-    const int Kp = (K + 15) / 16 * 16;
+    if (Kp < K || Kp % 16 != 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pad_qc_rows_kernel, dim3(grid_for(rows * (Kp / 16))), dim3(ST_THREADS), 0, s, in, out,
                        rows, K, Kp);
     return hipGetLastError();

@@ -124,8 +124,8 @@ int ftdx_destroy(ftdx_context* ctx);
  * cutlass_gemm_api.cu:1707-1878).  input/compute/output/format take the FTDX_*
  * values above; `tune` != 0 consults (and on a miss fills) the tile-config cache.
  * INT4 rows of K % 16 != 0, or at an address that is not 16-byte aligned, are copied
- * zero-padded to K rounded up to 16 into the context's scratch (at most 320 MB per chunk
- * of channels) before the fused tile runs on them; the FP16-input, ForceBaseline and E3M2
+ * zero-padded (K rounded up to 16, or to the tile's stage) into the context's scratch (at
+ * most 320 MB per chunk of channels) before the fused tile runs on them; the FP16-input, ForceBaseline and E3M2
  * paths stage converted rows there.  The scratch grows on first need unless
  * ftdx_init_herk sized it. */
 int ftdx_herk(ftdx_context* ctx, const void* A, void* C, int N, int K, int batch_count,

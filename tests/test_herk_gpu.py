@@ -293,11 +293,12 @@ def test_precast_paths_in_chunks(dev):
 
 def test_padded_rows_in_chunks(dev):
     """QC rows that 16-byte loads cannot take (K % 16 != 0, or A not 16-byte aligned) are
-    copied zero-padded to K rounded up to 16, at most 320 MB of rows per chunk, and the
-    fused tile runs on the copy.  2048 inputs x 170 channels x K=1000 (2 MB of padded rows
-    per channel) runs in several chunks.  Zero samples add nothing, so the output bytes
-    must equal those of the same rows padded on the host and run at K=1008 (16-byte loads,
-    no copy); the weighted checksums and exact rows check that output.  The same holds at
+    copied zero-padded (to 1024 samples here: the tile's stage, 1/64 more than K rounded up
+    to 16), at most 320 MB of rows per chunk, and the fused tile runs on the copy.  2048
+    inputs x 170 channels x K=1000 (2 MB of padded rows per channel) runs in several
+    chunks.  Zero samples add nothing, so the output bytes must equal those of the same
+    rows padded on the host and run at K=1008 (16-byte loads, no copy, a partial last
+    stage); the weighted checksums and exact rows check that output.  The same holds at
     an unaligned address, and for three replays of a captured call after init_herk."""
     import torch
     from ftd_amd import ftdx, selfcheck
