@@ -279,6 +279,17 @@ bool rows_take_vector_loads(const void* A, int K) {
     return K % 16 == 0 && (reinterpret_cast<uintptr_t>(A) & 15u) == 0;
 }
 
+// Polarization-sum launches: pad rows 16-byte loads cannot take, and also two K segments of
+// K % stage != 0 when padding to the stage is cheap -- the two-segment loop without bounds
+// checks needs whole stages, and the bounds-checked one is slower than the copy (2048
+// inputs x 16 channels x 2 x 2, K = 1008: 0.60 ms; K = 1000 padded to 1024: 0.50 ms).
+bool pol_rows_padded(const void* A, int K, int nseg, int cfg) {
+    if (K <= 0) return false;
+    if (!rows_take_vector_loads(A, K)) return true;
+    const int st = cfg == TILE_128x128_K128 ? 128 : 64;
+    return nseg == 2 && K % st != 0 && padded_row_samples(K, cfg) % st == 0;
+}
+
 // the plain HERK's case: one input batch per output batch
 int padded_chunk_cap(int N, int K, int batch) {
     return chunk_cap_bytes((size_t)N * padded_row_cap(K), batch);
@@ -1239,7 +1250,7 @@ int ftdx_herk_pol_reduce(ftdx_context* ctx, const void* A, void* C, int N, int K
     const bool fp16 = output == FTDX_OUTPUT_FP16;
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipError_t e;
-    if (K > 0 && !ftdx::rows_take_vector_loads(A, K)) {
+    if (ftdx::pol_rows_padded(A, K, n_pol, cfg)) {
         // rows padded per chunk of polarization groups, as ftdx_herk does
         int rc = ftdx::grow_precast(ctx, ftdx::padded_scratch_bytes(a), "ftdx_herk_pol_reduce: scratch");
         if (rc != FTDX_OK) return rc;
@@ -1288,7 +1299,7 @@ int ftdx_herk_pol_grid(ftdx_context* ctx, const void* A, int N, int K, int n_cha
     const int remap = ctx->forced_remap >= 0 ? ctx->forced_remap : ftdx::kDefaultRemap;
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipError_t e;
-    if (K > 0 && !ftdx::rows_take_vector_loads(A, K)) {
+    if (ftdx::pol_rows_padded(A, K, n_pol, cfg)) {
         // rows padded per chunk of channels, as ftdx_herk does
         int rc = ftdx::grow_precast(ctx, ftdx::padded_scratch_bytes(a), "ftdx_herk_pol_grid: scratch");
         if (rc != FTDX_OK) return rc;

@@ -263,3 +263,41 @@ def test_ragged_k_rows_padded_in_chunks(dev):
     assert scale > 0
     assert (grid - gref).abs().max().item() <= 1e-5 * scale
     g.close()
+
+
+@pytest.mark.parametrize("K", [1008, 496])
+def test_two_pol_segments_padded_to_stage(oracle, dev, K):
+    """Two polarization segments of K a multiple of 16 but not of the 64-sample stage are
+    copied zero-padded to the stage (1008 -> 1024, 496 -> 512) so they run the two-segment
+    loop without bounds checks; the polarization sum and the gridding stay bit-exact against
+    the oracle."""
+    import torch
+    from ftd_amd import ftdx
+    N, n_ch, n_pol, n_ti = 200, 2, 2, 2  # grid cell sums stay below 2^24: exact in any order
+    B = n_ch * n_pol * n_ti
+    qc = oracle.gen_qc_uniform(K + 5, N * K * B)
+    A = torch.from_numpy(qc).to(dev)
+    g = ftdx.CutlassComplexGemm()
+    g.set_launch_config(5, 1)
+    print(f"two-segment padded K={K}", flush=True)
+    fused = ftdx.herk_pol_reduce(g, A, N, K, n_ch, n_pol, n_ti)
+    torch.cuda.synchronize()
+    per = oracle.herk_int64(qc, N, K, B).reshape(n_ch, n_pol, n_ti, -1, 2).sum(axis=1)
+    ref = per.reshape(n_ch * n_ti, -1, 2).astype(np.float32)
+    assert np.array_equal(_bits(fused.cpu().numpy()), _bits(ref))
+    # gridding: channels of n_pol segments (n_time_inner = 1)
+    Ng, cell, alpha, fo = 64, 2e-3, 0.5, 0
+    n_chg = n_ch * n_ti
+    rng = np.random.default_rng(K)
+    pos = rng.standard_normal((N, 2)) * 0.015
+    ii, jj = np.tril_indices(N)
+    uv = (pos[ii] - pos[jj]).astype(np.float32)
+    freqs = np.array([0.9e9, 1.0e9, 1.1e9, 1.2e9])
+    grid = torch.zeros((n_chg, Ng, Ng, 2), dtype=torch.float32, device=dev)
+    ftdx.herk_pol_grid(g, A, N, K, n_chg, n_pol, torch.from_numpy(uv).to(dev), torch.from_numpy(freqs).to(dev),
+                       grid, Ng, cell, freq_offset=fo, alpha=alpha)
+    torch.cuda.synchronize()
+    gref = _grid_ref(oracle, qc, N, K, n_chg, n_pol, uv, freqs, Ng, cell, fo, alpha)
+    assert np.abs(gref).sum() > 0
+    assert np.array_equal(_bits(grid.cpu().numpy()), _bits(gref))
+    g.close()
